@@ -1,0 +1,844 @@
+// lym_oracle.cpp -- CPU FP64 restatement of LeonardYM's fermion-solve path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the correctness oracle and the
+// timed CPU baseline.  It is loaded only by tests/, by smoke() in
+// __graft_entry__.py and by the cpu_baseline leg of bench.py, always as the
+// checker, never as the thing measured or shipped.  The product path
+// (leonardym_amd/) never links, loads or calls it.
+//
+// Data at the boundary (identical to the reference's raw vectors):
+//   spinor  : V x [4 spin][3 colour] complex<double>, interleaved re/im
+//   links   : V x [4 mu][3x3 column-major] complex<double>, BC folded in
+//   F       : V x [6 planes][3x3 column-major] complex<double>
+//   inverse : V x [12x12 column-major] complex<double>
+// Site order is t-fastest lexicographic, idx = ((x*Ly + y)*Lz + z)*Lt + t
+// (source/MPILattice/LocalLayout.h:149-155); mu = 0,1,2,3 <-> x,y,z,t.
+//
+// Global sums accumulate in long double, as the reference does
+// (source/algebra_utils/AlgebraUtils.h:17-27,182-196).
+//
+// Three independent forms of the Wilson operator are kept so that each pins
+// the others: the unrolled form (DiracWilsonOperator.cpp:16-391), the
+// projector form (BasicDiracWilsonOperator.cpp:12-178) and a textbook form
+// built from explicit 4x4 gamma matrices (Gamma.cpp:9-10).  The clover term
+// likewise has the reference's chiral-block form
+// (ImprovedDiracWilsonOperator.cpp:200-243) and the textbook
+// Sheikholeslami-Wohlert form  -kappa*csw*sum_{mu<nu} sigma_{mu nu} Fhat_{mu nu}
+// with Fhat = -i F.
+//
+// Parity pinning (all in tests/test_oracle.py): free-field plane-wave closed
+// form, constant abelian field closed form, gauge covariance, gamma5
+// hermiticity, form A == form B == form C, csw = 0 -> Wilson, abelian
+// field-strength closed form, R * R^-1 = 1, Schur identity, solver solutions
+// against a dense LU solve of the same operator.
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef std::complex<double> cd;
+typedef long double ld;
+typedef std::complex<long double> cld;
+
+namespace {
+
+// ---------------------------------------------------------------- geometry
+// LocalLayout::initialize builds sup/down tables once
+// (source/MPILattice/LocalLayout.h:34-110); cached here per lattice shape.
+struct Geo {
+    int L[4];
+    long V;
+    std::vector<int> up, dn;  // [V][4]
+    std::vector<int> par;     // (x+y+z+t) % 2
+};
+
+std::mutex geo_mutex;
+std::map<std::vector<int>, Geo*> geo_cache;
+
+inline int wrap(int v, int m) { return v >= m ? v - m : (v < 0 ? v + m : v); }
+
+const Geo& geo(const int* d) {
+    std::vector<int> key(d, d + 4);
+    std::lock_guard<std::mutex> lock(geo_mutex);
+    auto it = geo_cache.find(key);
+    if (it != geo_cache.end()) return *it->second;
+    Geo* g = new Geo;
+    for (int i = 0; i < 4; ++i) g->L[i] = d[i];
+    g->V = (long)d[0] * d[1] * d[2] * d[3];
+    g->up.resize(g->V * 4);
+    g->dn.resize(g->V * 4);
+    g->par.resize(g->V);
+    for (int x = 0; x < d[0]; ++x)
+        for (int y = 0; y < d[1]; ++y)
+            for (int z = 0; z < d[2]; ++z)
+                for (int t = 0; t < d[3]; ++t) {
+                    int c[4] = {x, y, z, t};
+                    long s = ((long)(x * d[1] + y) * d[2] + z) * d[3] + t;
+                    for (int mu = 0; mu < 4; ++mu) {
+                        int cu[4] = {x, y, z, t}, cdn[4] = {x, y, z, t};
+                        cu[mu] = wrap(c[mu] + 1, d[mu]);
+                        cdn[mu] = wrap(c[mu] - 1, d[mu]);
+                        g->up[s * 4 + mu] = ((cu[0] * d[1] + cu[1]) * d[2] + cu[2]) * d[3] + cu[3];
+                        g->dn[s * 4 + mu] = ((cdn[0] * d[1] + cdn[1]) * d[2] + cdn[2]) * d[3] + cdn[3];
+                    }
+                    g->par[s] = (x + y + z + t) & 1;
+                }
+    geo_cache[key] = g;
+    return *g;
+}
+
+// ---------------------------------------------------------------- 3x3 algebra
+// column-major: element (i,j) at i + 3*j, as Eigen stores FermionicGroup.
+struct M3 {
+    cd a[9];
+    cd& operator()(int i, int j) { return a[i + 3 * j]; }
+    const cd& operator()(int i, int j) const { return a[i + 3 * j]; }
+};
+
+inline M3 load3(const cd* p) { M3 m; for (int k = 0; k < 9; ++k) m.a[k] = p[k]; return m; }
+inline M3 mul(const M3& A, const M3& B) {
+    M3 C;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            cd s = 0;
+            for (int k = 0; k < 3; ++k) s += A(i, k) * B(k, j);
+            C(i, j) = s;
+        }
+    return C;
+}
+inline M3 dag(const M3& A) {
+    M3 C;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) C(i, j) = std::conj(A(j, i));
+    return C;
+}
+inline M3 add(const M3& A, const M3& B) { M3 C; for (int k = 0; k < 9; ++k) C.a[k] = A.a[k] + B.a[k]; return C; }
+
+// U * v and U^dagger * v for a colour vector
+inline void mv(const cd* U, const cd* v, cd* o) {
+    for (int i = 0; i < 3; ++i) o[i] = U[i] * v[0] + U[i + 3] * v[1] + U[i + 6] * v[2];
+}
+inline void mdv(const cd* U, const cd* v, cd* o) {
+    for (int i = 0; i < 3; ++i)
+        o[i] = std::conj(U[3 * i]) * v[0] + std::conj(U[3 * i + 1]) * v[1] + std::conj(U[3 * i + 2]) * v[2];
+}
+
+// ---------------------------------------------------------------- gamma tables
+// Chiral basis of source/utils/Gamma.cpp:9-10 (rows listed).
+const cd I(0., 1.);
+cd GAMMA[4][4][4];
+cd SIGMA[6][4][4];  // sigma_{mu nu} = (i/2)[gamma_mu, gamma_nu], planes 01,02,03,12,13,23
+const int PLANE_MU[6] = {0, 0, 0, 1, 1, 2};
+const int PLANE_NU[6] = {1, 2, 3, 2, 3, 3};
+
+struct GammaInit {
+    GammaInit() {
+        const cd g[4][4][4] = {
+            {{0, 0, 0, -I}, {0, 0, -I, 0}, {0, I, 0, 0}, {I, 0, 0, 0}},
+            {{0, 0, 0, -1.}, {0, 0, 1., 0}, {0, 1., 0, 0}, {-1., 0, 0, 0}},
+            {{0, 0, -I, 0}, {0, 0, 0, I}, {I, 0, 0, 0}, {0, -I, 0, 0}},
+            {{0, 0, 1., 0}, {0, 0, 0, 1.}, {1., 0, 0, 0}, {0, 1., 0, 0}}};
+        std::memcpy(GAMMA, g, sizeof(g));
+        for (int k = 0; k < 6; ++k) {
+            int m = PLANE_MU[k], n = PLANE_NU[k];
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b) {
+                    cd s = 0;
+                    for (int c = 0; c < 4; ++c) s += GAMMA[m][a][c] * GAMMA[n][c][b] - GAMMA[n][a][c] * GAMMA[m][c][b];
+                    SIGMA[k][a][b] = 0.5 * I * s;
+                }
+        }
+    }
+} gamma_init;
+
+inline const cd* site_spinor(const cd* v, long s) { return v + s * 12; }
+inline cd* site_spinor(cd* v, long s) { return v + s * 12; }
+inline const cd* link(const cd* U, long s, int mu) { return U + (s * 4 + mu) * 9; }
+
+// ---------------------------------------------------------------- form A: unrolled
+// Per direction mu the (1 -+ gamma_mu) projection keeps two spin components.
+// h+ (from x+mu, multiplied by U_mu(x)) and h- (from x-mu, multiplied by
+// U_mu(x-mu)^dagger) are formed for upper spins a = 0,1; the lower spin p
+// partnered with a receives -g_pa (U h+ - U^dag h-), where g_pa is the
+// gamma_mu entry.  Written out direction by direction as
+// DiracWilsonOperator.cpp:21-192 does.
+inline cd mi(cd z) { return cd(z.imag(), -z.real()); }  // -i z
+inline cd pi(cd z) { return cd(-z.imag(), z.real()); }  //  i z
+
+void hop_unrolled(const Geo& g, const cd* U, const cd* in, long s, cd H[12]) {
+    cd hp[3], hm[3], tmm[3], tmp[3];
+    for (int k = 0; k < 12; ++k) H[k] = 0;
+    // mu = 0: h+_0 = psi0 + i psi3, h+_1 = psi1 + i psi2; lower: H3 += -i(...), H2 += -i(...)
+    {
+        long su = g.up[s * 4 + 0], sd = g.dn[s * 4 + 0];
+        const cd *pu = site_spinor(in, su), *pd = site_spinor(in, sd);
+        const cd *Uf = link(U, s, 0), *Ub = link(U, sd, 0);
+        for (int c = 0; c < 3; ++c) { hp[c] = pu[c] + pi(pu[9 + c]); hm[c] = pd[c] - pi(pd[9 + c]); }
+        mv(Uf, hp, tmm); mdv(Ub, hm, tmp);
+        for (int c = 0; c < 3; ++c) { H[c] += tmm[c] + tmp[c]; H[9 + c] += mi(tmm[c] - tmp[c]); }
+        for (int c = 0; c < 3; ++c) { hp[c] = pu[3 + c] + pi(pu[6 + c]); hm[c] = pd[3 + c] - pi(pd[6 + c]); }
+        mv(Uf, hp, tmm); mdv(Ub, hm, tmp);
+        for (int c = 0; c < 3; ++c) { H[3 + c] += tmm[c] + tmp[c]; H[6 + c] += mi(tmm[c] - tmp[c]); }
+    }
+    // mu = 1: h+_0 = psi0 + psi3 (H3 += +), h+_1 = psi1 - psi2 (H2 += -)
+    {
+        long su = g.up[s * 4 + 1], sd = g.dn[s * 4 + 1];
+        const cd *pu = site_spinor(in, su), *pd = site_spinor(in, sd);
+        const cd *Uf = link(U, s, 1), *Ub = link(U, sd, 1);
+        for (int c = 0; c < 3; ++c) { hp[c] = pu[c] + pu[9 + c]; hm[c] = pd[c] - pd[9 + c]; }
+        mv(Uf, hp, tmm); mdv(Ub, hm, tmp);
+        for (int c = 0; c < 3; ++c) { H[c] += tmm[c] + tmp[c]; H[9 + c] += tmm[c] - tmp[c]; }
+        for (int c = 0; c < 3; ++c) { hp[c] = pu[3 + c] - pu[6 + c]; hm[c] = pd[3 + c] + pd[6 + c]; }
+        mv(Uf, hp, tmm); mdv(Ub, hm, tmp);
+        for (int c = 0; c < 3; ++c) { H[3 + c] += tmm[c] + tmp[c]; H[6 + c] -= tmm[c] - tmp[c]; }
+    }
+    // mu = 2: h+_0 = psi0 + i psi2 (H2 += -i), h+_1 = psi1 - i psi3 (H3 += +i)
+    {
+        long su = g.up[s * 4 + 2], sd = g.dn[s * 4 + 2];
+        const cd *pu = site_spinor(in, su), *pd = site_spinor(in, sd);
+        const cd *Uf = link(U, s, 2), *Ub = link(U, sd, 2);
+        for (int c = 0; c < 3; ++c) { hp[c] = pu[c] + pi(pu[6 + c]); hm[c] = pd[c] - pi(pd[6 + c]); }
+        mv(Uf, hp, tmm); mdv(Ub, hm, tmp);
+        for (int c = 0; c < 3; ++c) { H[c] += tmm[c] + tmp[c]; H[6 + c] += mi(tmm[c] - tmp[c]); }
+        for (int c = 0; c < 3; ++c) { hp[c] = pu[3 + c] - pi(pu[9 + c]); hm[c] = pd[3 + c] + pi(pd[9 + c]); }
+        mv(Uf, hp, tmm); mdv(Ub, hm, tmp);
+        for (int c = 0; c < 3; ++c) { H[3 + c] += tmm[c] + tmp[c]; H[9 + c] += pi(tmm[c] - tmp[c]); }
+    }
+    // mu = 3: h+_0 = psi0 - psi2 (H2 += -), h+_1 = psi1 - psi3 (H3 += -)
+    {
+        long su = g.up[s * 4 + 3], sd = g.dn[s * 4 + 3];
+        const cd *pu = site_spinor(in, su), *pd = site_spinor(in, sd);
+        const cd *Uf = link(U, s, 3), *Ub = link(U, sd, 3);
+        for (int c = 0; c < 3; ++c) { hp[c] = pu[c] - pu[6 + c]; hm[c] = pd[c] + pd[6 + c]; }
+        mv(Uf, hp, tmm); mdv(Ub, hm, tmp);
+        for (int c = 0; c < 3; ++c) { H[c] += tmm[c] + tmp[c]; H[6 + c] -= tmm[c] - tmp[c]; }
+        for (int c = 0; c < 3; ++c) { hp[c] = pu[3 + c] - pu[9 + c]; hm[c] = pd[3 + c] + pd[9 + c]; }
+        mv(Uf, hp, tmm); mdv(Ub, hm, tmp);
+        for (int c = 0; c < 3; ++c) { H[3 + c] += tmm[c] + tmp[c]; H[9 + c] -= tmm[c] - tmp[c]; }
+    }
+}
+
+// ---------------------------------------------------------------- form B: projector
+// BasicDiracWilsonOperator.cpp:16-93: tmp_plus = U h+ + U^dag h-,
+// tmp_minus = U h+ - U^dag h- for both upper spins and all mu, then the
+// lower spins are assembled from tmp_minus in one final expression.
+void hop_projector(const Geo& g, const cd* U, const cd* in, long s, cd H[12]) {
+    cd tp[4][2][3], tm[4][2][3];
+    cd h[2][3], a[3], b[3];
+    for (int mu = 0; mu < 4; ++mu) {
+        const cd* p = site_spinor(in, g.up[s * 4 + mu]);
+        for (int c = 0; c < 3; ++c) {
+            switch (mu) {
+                case 0: h[0][c] = p[c] + pi(p[9 + c]); h[1][c] = p[3 + c] + pi(p[6 + c]); break;
+                case 1: h[0][c] = p[c] + p[9 + c]; h[1][c] = p[3 + c] - p[6 + c]; break;
+                case 2: h[0][c] = p[c] + pi(p[6 + c]); h[1][c] = p[3 + c] - pi(p[9 + c]); break;
+                default: h[0][c] = p[c] - p[6 + c]; h[1][c] = p[3 + c] - p[9 + c]; break;
+            }
+        }
+        for (int n = 0; n < 2; ++n) mv(link(U, s, mu), h[n], tp[mu][n]);
+    }
+    for (int mu = 0; mu < 4; ++mu) {
+        long sd = g.dn[s * 4 + mu];
+        const cd* p = site_spinor(in, sd);
+        for (int c = 0; c < 3; ++c) {
+            switch (mu) {
+                case 0: h[0][c] = p[c] - pi(p[9 + c]); h[1][c] = p[3 + c] - pi(p[6 + c]); break;
+                case 1: h[0][c] = p[c] - p[9 + c]; h[1][c] = p[3 + c] + p[6 + c]; break;
+                case 2: h[0][c] = p[c] - pi(p[6 + c]); h[1][c] = p[3 + c] + pi(p[9 + c]); break;
+                default: h[0][c] = p[c] + p[6 + c]; h[1][c] = p[3 + c] + p[9 + c]; break;
+            }
+        }
+        for (int n = 0; n < 2; ++n) {
+            mdv(link(U, sd, mu), h[n], a);
+            for (int c = 0; c < 3; ++c) { b[c] = tp[mu][n][c]; tm[mu][n][c] = b[c] - a[c]; tp[mu][n][c] = b[c] + a[c]; }
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        H[c] = tp[0][0][c] + tp[1][0][c] + tp[2][0][c] + tp[3][0][c];
+        H[3 + c] = tp[0][1][c] + tp[1][1][c] + tp[2][1][c] + tp[3][1][c];
+        // lower spins: D_2 = psi_2 + kappa (tm11 + tm30 + i tm01 + i tm20),
+        //              D_3 = psi_3 + kappa (i tm00 - i tm21 - tm10 + tm31)  (:85-89)
+        H[6 + c] = -tm[1][1][c] - tm[3][0][c] - pi(tm[0][1][c]) - pi(tm[2][0][c]);
+        H[9 + c] = -pi(tm[0][0][c]) + pi(tm[2][1][c]) + tm[1][0][c] - tm[3][1][c];
+    }
+}
+
+// ---------------------------------------------------------------- form C: textbook
+// D psi(x) = psi(x) - kappa sum_mu [(1 - g_mu) U_mu(x) psi(x+mu) + (1 + g_mu) U_mu(x-mu)^dag psi(x-mu)]
+// with full 4x4 gamma matrices, no projection.
+void hop_textbook(const Geo& g, const cd* U, const cd* in, long s, cd H[12]) {
+    for (int k = 0; k < 12; ++k) H[k] = 0;
+    cd w[4][3];
+    for (int mu = 0; mu < 4; ++mu) {
+        long su = g.up[s * 4 + mu], sd = g.dn[s * 4 + mu];
+        const cd *pu = site_spinor(in, su), *pd = site_spinor(in, sd);
+        for (int b = 0; b < 4; ++b) mv(link(U, s, mu), pu + 3 * b, w[b]);
+        for (int a = 0; a < 4; ++a)
+            for (int c = 0; c < 3; ++c) {
+                cd acc = w[a][c];
+                for (int b = 0; b < 4; ++b) acc -= GAMMA[mu][a][b] * w[b][c];
+                H[3 * a + c] += acc;
+            }
+        for (int b = 0; b < 4; ++b) mdv(link(U, sd, mu), pd + 3 * b, w[b]);
+        for (int a = 0; a < 4; ++a)
+            for (int c = 0; c < 3; ++c) {
+                cd acc = w[a][c];
+                for (int b = 0; b < 4; ++b) acc += GAMMA[mu][a][b] * w[b][c];
+                H[3 * a + c] += acc;
+            }
+    }
+}
+
+// ---------------------------------------------------------------- clover
+// Reference chiral-block combination, ImprovedDiracWilsonOperator.cpp:200-243:
+// clover_a = sum_j B_aj(F) psi_j, added to Q as +kappa*csw*clover.
+void clover_block(const cd* F, const cd* psi, cd out[12]) {
+    const cd *F0 = F, *F1 = F + 9, *F2 = F + 18, *F3 = F + 27, *F4 = F + 36, *F5 = F + 45;
+    for (int i = 0; i < 3; ++i) {
+        cd c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+        for (int j = 0; j < 3; ++j) {
+            int e = i + 3 * j;
+            c0 += pi((-F0[e] + F5[e]) * psi[j]);
+            c1 += pi((F0[e] - F5[e]) * psi[3 + j]);
+            c2 += pi((F0[e] + F5[e]) * psi[6 + j]);
+            c3 += pi((-F0[e] - F5[e]) * psi[9 + j]);
+            c0 += (F1[e] + F4[e] + pi(F2[e] - F3[e])) * psi[3 + j];
+            c1 += (-F1[e] - F4[e] + pi(F2[e] - F3[e])) * psi[j];
+            c2 += (-F1[e] + F4[e] + pi(F2[e] + F3[e])) * psi[9 + j];
+            c3 += (F1[e] - F4[e] + pi(F2[e] + F3[e])) * psi[6 + j];
+        }
+        out[i] = c0; out[3 + i] = c1; out[6 + i] = c2; out[9 + i] = c3;
+    }
+}
+
+// Textbook SW term in D: -kappa*csw*sum_{mu<nu} sigma_{mu nu} Fhat_{mu nu} psi, Fhat = -i F.
+void clover_sigma(const cd* F, const cd* psi, double kc, cd out[12]) {
+    for (int k = 0; k < 12; ++k) out[k] = 0;
+    cd w[4][3];
+    for (int p = 0; p < 6; ++p) {
+        const cd* Fp = F + 9 * p;
+        for (int b = 0; b < 4; ++b)
+            for (int i = 0; i < 3; ++i) {
+                cd acc = 0;
+                for (int j = 0; j < 3; ++j) acc += (-I * Fp[i + 3 * j]) * psi[3 * b + j];
+                w[b][i] = acc;
+            }
+        for (int a = 0; a < 4; ++a)
+            for (int i = 0; i < 3; ++i) {
+                cd acc = 0;
+                for (int b = 0; b < 4; ++b) acc += SIGMA[p][a][b] * w[b][i];
+                out[3 * a + i] -= kc * acc;
+            }
+    }
+}
+
+// ---------------------------------------------------------------- operator kinds
+enum { K_WILSON = 0, K_WILSON_BASIC = 1, K_CLOVER = 2, K_EO_CLOVER = 3, K_TEXTBOOK = 4 };
+
+struct Op {
+    const Geo* g;
+    int kind, power, gamma5;
+    double kappa, csw;
+    const cd *U, *F, *inv;
+};
+
+// out = Op v1 + alpha v2 for the single-power operators (Q when gamma5, else D).
+// The gamma5 = false fix-up of the reference (DiracWilsonOperator.cpp:383-388)
+// reduces to exactly this.
+void apply1(const Op& op, cd* out, const cd* v1, const cd* v2, cd alpha) {
+    const Geo& g = *op.g;
+    const long V = g.V;
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < V; ++s) {
+        cd H[12], C[12];
+        const cd* ps = site_spinor(v1, s);
+        cd* o = site_spinor(out, s);
+        if (op.kind == K_WILSON || op.kind == K_CLOVER) hop_unrolled(g, op.U, v1, s, H);
+        else if (op.kind == K_WILSON_BASIC) hop_projector(g, op.U, v1, s, H);
+        else hop_textbook(g, op.U, v1, s, H);
+        // Q = gamma5 (psi - kappa H)
+        for (int k = 0; k < 6; ++k) o[k] = ps[k] - op.kappa * H[k];
+        for (int k = 6; k < 12; ++k) o[k] = -(ps[k] - op.kappa * H[k]);
+        if (op.kind == K_CLOVER && op.csw != 0.) {
+            clover_block(op.F + s * 54, ps, C);
+            for (int k = 0; k < 12; ++k) o[k] += (op.kappa * op.csw) * C[k];
+        } else if (op.kind == K_TEXTBOOK && op.csw != 0.) {
+            clover_sigma(op.F + s * 54, ps, op.kappa * op.csw, C);  // term in D
+            for (int k = 0; k < 6; ++k) o[k] += C[k];
+            for (int k = 6; k < 12; ++k) o[k] -= C[k];
+        }
+        if (!op.gamma5) for (int k = 6; k < 12; ++k) o[k] = -o[k];
+        if (v2) for (int k = 0; k < 12; ++k) o[k] += alpha * v2[s * 12 + k];
+    }
+}
+
+// ---------------------------------------------------------------- even-odd
+// EvenOddImprovedDiracWilsonOperator.cpp:204-281: on sites of parity `part`
+// out = -kappa H in (the hopping part of D), elsewhere out = in.  Aliasing of
+// out and in is allowed as in the reference (it reads the other parity only).
+void eo_hop(const Op& op, cd* out, const cd* in, int part) {
+    const Geo& g = *op.g;
+    std::vector<cd> res(g.V * 12);
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < g.V; ++s) {
+        if (g.par[s] == part) {
+            cd H[12];
+            hop_unrolled(g, op.U, in, s, H);
+            for (int k = 0; k < 12; ++k) res[s * 12 + k] = -op.kappa * H[k];
+        } else {
+            for (int k = 0; k < 12; ++k) res[s * 12 + k] = in[s * 12 + k];
+        }
+    }
+    std::memcpy(out, res.data(), sizeof(cd) * 12 * g.V);
+}
+
+// R = 1 + kappa*csw*gamma5*B on one site as a dense 12x12 (index 3*spin + colour),
+// EvenOddImprovedDiracWilsonOperator.cpp:291-311.
+void site_R(const cd* F, double kc, cd R[144]) {
+    for (int k = 0; k < 144; ++k) R[k] = 0;
+    cd col[12], unit[12];
+    for (int b = 0; b < 12; ++b) {
+        for (int k = 0; k < 12; ++k) unit[k] = (k == b) ? 1. : 0.;
+        clover_block(F, unit, col);
+        for (int a = 0; a < 12; ++a) {
+            cd v = kc * col[a];
+            if (a >= 6) v = -v;
+            R[a + 12 * b] = v + (a == b ? 1. : 0.);
+        }
+    }
+}
+
+// Gauss-Jordan with partial pivoting, column-major n x n.
+bool invert(cd* A, cd* Ainv, int n) {
+    std::vector<cd> M(A, A + n * n);
+    for (int i = 0; i < n * n; ++i) Ainv[i] = 0;
+    for (int i = 0; i < n; ++i) Ainv[i + n * i] = 1;
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        double best = std::abs(M[c + n * c]);
+        for (int r = c + 1; r < n; ++r)
+            if (std::abs(M[r + n * c]) > best) { best = std::abs(M[r + n * c]); piv = r; }
+        if (best == 0.) return false;
+        if (piv != c)
+            for (int j = 0; j < n; ++j) { std::swap(M[c + n * j], M[piv + n * j]); std::swap(Ainv[c + n * j], Ainv[piv + n * j]); }
+        cd d = 1. / M[c + n * c];
+        for (int j = 0; j < n; ++j) { M[c + n * j] *= d; Ainv[c + n * j] *= d; }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            cd f = M[r + n * c];
+            if (f == 0.) continue;
+            for (int j = 0; j < n; ++j) { M[r + n * j] -= f * M[c + n * j]; Ainv[r + n * j] -= f * Ainv[c + n * j]; }
+        }
+    }
+    return true;
+}
+
+// R^-1 applied on the even sites in place (multiplyEvenEvenInverse, :61-89).
+void eo_inv_even(const Op& op, cd* v) {
+    const Geo& g = *op.g;
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < g.V; ++s) {
+        if (g.par[s] != 0) continue;
+        cd a[12], *o = v + s * 12;
+        const cd* M = op.inv + s * 144;
+        for (int k = 0; k < 12; ++k) a[k] = o[k];
+        for (int r = 0; r < 12; ++r) {
+            cd acc = 0;
+            for (int c = 0; c < 12; ++c) acc += M[r + 12 * c] * a[c];
+            o[r] = acc;
+        }
+    }
+}
+
+// multiplyOddOddMinusIdentity (:148-201): on parity `part`,
+// out = R in - out  (R = 1 + kappa csw gamma5 B).
+void eo_oddodd_minus(const Op& op, cd* out, const cd* in, int part) {
+    const Geo& g = *op.g;
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < g.V; ++s) {
+        if (g.par[s] != part) continue;
+        cd C[12];
+        const cd* ps = in + s * 12;
+        cd* o = out + s * 12;
+        clover_block(op.F + s * 54, ps, C);
+        for (int k = 0; k < 6; ++k) o[k] = ps[k] + (op.kappa * op.csw) * C[k] - o[k];
+        for (int k = 6; k < 12; ++k) o[k] = ps[k] - (op.kappa * op.csw) * C[k] - o[k];
+    }
+}
+
+// EvenOdd multiply (:26-47): out_e = in_e; out_o = gamma5^[gamma5] (R_oo - D_oe R_ee^-1 D_eo) in_o.
+void eo_apply(const Op& op, cd* out, const cd* in) {
+    const Geo& g = *op.g;
+    eo_hop(op, out, in, 0);
+    eo_inv_even(op, out);
+    eo_hop(op, out, out, 1);
+    eo_oddodd_minus(op, out, in, 1);
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < g.V; ++s) {
+        if (g.par[s] == 0) {
+            for (int k = 0; k < 12; ++k) out[s * 12 + k] = in[s * 12 + k];
+        } else if (op.gamma5) {
+            for (int k = 6; k < 12; ++k) out[s * 12 + k] = -out[s * 12 + k];
+        }
+    }
+}
+
+void apply_op(const Op& op, cd* out, const cd* in, std::vector<cd>& tmp) {
+    if (op.kind == K_EO_CLOVER) {
+        if (op.power == 1) { eo_apply(op, out, in); return; }
+        tmp.resize(op.g->V * 12);
+        eo_apply(op, tmp.data(), in);
+        eo_apply(op, out, tmp.data());
+        return;
+    }
+    if (op.power == 1) { apply1(op, out, in, nullptr, 0.); return; }
+    tmp.resize(op.g->V * 12);
+    apply1(op, tmp.data(), in, nullptr, 0.);
+    apply1(op, out, tmp.data(), nullptr, 0.);
+}
+
+// ---------------------------------------------------------------- BLAS-1 (long double sums)
+ld norm2(long n, const cd* a) {
+    ld r = 0;
+#pragma omp parallel for reduction(+ : r) schedule(static)
+    for (long i = 0; i < n; ++i) r += (ld)std::norm(a[i]);
+    return r;
+}
+cld dotc(long n, const cd* a, const cd* b) {
+    ld re = 0, im = 0;
+#pragma omp parallel for reduction(+ : re, im) schedule(static)
+    for (long i = 0; i < n; ++i) {
+        cd p = std::conj(a[i]) * b[i];
+        re += p.real();
+        im += p.imag();
+    }
+    return cld(re, im);
+}
+
+Op make_op(const int* dims, int kind, int power, double kappa, double csw, int gamma5,
+           const double* links, const double* F, const double* inv) {
+    Op op;
+    op.g = &geo(dims);
+    op.kind = kind; op.power = power; op.gamma5 = gamma5;
+    op.kappa = kappa; op.csw = csw;
+    op.U = reinterpret_cast<const cd*>(links);
+    op.F = reinterpret_cast<const cd*>(F);
+    op.inv = reinterpret_cast<const cd*>(inv);
+    return op;
+}
+
+// splitmix64, used only for the BiCGStab random start on a zero source
+inline uint64_t splitmix(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+}  // namespace
+
+extern "C" {
+
+typedef struct {
+    int dims[4];
+    int kind;    // 0 DiracWilson, 1 BasicDiracWilson, 2 Improved (clover), 3 EvenOdd clover, 4 textbook
+    int power;   // 1 or 2
+    double kappa, csw;
+    int gamma5;
+    const double* links;
+    const double* F;
+    const double* inv;
+} orc_op_desc;
+
+int orc_version(void) { return 1; }
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+// neighbour tables (LocalLayout.h:71-104)
+void orc_tables(const int* dims, int* sup, int* sdn, int* parity) {
+    const Geo& g = geo(dims);
+    std::memcpy(sup, g.up.data(), sizeof(int) * g.V * 4);
+    std::memcpy(sdn, g.dn.data(), sizeof(int) * g.V * 4);
+    if (parity) std::memcpy(parity, g.par.data(), sizeof(int) * g.V);
+}
+
+// switchAntiperiodicBc (LatticeBoundaryConditions.h:8-14): U_t(x) -> -U_t(x) at t = T-1
+void orc_apply_bc(const int* dims, double* links) {
+    const Geo& g = geo(dims);
+    cd* U = reinterpret_cast<cd*>(links);
+    for (long s = 0; s < g.V; ++s)
+        if (s % g.L[3] == g.L[3] - 1)
+            for (int k = 0; k < 9; ++k) U[(s * 4 + 3) * 9 + k] = -U[(s * 4 + 3) * 9 + k];
+}
+
+// out = Op v1 (+ alpha v2 if v2 != NULL), single power, kinds 0,1,2,4
+void orc_multiply(const orc_op_desc* d, double* out, const double* v1, const double* v2, double are, double aim) {
+    Op op = make_op(d->dims, d->kind, d->power, d->kappa, d->csw, d->gamma5, d->links, d->F, d->inv);
+    cd* o = reinterpret_cast<cd*>(out);
+    const cd* a = reinterpret_cast<const cd*>(v1);
+    const cd* b = reinterpret_cast<const cd*>(v2);
+    std::vector<cd> tmp;
+    if (op.kind == K_EO_CLOVER || op.power == 2) {
+        apply_op(op, o, a, tmp);
+        if (b) {
+            const long n = op.g->V * 12;
+            cd al(are, aim);
+            for (long i = 0; i < n; ++i) o[i] += al * b[i];
+        }
+        return;
+    }
+    apply1(op, o, a, b, cd(are, aim));
+}
+
+// Field strength, ImprovedDiracWilsonOperator.cpp:502-523.  Clover leaves of
+// plane (mu,nu) at x, all starting and ending at x; F = (Q - Q^dag)/8.
+void orc_field_strength(const int* dims, const double* links, double* Fout) {
+    const Geo& g = geo(dims);
+    const cd* U = reinterpret_cast<const cd*>(links);
+    cd* F = reinterpret_cast<cd*>(Fout);
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < g.V; ++s) {
+        for (int p = 0; p < 6; ++p) {
+            int m = PLANE_MU[p], n = PLANE_NU[p];
+            long xm = g.dn[s * 4 + m], xn = g.dn[s * 4 + n];
+            long xmn = g.dn[xm * 4 + n];
+            long xpm = g.up[s * 4 + m], xpn = g.up[s * 4 + n];
+            long xnpm = g.up[xn * 4 + m];   // x - nu + mu
+            long xmpn = g.up[xm * 4 + n];   // x - mu + nu
+            M3 Um_x = load3(link(U, s, m)), Un_x = load3(link(U, s, n));
+            M3 Um_xm = load3(link(U, xm, m)), Un_xm = load3(link(U, xm, n));
+            M3 Um_xmn = load3(link(U, xmn, m)), Un_xmn = load3(link(U, xmn, n));
+            M3 Un_xn = load3(link(U, xn, n)), Um_xn = load3(link(U, xn, m));
+            M3 Un_xnpm = load3(link(U, xnpm, n));
+            M3 Un_xpm = load3(link(U, xpm, n)), Um_xpn = load3(link(U, xpn, m));
+            M3 Um_xmpn = load3(link(U, xmpn, m));
+            M3 l1 = mul(mul(dag(Um_xm), dag(Un_xmn)), mul(Um_xmn, Un_xn));
+            M3 l2 = mul(mul(dag(Un_xn), Um_xn), mul(Un_xnpm, dag(Um_x)));
+            M3 l3 = mul(mul(Um_x, Un_xpm), mul(dag(Um_xpn), dag(Un_x)));
+            M3 l4 = mul(mul(Un_x, dag(Um_xmpn)), mul(dag(Un_xm), Um_xm));
+            M3 Q = add(add(l1, l2), add(l3, l4));
+            M3 Qd = dag(Q);
+            for (int k = 0; k < 9; ++k) F[(s * 6 + p) * 9 + k] = 0.125 * (Q.a[k] - Qd.a[k]);
+        }
+    }
+}
+
+// R_ee^-1 for every site (calculateInverseEvenEven, :285-317).  Returns the
+// number of singular sites.
+int orc_clover_inverse(const int* dims, const double* Fin, double kappa, double csw, double* inv_out) {
+    const Geo& g = geo(dims);
+    const cd* F = reinterpret_cast<const cd*>(Fin);
+    cd* inv = reinterpret_cast<cd*>(inv_out);
+    int bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+    for (long s = 0; s < g.V; ++s) {
+        cd R[144];
+        site_R(F + s * 54, kappa * csw, R);
+        if (!invert(R, inv + s * 144, 12)) ++bad;
+    }
+    return bad;
+}
+
+// dense site matrix R = 1 + kappa csw gamma5 B (for tests)
+void orc_clover_site_matrix(const double* Fsite, double kappa, double csw, double* R) {
+    site_R(reinterpret_cast<const cd*>(Fsite), kappa * csw, reinterpret_cast<cd*>(R));
+}
+
+// EO building blocks on full-V vectors
+void orc_eo_hop(const orc_op_desc* d, double* out, const double* in, int part) {
+    Op op = make_op(d->dims, K_EO_CLOVER, 1, d->kappa, d->csw, d->gamma5, d->links, d->F, d->inv);
+    eo_hop(op, reinterpret_cast<cd*>(out), reinterpret_cast<const cd*>(in), part);
+}
+void orc_eo_inv_even(const orc_op_desc* d, double* v) {
+    Op op = make_op(d->dims, K_EO_CLOVER, 1, d->kappa, d->csw, d->gamma5, d->links, d->F, d->inv);
+    eo_inv_even(op, reinterpret_cast<cd*>(v));
+}
+void orc_eo_oddodd_minus(const orc_op_desc* d, double* out, const double* in, int part) {
+    Op op = make_op(d->dims, K_EO_CLOVER, 1, d->kappa, d->csw, d->gamma5, d->links, d->F, d->inv);
+    eo_oddodd_minus(op, reinterpret_cast<cd*>(out), reinterpret_cast<const cd*>(in), part);
+}
+
+// ------------------------------------------------ AlgebraUtils (AlgebraUtils.h)
+double orc_squared_norm(long nsites, const double* a) {
+    return (double)norm2(nsites * 12, reinterpret_cast<const cd*>(a));
+}
+void orc_dot(long nsites, const double* a, const double* b, double* re, double* im) {
+    cld r = dotc(nsites * 12, reinterpret_cast<const cd*>(a), reinterpret_cast<const cd*>(b));
+    *re = (double)r.real();
+    *im = (double)r.imag();
+}
+void orc_gamma5dot(long nsites, const double* a, const double* b, double* re, double* im) {
+    const cd* x = reinterpret_cast<const cd*>(a);
+    const cd* y = reinterpret_cast<const cd*>(b);
+    ld sr = 0, si = 0;
+    for (long s = 0; s < nsites; ++s)
+        for (int k = 0; k < 12; ++k) {
+            cd p = std::conj(x[s * 12 + k]) * y[s * 12 + k];
+            if (k < 6) { sr += p.real(); si += p.imag(); }
+            else { sr -= p.real(); si -= p.imag(); }
+        }
+    *re = (double)sr;
+    *im = (double)si;
+}
+double orc_difference_norm(long nsites, const double* a, const double* b) {
+    const cd* x = reinterpret_cast<const cd*>(a);
+    const cd* y = reinterpret_cast<const cd*>(b);
+    ld r = 0;
+    for (long i = 0; i < nsites * 12; ++i) r += (ld)std::norm(x[i] - y[i]);
+    return (double)r;
+}
+
+// ------------------------------------------------ solvers
+typedef struct {
+    int converged;
+    unsigned steps;     // lastSteps as the reference reports it
+    double last_error;  // final |r|^2
+    unsigned applies;   // operator applications
+} orc_solve_info;
+
+// ConjugateGradient::solve (ConjugateGradient.cpp:10-74), stop |r|^2 < eps.
+int orc_cg(const orc_op_desc* d, const double* bsrc, double* xout, const double* x0, double eps,
+           unsigned max_steps, orc_solve_info* info) {
+    Op op = make_op(d->dims, d->kind, d->power, d->kappa, d->csw, d->gamma5, d->links, d->F, d->inv);
+    const long n = op.g->V * 12;
+    const cd* b = reinterpret_cast<const cd*>(bsrc);
+    std::vector<cd> x(n), r(n), p(n), tmp(n), scratch;
+    const cd* x0c = x0 ? reinterpret_cast<const cd*>(x0) : b;
+    std::copy(x0c, x0c + n, x.begin());
+    unsigned applies = 0;
+    apply_op(op, tmp.data(), x.data(), scratch); ++applies;
+    for (long i = 0; i < n; ++i) { r[i] = b[i] - tmp[i]; p[i] = r[i]; }
+    ld norm = norm2(n, r.data()), norm_next = norm;
+    int ok = 0;
+    unsigned steps = max_steps;
+    for (unsigned step = 0; step < max_steps; ++step) {
+        apply_op(op, tmp.data(), p.data(), scratch); ++applies;
+        norm = norm_next;
+        cld pAp = dotc(n, p.data(), tmp.data());
+        cd alpha = cd(cld(norm, 0) / pAp);
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; ++i) { x[i] += alpha * p[i]; r[i] -= alpha * tmp[i]; }
+        norm_next = norm2(n, r.data());
+        if (norm_next < eps) { steps = step; ok = 1; break; }
+        double beta = (double)(norm_next / norm);
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < n; ++i) p[i] = r[i] + beta * p[i];
+    }
+    std::copy(x.begin(), x.end(), reinterpret_cast<cd*>(xout));
+    if (info) { info->converged = ok; info->steps = steps; info->last_error = (double)norm_next; info->applies = applies; }
+    return ok;
+}
+
+// BiConjugateGradient::solve, plain BiCGStab (BiConjugateGradient.cpp:321-507).
+int orc_bicgstab(const orc_op_desc* d, const double* bsrc, double* xout, const double* x0, double precision,
+                 unsigned max_steps, orc_solve_info* info) {
+    Op op = make_op(d->dims, d->kind, d->power, d->kappa, d->csw, d->gamma5, d->links, d->F, d->inv);
+    const long n = op.g->V * 12;
+    const cd* b = reinterpret_cast<const cd*>(bsrc);
+    std::vector<cd> x(n), r(n), rh(n), p(n), nu(n), s(n), t(n), scratch;
+    if (x0) {
+        std::copy(reinterpret_cast<const cd*>(x0), reinterpret_cast<const cd*>(x0) + n, x.begin());
+    } else if ((double)norm2(n, b) > precision) {
+        std::copy(b, b + n, x.begin());
+    } else {
+        // the reference draws a time-seeded random vector here; a fixed seed is used instead
+        for (long i = 0; i < n; ++i)
+            x[i] = cd((splitmix(2 * i) >> 11) * 0x1.0p-53, (splitmix(2 * i + 1) >> 11) * 0x1.0p-53);
+    }
+    unsigned applies = 0;
+    apply_op(op, p.data(), x.data(), scratch); ++applies;
+    for (long i = 0; i < n; ++i) { r[i] = b[i] - p[i]; rh[i] = b[i] + p[i]; p[i] = 0; nu[i] = 0; }
+    cd alpha = 1., omega = 1.;
+    cld rho = 1.;
+    unsigned step = 0;
+    ld norm = 0;
+    int ok = 0;
+    double last = 0;
+    while (step < max_steps) {
+        cld rho_next = dotc(n, rh.data(), r.data());
+        if (std::norm(rho_next) == 0.) { ok = 0; last = (double)norm; goto done; }
+        {
+            cd beta = cd(rho_next / rho) * (alpha / omega);
+#pragma omp parallel for schedule(static)
+            for (long i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - omega * nu[i]);
+            apply_op(op, nu.data(), p.data(), scratch); ++applies;
+            cld at = dotc(n, rh.data(), nu.data());
+            alpha = cd(rho_next / at);
+#pragma omp parallel for schedule(static)
+            for (long i = 0; i < n; ++i) s[i] = r[i] - alpha * nu[i];
+            apply_op(op, t.data(), s.data(), scratch); ++applies;
+            cld t1 = dotc(n, t.data(), s.data());
+            cld t2 = dotc(n, t.data(), t.data());
+            omega = cd(t1 / t2);
+            if (t2.real() == 0) {
+                std::copy(b, b + n, x.begin());
+                ok = 1;
+                last = 0;
+                goto done;
+            }
+#pragma omp parallel for schedule(static)
+            for (long i = 0; i < n; ++i) { x[i] += alpha * p[i] + omega * s[i]; r[i] = s[i] - omega * t[i]; }
+            norm = norm2(n, r.data());
+            last = (double)norm;
+            if (norm < precision) { ok = 1; goto done; }
+            rho = rho_next;
+            ++step;
+        }
+    }
+    step = max_steps;
+done:
+    std::copy(x.begin(), x.end(), reinterpret_cast<cd*>(xout));
+    if (info) { info->converged = ok; info->steps = step; info->last_error = last; info->applies = applies; }
+    return ok;
+}
+
+// PreconditionedBiCGStab::solve, even-odd branch (PreconditionedBiCGStab.cpp:33-100).
+// d must describe the clover operator (kind 2) with F and inv (built with the
+// same kappa, csw).  Returns x = D^-1 b.
+int orc_precond_bicgstab(const orc_op_desc* d, const double* bsrc, double* xout, const double* x0,
+                         double precision, unsigned max_steps, orc_solve_info* info) {
+    Op eo = make_op(d->dims, K_EO_CLOVER, 1, d->kappa, d->csw, 1, d->links, d->F, d->inv);
+    const Geo& g = *eo.g;
+    const long n = g.V * 12;
+    const cd* b = reinterpret_cast<const cd*>(bsrc);
+    std::vector<cd> src(b, b + n), src_eo(n), xo(n), xe(n);
+    eo_inv_even(eo, src.data());
+    eo_hop(eo, src.data(), src.data(), 1);
+    for (long s = 0; s < g.V; ++s) {
+        for (int k = 0; k < 12; ++k) {
+            cd v = (g.par[s] == 1) ? b[s * 12 + k] - src[s * 12 + k] : b[s * 12 + k];
+            src[s * 12 + k] = (k >= 6) ? -v : v;
+        }
+    }
+    eo_apply(eo, src_eo.data(), src.data());
+    orc_op_desc sq = *d;
+    sq.kind = K_EO_CLOVER;
+    sq.power = 2;
+    sq.gamma5 = 1;
+    int ok = orc_bicgstab(&sq, reinterpret_cast<const double*>(src_eo.data()), reinterpret_cast<double*>(xo.data()),
+                          x0, precision, max_steps, info);
+    eo_hop(eo, xe.data(), xo.data(), 0);
+    for (long s = 0; s < g.V; ++s)
+        for (int k = 0; k < 12; ++k)
+            xe[s * 12 + k] = (g.par[s] == 0) ? b[s * 12 + k] - xe[s * 12 + k] : xo[s * 12 + k];
+    eo_inv_even(eo, xe.data());
+    std::copy(xe.begin(), xe.end(), reinterpret_cast<cd*>(xout));
+    return ok;
+}
+
+}  // extern "C"
