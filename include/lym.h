@@ -1,0 +1,169 @@
+/* lym.h -- C ABI of the MI355X (gfx950) LeonardYM fermion-solve library.
+ *
+ * Plain pointers, sizes and opaque handles; no C++ or torch types.  Every
+ * function returns LYM_OK (0) or an error code and never exits the process;
+ * lym_last_error() gives the message of the calling thread's last failure.
+ * (The reference exits on configuration errors:
+ *  source/dirac_operators/DiracOperator.cpp:84-87, EvenOddImprovedDiracWilsonOperator.cpp:49-52.)
+ *
+ * Host data layouts are the reference's raw vectors:
+ *   spinor : V x [4 spin][3 colour] complex double (re, im interleaved)
+ *            (reduced_dirac_vector_t, source/LatticeDefinitions.h:86-114)
+ *   links  : V x [4 mu][3x3 column-major] complex double, or for the SU(2)
+ *            adjoint representation V x [4][3x3 column-major] real double
+ *            (FermionicGroup, source/MatrixTypedef.h:61-90)
+ * with sites in t-fastest lexicographic order
+ *   idx = ((x*Ly + y)*Lz + z)*Lt + t   (source/MPILattice/LocalLayout.h:149-155).
+ * Lt must be even; even-odd operators need all four extents even
+ * (source/inverters/PreconditionedBiCGStab.cpp:43).
+ */
+#ifndef LYM_H
+#define LYM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LYM_OK 0
+#define LYM_ERR_INVALID 1     /* bad argument or configuration */
+#define LYM_ERR_DEVICE 2      /* HIP runtime failure or no device */
+#define LYM_ERR_NOMEM 3       /* device allocation failed */
+#define LYM_ERR_UNSUPPORTED 4 /* valid request the library does not implement */
+#define LYM_ERR_SINGULAR 5    /* a clover block could not be inverted */
+
+/* gauge-group representations of the fermion links */
+#define LYM_REP_SU3_FUND 0 /* NUMCOLORS=3, fundamental */
+#define LYM_REP_SU2_ADJ 1  /* NUMCOLORS=2, ADJOINT: real 3x3 links */
+
+/* operator kinds (names of DiracOperator::getInstance, DiracOperator.cpp:20-165) */
+#define LYM_OP_WILSON 0       /* "DiracWilson":      DiracWilsonOperator.cpp:16-391 */
+#define LYM_OP_BASIC_WILSON 1 /* "BasicDiracWilson": BasicDiracWilsonOperator.cpp:12-178 (same operator) */
+#define LYM_OP_CLOVER 2       /* "Improved":         ImprovedDiracWilsonOperator.cpp:22-487 */
+#define LYM_OP_EO_CLOVER 3    /* EvenOddImprovedDiracWilsonOperator.cpp:26-47 */
+
+/* solvers */
+#define LYM_SOLVER_CG 0                 /* ConjugateGradient.cpp:10-74 */
+#define LYM_SOLVER_BICGSTAB 1           /* BiConjugateGradient.cpp:321-507 */
+#define LYM_SOLVER_PRECOND_BICGSTAB 2   /* PreconditionedBiCGStab.cpp:33-100 */
+
+/* link storage on the device */
+#define LYM_LINKS_FULL 0          /* 18 reals (9 for adjoint) */
+#define LYM_LINKS_RECONSTRUCT 1   /* 12 reals (6 for adjoint), third row rebuilt */
+
+typedef struct lym_ctx lym_ctx;
+typedef struct lym_gauge lym_gauge;
+typedef struct lym_spinor lym_spinor;
+typedef struct lym_op lym_op;
+typedef struct lym_solver lym_solver;
+
+typedef struct {
+    int converged;      /* solve() return value of the reference */
+    unsigned steps;     /* getLastSteps(): iteration index at convergence, or max steps */
+    double last_error;  /* final |r|^2 of the Krylov recursion (getLastError()) */
+    unsigned applies;   /* operator applications performed */
+    double seconds;     /* wall time of the solve on the device stream */
+} lym_solve_info;
+
+/* ------------------------------------------------------------------ errors / device */
+const char* lym_last_error(void);
+int lym_device_count(int* n);
+
+/* ------------------------------------------------------------------ context
+ * Lattice geometry + stream: replaces LocalLayout::initialize()
+ * (source/MPILattice/LocalLayout.h:34-110) and the boundary-condition choice of
+ * Environment::synchronize (source/Environment.cpp:58-78): bc_t = -1 is
+ * "fermion_antiperiodic", +1 "fermion_periodic". */
+int lym_ctx_create(const int dims[4], int rep, int bc_t, lym_ctx** out);
+void lym_ctx_destroy(lym_ctx* ctx);
+int lym_ctx_synchronize(lym_ctx* ctx);
+/* hipStream_t the context launches on (for callers that order their own work) */
+void* lym_ctx_stream(lym_ctx* ctx);
+/* HIP events around work on the context stream; stop returns elapsed ms */
+int lym_ctx_timer_start(lym_ctx* ctx);
+int lym_ctx_timer_stop(lym_ctx* ctx, float* ms);
+
+/* ------------------------------------------------------------------ gauge field
+ * Device-resident fermion links: the copy DiracOperator::setLattice makes
+ * (source/dirac_operators/DiracOperator.cpp:332-334), shared by any number of
+ * operators.  links_have_bc_folded = 1 when the time-boundary sign is already
+ * in the links, as Environment::synchronize leaves them. */
+int lym_gauge_create(lym_ctx* ctx, const double* links, int links_have_bc_folded, int storage, lym_gauge** out);
+void lym_gauge_destroy(lym_gauge* g);
+
+/* ------------------------------------------------------------------ spinors
+ * A full-lattice reduced_dirac_vector_t resident on the device. */
+int lym_spinor_create(lym_ctx* ctx, lym_spinor** out);
+void lym_spinor_destroy(lym_spinor* s);
+int lym_spinor_upload(lym_spinor* s, const double* host);
+int lym_spinor_download(const lym_spinor* s, double* host);
+int lym_spinor_copy(lym_spinor* dst, const lym_spinor* src);
+
+/* AlgebraUtils (source/algebra_utils/AlgebraUtils.h) */
+int lym_spinor_set_zero(lym_spinor* s);                                                    /* setToZero :256-263 */
+int lym_spinor_gamma5(lym_spinor* s);                                                      /* gamma5 :268-275 */
+int lym_spinor_squared_norm(const lym_spinor* s, double* out);                             /* squaredNorm :17-27 */
+int lym_spinor_dot(const lym_spinor* a, const lym_spinor* b, double* re, double* im);      /* dot :182-196 */
+int lym_spinor_gamma5dot(const lym_spinor* a, const lym_spinor* b, double* re, double* im);/* gamma5dot :219-238 */
+int lym_spinor_difference_norm(const lym_spinor* a, const lym_spinor* b, double* out);     /* differenceNorm :244-254 */
+int lym_spinor_axpy(lym_spinor* y, double are, double aim, const lym_spinor* x);            /* y += a x */
+
+/* ------------------------------------------------------------------ operators
+ * DiracOperator (source/dirac_operators/DiracOperator.h:11-140).  power 2 gives
+ * the Square* operators Q^2 (SquareDiracWilsonOperator.cpp:12-22,
+ * SquareImprovedDiracWilsonOperator.cpp:13-23, SquareEvenOddImprovedDiracWilsonOperator.cpp:13-27).
+ * gamma5 = 1: multiply computes Q = gamma5 D; 0: D (DiracWilsonOperator.cpp:194-199). */
+int lym_op_create(lym_ctx* ctx, int kind, unsigned power, double kappa, double csw, int gamma5, lym_op** out);
+/* by factory name "DiracWilson" | "BasicDiracWilson" | "Improved" | "EvenOddImproved"
+ * (DiracOperator::getInstance, DiracOperator.cpp:20-165) */
+int lym_op_create_by_name(lym_ctx* ctx, const char* name, unsigned power, double kappa, double csw, int gamma5,
+                          lym_op** out);
+void lym_op_destroy(lym_op* op);
+int lym_op_set_gauge(lym_op* op, const lym_gauge* g);   /* setLattice (:125) */
+int lym_op_set_kappa(lym_op* op, double kappa);         /* setKappa (:107); diagonal blocks rebuilt */
+int lym_op_set_csw(lym_op* op, double csw);             /* ImprovedDiracWilsonOperator::setCSW */
+int lym_op_set_gamma5(lym_op* op, int gamma5);          /* setGamma5 (:127) */
+int lym_op_get_params(const lym_op* op, double* kappa, double* csw, int* gamma5, int* kind, unsigned* power);
+/* multiply (DiracOperator.h:90) */
+int lym_op_multiply(lym_op* op, lym_spinor* out, const lym_spinor* in);
+/* multiplyAdd (DiracOperator.h:99): out = Op v1 + alpha v2 */
+int lym_op_multiply_add(lym_op* op, lym_spinor* out, const lym_spinor* v1, const lym_spinor* v2, double are,
+                        double aim);
+
+/* ------------------------------------------------------------------ solvers
+ * Solver (source/inverters/Solver.h:8-67), ConjugateGradient
+ * (ConjugateGradient.h:8-36), BiConjugateGradient, PreconditionedBiCGStab
+ * (PreconditionedBiCGStab.h:9-31).  Stopping rule |r|^2 < precision
+ * (absolute, as the reference).  Defaults: CG 1e-11 / 3000 steps
+ * (ConjugateGradient.cpp:6), BiCGStab 1e-10 / 1000 (Solver.h:10). */
+int lym_solver_create(lym_ctx* ctx, int kind, lym_solver** out);
+void lym_solver_destroy(lym_solver* s);
+int lym_solver_set_precision(lym_solver* s, double precision);
+int lym_solver_set_max_steps(lym_solver* s, unsigned max_steps);
+int lym_solver_set_even_odd(lym_solver* s, int use_even_odd);   /* setUseEvenOddPreconditioning */
+/* solve(dirac, source, solution, initial_guess) -- x0 may be NULL */
+int lym_solver_solve(lym_solver* s, lym_op* op, const lym_spinor* source, lym_spinor* solution,
+                     const lym_spinor* initial_guess, lym_solve_info* info);
+
+/* ------------------------------------------------------------------ host-side input preparation
+ * Seeded synthetic inputs (the reference seeds from time(NULL),
+ * source/utils/RandomSeed.cpp:17-28, so inputs are generated here instead). */
+/* U = exp(i eps H), H = sum_a w_a lambda_a / 2, w_a ~ N(0,1); SU(3), column-major */
+int lym_gen_su3_links(const int dims[4], double eps, uint64_t seed, double* links);
+/* U = exp(i eps sigma.a), a ~ N(0,1)^3; SU(2) 2x2 column-major complex, V x 4 x 4 complex */
+int lym_gen_su2_links(const int dims[4], double eps, uint64_t seed, double* links);
+/* ConvertLattice::toAdjoint (source/utils/ConvertLattice.h:24-34): R_ab = 2 Re tr(T_a U T_b U^+), T = sigma/2 */
+int lym_su2_to_adjoint(long nlinks, const double* su2, double* adj);
+/* switchAntiperiodicBc (source/LatticeBoundaryConditions.h:8-14) on host links */
+int lym_apply_bc(const int dims[4], int rep, int bc_t, double* links);
+/* complex Gaussian, sigma = 1/sqrt(2) per real part (AlgebraUtils.h:92-127) */
+int lym_gen_gaussian_spinor(long nsites, uint64_t seed, double* out);
+/* unit point source at `site` (StochasticEstimator::generateSource, StochasticEstimator.cpp:149-161) */
+int lym_point_source(const int dims[4], const int site[4], int spin, int colour, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LYM_H */

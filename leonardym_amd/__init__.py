@@ -1,0 +1,356 @@
+"""Python binding of the MI355X LeonardYM fermion-solve library (liblym.so).
+
+This is plumbing for tests and benchmarks: every call goes through the C ABI
+declared in include/lym.h, every result is computed by the HIP kernels, and
+there is no CPU fallback -- importing without a built liblym.so raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblym.so")
+
+REP_SU3_FUND, REP_SU2_ADJ = 0, 1
+OP_WILSON, OP_BASIC_WILSON, OP_CLOVER, OP_EO_CLOVER = 0, 1, 2, 3
+SOLVER_CG, SOLVER_BICGSTAB, SOLVER_PRECOND_BICGSTAB = 0, 1, 2
+LINKS_FULL, LINKS_RECONSTRUCT = 0, 1
+
+# C ABI: name -> (restype, argtypes)
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_U = ctypes.c_uint
+_U64 = ctypes.c_uint64
+_L = ctypes.c_long
+_PD = ctypes.POINTER(ctypes.c_double)
+_PI = ctypes.POINTER(ctypes.c_int)
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("converged", ctypes.c_int), ("steps", ctypes.c_uint), ("last_error", ctypes.c_double),
+                ("applies", ctypes.c_uint), ("seconds", ctypes.c_double)]
+
+
+SIGNATURES = {
+    "lym_last_error": (ctypes.c_char_p, []),
+    "lym_device_count": (_I, [_PI]),
+    "lym_ctx_create": (_I, [_P, _I, _I, ctypes.POINTER(_P)]),
+    "lym_ctx_destroy": (None, [_P]),
+    "lym_ctx_synchronize": (_I, [_P]),
+    "lym_ctx_stream": (_P, [_P]),
+    "lym_ctx_timer_start": (_I, [_P]),
+    "lym_ctx_timer_stop": (_I, [_P, ctypes.POINTER(ctypes.c_float)]),
+    "lym_gauge_create": (_I, [_P, _P, _I, _I, ctypes.POINTER(_P)]),
+    "lym_gauge_destroy": (None, [_P]),
+    "lym_spinor_create": (_I, [_P, ctypes.POINTER(_P)]),
+    "lym_spinor_destroy": (None, [_P]),
+    "lym_spinor_upload": (_I, [_P, _P]),
+    "lym_spinor_download": (_I, [_P, _P]),
+    "lym_spinor_copy": (_I, [_P, _P]),
+    "lym_spinor_set_zero": (_I, [_P]),
+    "lym_spinor_gamma5": (_I, [_P]),
+    "lym_spinor_squared_norm": (_I, [_P, _PD]),
+    "lym_spinor_dot": (_I, [_P, _P, _PD, _PD]),
+    "lym_spinor_gamma5dot": (_I, [_P, _P, _PD, _PD]),
+    "lym_spinor_difference_norm": (_I, [_P, _P, _PD]),
+    "lym_spinor_axpy": (_I, [_P, _D, _D, _P]),
+    "lym_op_create": (_I, [_P, _I, _U, _D, _D, _I, ctypes.POINTER(_P)]),
+    "lym_op_create_by_name": (_I, [_P, ctypes.c_char_p, _U, _D, _D, _I, ctypes.POINTER(_P)]),
+    "lym_op_destroy": (None, [_P]),
+    "lym_op_set_gauge": (_I, [_P, _P]),
+    "lym_op_set_kappa": (_I, [_P, _D]),
+    "lym_op_set_csw": (_I, [_P, _D]),
+    "lym_op_set_gamma5": (_I, [_P, _I]),
+    "lym_op_get_params": (_I, [_P, _PD, _PD, _PI, _PI, ctypes.POINTER(_U)]),
+    "lym_op_multiply": (_I, [_P, _P, _P]),
+    "lym_op_multiply_add": (_I, [_P, _P, _P, _P, _D, _D]),
+    "lym_solver_create": (_I, [_P, _I, ctypes.POINTER(_P)]),
+    "lym_solver_destroy": (None, [_P]),
+    "lym_solver_set_precision": (_I, [_P, _D]),
+    "lym_solver_set_max_steps": (_I, [_P, _U]),
+    "lym_solver_set_even_odd": (_I, [_P, _I]),
+    "lym_solver_solve": (_I, [_P, _P, _P, _P, _P, ctypes.POINTER(SolveInfo)]),
+    "lym_gen_su3_links": (_I, [_P, _D, _U64, _P]),
+    "lym_gen_su2_links": (_I, [_P, _D, _U64, _P]),
+    "lym_su2_to_adjoint": (_I, [_L, _P, _P]),
+    "lym_apply_bc": (_I, [_P, _I, _I, _P]),
+    "lym_gen_gaussian_spinor": (_I, [_L, _U64, _P]),
+    "lym_point_source": (_I, [_P, _P, _I, _I, _P]),
+}
+
+_lib = None
+
+
+class LymError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded liblym.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LymError("liblym.so is not built (run __graft_entry__.build() or make -C leonardym_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise LymError("lym error %d: %s" % (rc, lib().lym_last_error().decode()))
+
+
+def _arr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def device_count():
+    n = ctypes.c_int()
+    rc = lib().lym_device_count(ctypes.byref(n))
+    return n.value if rc == 0 else 0
+
+
+# ---------------------------------------------------------------- host-side inputs
+def gen_su3_links(dims, eps, seed):
+    V = int(np.prod(dims))
+    out = np.empty((V, 4, 3, 3), np.complex128)  # memory: column-major per link
+    _check(lib().lym_gen_su3_links(_arr(np.array(dims, np.int32)), eps, seed, _arr(out)))
+    return np.ascontiguousarray(np.swapaxes(out, -1, -2))  # -> [..., i, j]
+
+
+def gen_su2_links(dims, eps, seed):
+    V = int(np.prod(dims))
+    out = np.empty((V, 4, 2, 2), np.complex128)
+    _check(lib().lym_gen_su2_links(_arr(np.array(dims, np.int32)), eps, seed, _arr(out)))
+    return np.ascontiguousarray(np.swapaxes(out, -1, -2))
+
+
+def su2_to_adjoint(U2):
+    """[..., 2, 2] complex (i, j) -> [..., 3, 3] real (a, b)."""
+    shp = U2.shape[:-2]
+    cm = np.ascontiguousarray(np.swapaxes(U2, -1, -2)).reshape(-1, 2, 2)
+    out = np.empty((cm.shape[0], 3, 3), np.float64)
+    _check(lib().lym_su2_to_adjoint(cm.shape[0], _arr(cm), _arr(out)))
+    return np.ascontiguousarray(np.swapaxes(out, -1, -2)).reshape(shp + (3, 3))
+
+
+def apply_bc(dims, links, rep, bc_t=-1):
+    """Fold the time-boundary sign into [V,4,3,3] links (returns a copy)."""
+    cm = np.ascontiguousarray(np.swapaxes(links, -1, -2))
+    _check(lib().lym_apply_bc(_arr(np.array(dims, np.int32)), rep, bc_t, _arr(cm)))
+    return np.ascontiguousarray(np.swapaxes(cm, -1, -2))
+
+
+def gaussian_spinor(nsites, seed):
+    out = np.empty((nsites, 4, 3), np.complex128)
+    _check(lib().lym_gen_gaussian_spinor(nsites, seed, _arr(out)))
+    return out
+
+
+def point_source(dims, site, spin, colour):
+    V = int(np.prod(dims))
+    out = np.empty((V, 4, 3), np.complex128)
+    _check(lib().lym_point_source(_arr(np.array(dims, np.int32)), _arr(np.array(site, np.int32)), spin, colour,
+                                  _arr(out)))
+    return out
+
+
+# ---------------------------------------------------------------- device objects
+class Context:
+    def __init__(self, dims, rep=REP_SU3_FUND, bc_t=-1):
+        self.dims = tuple(int(d) for d in dims)
+        self.V = int(np.prod(self.dims))
+        self.rep = rep
+        h = ctypes.c_void_p()
+        _check(lib().lym_ctx_create(_arr(np.array(self.dims, np.int32)), rep, bc_t, ctypes.byref(h)))
+        self.h = h
+
+    def synchronize(self):
+        _check(lib().lym_ctx_synchronize(self.h))
+
+    def timer_start(self):
+        _check(lib().lym_ctx_timer_start(self.h))
+
+    def timer_stop(self):
+        ms = ctypes.c_float()
+        _check(lib().lym_ctx_timer_stop(self.h, ctypes.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self.h:
+            lib().lym_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Gauge:
+    def __init__(self, ctx, links, folded=True, storage=LINKS_FULL):
+        """links: [V,4,3,3] complex (SU(3)) or real (SU(2) adjoint), element (i, j)."""
+        self.ctx = ctx
+        if ctx.rep == REP_SU2_ADJ:
+            cm = np.ascontiguousarray(np.swapaxes(np.asarray(links, np.float64), -1, -2))
+        else:
+            cm = np.ascontiguousarray(np.swapaxes(np.asarray(links, np.complex128), -1, -2))
+        h = ctypes.c_void_p()
+        _check(lib().lym_gauge_create(ctx.h, _arr(cm), int(folded), storage, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().lym_gauge_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Spinor:
+    def __init__(self, ctx, host=None):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        _check(lib().lym_spinor_create(ctx.h, ctypes.byref(h)))
+        self.h = h
+        if host is not None:
+            self.upload(host)
+
+    def upload(self, host):
+        a = np.ascontiguousarray(host, np.complex128)
+        assert a.size == self.ctx.V * 12
+        _check(lib().lym_spinor_upload(self.h, _arr(a)))
+
+    def download(self):
+        out = np.empty((self.ctx.V, 4, 3), np.complex128)
+        _check(lib().lym_spinor_download(self.h, _arr(out)))
+        return out
+
+    def squared_norm(self):
+        r = ctypes.c_double()
+        _check(lib().lym_spinor_squared_norm(self.h, ctypes.byref(r)))
+        return r.value
+
+    def dot(self, other):
+        re, im = ctypes.c_double(), ctypes.c_double()
+        _check(lib().lym_spinor_dot(self.h, other.h, ctypes.byref(re), ctypes.byref(im)))
+        return complex(re.value, im.value)
+
+    def gamma5dot(self, other):
+        re, im = ctypes.c_double(), ctypes.c_double()
+        _check(lib().lym_spinor_gamma5dot(self.h, other.h, ctypes.byref(re), ctypes.byref(im)))
+        return complex(re.value, im.value)
+
+    def difference_norm(self, other):
+        r = ctypes.c_double()
+        _check(lib().lym_spinor_difference_norm(self.h, other.h, ctypes.byref(r)))
+        return r.value
+
+    def axpy(self, a, x):
+        _check(lib().lym_spinor_axpy(self.h, complex(a).real, complex(a).imag, x.h))
+
+    def gamma5(self):
+        _check(lib().lym_spinor_gamma5(self.h))
+
+    def set_zero(self):
+        _check(lib().lym_spinor_set_zero(self.h))
+
+    def copy_from(self, other):
+        _check(lib().lym_spinor_copy(self.h, other.h))
+
+    def close(self):
+        if self.h:
+            lib().lym_spinor_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Operator:
+    def __init__(self, ctx, kind, power=1, kappa=0.125, csw=1.0, gamma5=True, gauge=None, name=None):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        if name is not None:
+            _check(lib().lym_op_create_by_name(ctx.h, name.encode(), power, kappa, csw, int(gamma5), ctypes.byref(h)))
+        else:
+            _check(lib().lym_op_create(ctx.h, kind, power, kappa, csw, int(gamma5), ctypes.byref(h)))
+        self.h = h
+        if gauge is not None:
+            self.set_gauge(gauge)
+
+    def set_gauge(self, g):
+        self.gauge = g
+        _check(lib().lym_op_set_gauge(self.h, g.h))
+
+    def set_kappa(self, k):
+        _check(lib().lym_op_set_kappa(self.h, k))
+
+    def set_csw(self, c):
+        _check(lib().lym_op_set_csw(self.h, c))
+
+    def set_gamma5(self, g):
+        _check(lib().lym_op_set_gamma5(self.h, int(g)))
+
+    def multiply(self, out, inp):
+        _check(lib().lym_op_multiply(self.h, out.h, inp.h))
+
+    def multiply_add(self, out, v1, v2, alpha):
+        a = complex(alpha)
+        _check(lib().lym_op_multiply_add(self.h, out.h, v1.h, v2.h, a.real, a.imag))
+
+    def close(self):
+        if self.h:
+            lib().lym_op_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Solver:
+    def __init__(self, ctx, kind, precision=None, max_steps=None, even_odd=True):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        _check(lib().lym_solver_create(ctx.h, kind, ctypes.byref(h)))
+        self.h = h
+        if precision is not None:
+            _check(lib().lym_solver_set_precision(self.h, precision))
+        if max_steps is not None:
+            _check(lib().lym_solver_set_max_steps(self.h, max_steps))
+        _check(lib().lym_solver_set_even_odd(self.h, int(even_odd)))
+
+    def solve(self, op, source, solution, guess=None):
+        info = SolveInfo()
+        _check(lib().lym_solver_solve(self.h, op.h, source.h, solution.h, guess.h if guess is not None else None,
+                                      ctypes.byref(info)))
+        return info
+
+    def close(self):
+        if self.h:
+            lib().lym_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,634 @@
+// api.cpp -- C ABI of the MI355X fermion-solve library (declared in include/lym.h).
+//
+// Objects mirror the reference's: a context holds the lattice geometry and one
+// HIP stream (default priority); a gauge holds the device links shared by any
+// number of operators; an operator holds kappa, csw, gamma5, power and its
+// clover blocks; a solver holds its work vectors and a device state block.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/lym.h"
+#include "lym_kernels.h"
+
+using lym::zd;
+
+namespace lym_internal {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+}  // namespace lym_internal
+
+using lym_internal::fail;
+using lym_internal::g_err;
+
+namespace {
+
+#define HIPCHECK(expr)                                                                      \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) return fail(LYM_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+#define CHECK_LAUNCH()                                                                        \
+    do {                                                                                      \
+        hipError_t e_ = hipGetLastError();                                                    \
+        if (e_ != hipSuccess) return fail(LYM_ERR_DEVICE, "kernel launch: %s", hipGetErrorString(e_)); \
+    } while (0)
+
+inline long half_n(const lym_ctx* c) { return 12L * c->g.stride; }
+inline long full_n(const lym_ctx* c) { return 24L * c->g.stride; }
+
+int dmalloc(void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(LYM_ERR_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    }
+    e = hipMemset(*p, 0, bytes);
+    if (e != hipSuccess) return fail(LYM_ERR_DEVICE, "hipMemset: %s", hipGetErrorString(e));
+    return LYM_OK;
+}
+
+int link_kind(const lym_gauge* g) {
+    if (g->real_links) return g->compression == 6 ? 3 : 2;
+    return g->compression == 12 ? 1 : 0;
+}
+
+int ensure_staging(lym_ctx* c, size_t bytes) {
+    if (c->staging_bytes >= bytes) return LYM_OK;
+    if (c->staging) (void)hipFree(c->staging);
+    c->staging = nullptr;
+    c->staging_bytes = 0;
+    int rc = dmalloc(&c->staging, bytes);
+    if (rc) return rc;
+    c->staging_bytes = bytes;
+    return LYM_OK;
+}
+
+// ---------------------------------------------------------------- operator internals
+bool op_has_clover(const lym_op* op) {
+    return (op->kind == LYM_OP_CLOVER || op->kind == LYM_OP_EO_CLOVER) && op->csw != 0.0;
+}
+
+// builds B (gauge), A and Ainv (op) when stale; force_blocks also for csw = 0
+int op_prepare(lym_op* op, bool force_blocks) {
+    lym_ctx* c = op->ctx;
+    if (!op->gauge) return fail(LYM_ERR_INVALID, "operator has no gauge field (call lym_op_set_gauge)");
+    bool need_blocks = force_blocks || op_has_clover(op) || op->kind == LYM_OP_EO_CLOVER;
+    if (!need_blocks || (!op->diag_dirty && op->A)) return LYM_OK;
+    lym_gauge* g = const_cast<lym_gauge*>(op->gauge);
+    const size_t blk = sizeof(zd) * 36 * (size_t)c->g.stride;
+    if (!g->B) {
+        int rc = dmalloc((void**)&g->B, 2 * blk);
+        if (rc) return rc;
+        lym::launch_clover_B(c->g, g->U, link_kind(g), reinterpret_cast<zd*>(g->B), c->stream);
+        CHECK_LAUNCH();
+    }
+    if (!op->A) {
+        int rc = dmalloc((void**)&op->A, 2 * blk);
+        if (rc) return rc;
+    }
+    if (!op->Ainv) {
+        int rc = dmalloc((void**)&op->Ainv, blk);
+        if (rc) return rc;
+    }
+    lym::launch_clover_A(c->g, reinterpret_cast<const zd*>(g->B), op->kappa * op->csw, reinterpret_cast<zd*>(op->A),
+                         c->stream);
+    CHECK_LAUNCH();
+    int* flag = reinterpret_cast<int*>(c->red_result) + 32;
+    HIPCHECK(hipMemsetAsync(flag, 0, sizeof(int), c->stream));
+    lym::launch_clover_inv(c->g, reinterpret_cast<const zd*>(op->A), reinterpret_cast<zd*>(op->Ainv), flag,
+                           c->stream);
+    CHECK_LAUNCH();
+    int hflag = 0;
+    HIPCHECK(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (hflag) return fail(LYM_ERR_SINGULAR, "%d clover blocks are singular", hflag);
+    op->diag_dirty = 0;
+    return LYM_OK;
+}
+
+lym::DslashParams base_params(const lym_op* op) {
+    lym::DslashParams a;
+    std::memset(&a, 0, sizeof(a));
+    a.g = op->ctx->g;
+    a.links = op->gauge->U;
+    a.A = reinterpret_cast<const zd*>(op->A);
+    a.Ainv = reinterpret_cast<const zd*>(op->Ainv);
+    a.kappa = op->kappa;
+    a.hop_sign = 1.0;
+    a.alpha = lym::mk(0.0, 0.0);
+    return a;
+}
+
+inline zd* par(zd* f, const lym_ctx* c, int p) { return f + (size_t)p * 12 * c->g.stride; }
+inline const zd* par(const zd* f, const lym_ctx* c, int p) { return f + (size_t)p * 12 * c->g.stride; }
+
+// full-lattice single power: out = Op in (+ alpha v2); Wilson or clover
+void apply_full1(lym_op* op, zd* out, const zd* in, const zd* v2, zd alpha, int gamma5, const int* skip) {
+    lym_ctx* c = op->ctx;
+    lym::DslashParams a = base_params(op);
+    for (int p = 0; p < 2; ++p) {
+        a.in[p] = par(in, c, p);
+        a.diag_in[p] = par(in, c, p);
+        a.out[p] = par(out, c, p);
+        a.v2[p] = v2 ? par(v2, c, p) : nullptr;
+    }
+    a.alpha = alpha;
+    a.gamma5 = gamma5;
+    a.parity = -1;
+    a.skip = skip;
+    lym::launch_dslash(a, link_kind(op->gauge), op_has_clover(op) ? 2 : 1, false, c->stream);
+}
+
+// Schur complement on odd half fields: out_o = gamma5^g (A_o in_o - D_oe Ainv_e D_eo in_o)
+void apply_schur1(lym_op* op, zd* out_o, const zd* in_o, int gamma5, const int* skip) {
+    lym_ctx* c = op->ctx;
+    int lk = link_kind(op->gauge);
+    lym::DslashParams a = base_params(op);
+    a.skip = skip;
+    a.parity = 0;
+    a.in[1] = in_o;
+    a.out[0] = reinterpret_cast<zd*>(op->tmp2);
+    lym::launch_dslash(a, lk, 0, true, c->stream);
+    lym::DslashParams b = base_params(op);
+    b.skip = skip;
+    b.parity = 1;
+    b.in[0] = reinterpret_cast<zd*>(op->tmp2);
+    b.diag_in[1] = in_o;
+    b.out[1] = out_o;
+    b.hop_sign = -1.0;
+    b.gamma5 = gamma5;
+    lym::launch_dslash(b, lk, op->csw != 0.0 ? 2 : 1, false, c->stream);
+}
+
+int ensure_op_scratch(lym_op* op) {
+    lym_ctx* c = op->ctx;
+    if (!op->tmp) {
+        int rc = dmalloc((void**)&op->tmp, sizeof(zd) * full_n(c));
+        if (rc) return rc;
+    }
+    if (!op->tmp2) {
+        int rc = dmalloc((void**)&op->tmp2, sizeof(zd) * half_n(c));
+        if (rc) return rc;
+    }
+    return LYM_OK;
+}
+
+// Op^power applied to a vector of the operator's natural domain:
+// full fields for Wilson/clover, odd half fields for the even-odd operator.
+void apply_power(lym_op* op, zd* out, const zd* in, const int* skip) {
+    if (op->kind == LYM_OP_EO_CLOVER) {
+        if (op->power == 1) {
+            apply_schur1(op, out, in, op->gamma5, skip);
+        } else {
+            zd* t = reinterpret_cast<zd*>(op->tmp);
+            apply_schur1(op, t, in, op->gamma5, skip);
+            apply_schur1(op, out, t, op->gamma5, skip);
+        }
+        return;
+    }
+    if (op->power == 1) {
+        apply_full1(op, out, in, nullptr, lym::mk(0.0, 0.0), op->gamma5, skip);
+    } else {
+        zd* t = reinterpret_cast<zd*>(op->tmp);
+        apply_full1(op, t, in, nullptr, lym::mk(0.0, 0.0), op->gamma5, skip);
+        apply_full1(op, out, t, nullptr, lym::mk(0.0, 0.0), op->gamma5, skip);
+    }
+}
+
+int check_op_ready(lym_op* op) {
+    if (!op) return fail(LYM_ERR_INVALID, "null operator");
+    int rc = op_prepare(op, false);
+    if (rc) return rc;
+    return ensure_op_scratch(op);
+}
+
+}  // namespace
+
+namespace lym_internal {
+int op_ready(lym_op* op, bool force_blocks) {
+    if (!op) return fail(LYM_ERR_INVALID, "null operator");
+    int rc = op_prepare(op, force_blocks);
+    if (rc) return rc;
+    return ensure_op_scratch(op);
+}
+void apply_schur(lym_op* op, zd* out_o, const zd* in_o, int gamma5, const int* skip) {
+    apply_schur1(op, out_o, in_o, gamma5, skip);
+}
+void apply_full(lym_op* op, zd* out, const zd* in, int gamma5, const int* skip) {
+    apply_full1(op, out, in, nullptr, lym::mk(0.0, 0.0), gamma5, skip);
+}
+int link_kind_of(const lym_gauge* g) { return link_kind(g); }
+}  // namespace lym_internal
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* lym_last_error(void) { return g_err.c_str(); }
+
+int lym_device_count(int* n) {
+    int k = 0;
+    hipError_t e = hipGetDeviceCount(&k);
+    if (e != hipSuccess) {
+        *n = 0;
+        return fail(LYM_ERR_DEVICE, "hipGetDeviceCount: %s", hipGetErrorString(e));
+    }
+    *n = k;
+    return LYM_OK;
+}
+
+int lym_ctx_create(const int dims[4], int rep, int bc_t, lym_ctx** out) {
+    if (!dims || !out) return fail(LYM_ERR_INVALID, "null argument");
+    *out = nullptr;
+    for (int i = 0; i < 4; ++i)
+        if (dims[i] < 2) return fail(LYM_ERR_INVALID, "lattice extent %d must be >= 2", dims[i]);
+    if (dims[3] % 2) return fail(LYM_ERR_INVALID, "Lt = %d must be even for the even-odd device layout", dims[3]);
+    if (rep != LYM_REP_SU3_FUND && rep != LYM_REP_SU2_ADJ) return fail(LYM_ERR_INVALID, "unknown representation %d", rep);
+    if (bc_t != 1 && bc_t != -1) return fail(LYM_ERR_INVALID, "bc_t must be +1 or -1");
+    long V = (long)dims[0] * dims[1] * dims[2] * dims[3];
+    if (V / 2 > (1L << 30)) return fail(LYM_ERR_INVALID, "lattice too large");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(LYM_ERR_DEVICE, "no HIP device available");
+    lym_ctx* c = new lym_ctx();
+    std::memset(c, 0, sizeof(*c));
+    for (int i = 0; i < 4; ++i) {
+        c->dims[i] = dims[i];
+        c->g.L[i] = dims[i];
+    }
+    c->V = V;
+    c->g.Lth = dims[3] / 2;
+    c->g.Vh = (int)(V / 2);
+    c->g.stride = (int)(((V / 2) + 63) / 64 * 64);
+    c->g.bc_t = bc_t;
+    c->rep = rep;
+    (void)hipGetDevice(&c->device);
+    int rc = LYM_OK;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = fail(LYM_ERR_DEVICE, "stream");
+    if (!rc) rc = dmalloc((void**)&c->red_partials, sizeof(double) * 8 * lym::RED_BLOCKS);
+    if (!rc) rc = dmalloc((void**)&c->red_result, 256);
+    if (!rc && hipHostMalloc((void**)&c->host_pinned, 4096, hipHostMallocDefault) != hipSuccess)
+        rc = fail(LYM_ERR_DEVICE, "hipHostMalloc");
+    if (!rc && (hipEventCreate(&c->ev_start) != hipSuccess || hipEventCreate(&c->ev_stop) != hipSuccess))
+        rc = fail(LYM_ERR_DEVICE, "hipEventCreate");
+    if (rc) {
+        lym_ctx_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return LYM_OK;
+}
+
+void lym_ctx_destroy(lym_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->staging) (void)hipFree(c->staging);
+    if (c->red_partials) (void)hipFree(c->red_partials);
+    if (c->red_result) (void)hipFree(c->red_result);
+    if (c->host_pinned) (void)hipHostFree(c->host_pinned);
+    if (c->ev_start) (void)hipEventDestroy(c->ev_start);
+    if (c->ev_stop) (void)hipEventDestroy(c->ev_stop);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int lym_ctx_synchronize(lym_ctx* c) {
+    if (!c) return fail(LYM_ERR_INVALID, "null context");
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return LYM_OK;
+}
+
+void* lym_ctx_stream(lym_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int lym_ctx_timer_start(lym_ctx* c) {
+    if (!c) return fail(LYM_ERR_INVALID, "null context");
+    HIPCHECK(hipEventRecord(c->ev_start, c->stream));
+    return LYM_OK;
+}
+
+int lym_ctx_timer_stop(lym_ctx* c, float* ms) {
+    if (!c || !ms) return fail(LYM_ERR_INVALID, "null argument");
+    HIPCHECK(hipEventRecord(c->ev_stop, c->stream));
+    HIPCHECK(hipEventSynchronize(c->ev_stop));
+    HIPCHECK(hipEventElapsedTime(ms, c->ev_start, c->ev_stop));
+    return LYM_OK;
+}
+
+// ---------------------------------------------------------------- gauge
+int lym_gauge_create(lym_ctx* c, const double* links, int folded, int storage, lym_gauge** out) {
+    if (!c || !links || !out) return fail(LYM_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (storage != LYM_LINKS_FULL && storage != LYM_LINKS_RECONSTRUCT) return fail(LYM_ERR_INVALID, "storage");
+    lym_gauge* g = new lym_gauge();
+    std::memset(g, 0, sizeof(*g));
+    g->ctx = c;
+    g->real_links = c->rep == LYM_REP_SU2_ADJ;
+    g->compression = g->real_links ? (storage ? 6 : 9) : (storage ? 12 : 18);
+    const size_t elem = g->real_links ? sizeof(double) : sizeof(zd);
+    const int ncomp = g->real_links ? g->compression : g->compression / 2;
+    const size_t host_bytes = (size_t)c->V * 4 * 9 * elem;
+    int rc = dmalloc(&g->U, (size_t)2 * 4 * ncomp * c->g.stride * elem);
+    if (!rc) rc = ensure_staging(c, host_bytes);
+    if (rc) {
+        lym_gauge_destroy(g);
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(c->staging, links, host_bytes, hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) {
+        lym_gauge_destroy(g);
+        return fail(LYM_ERR_DEVICE, "upload links: %s", hipGetErrorString(e));
+    }
+    lym::launch_links_to_eo(c->g, c->staging, g->real_links, ncomp, folded, g->U, c->stream);
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        lym_gauge_destroy(g);
+        return fail(LYM_ERR_DEVICE, "links layout: %s", hipGetErrorString(e));
+    }
+    *out = g;
+    return LYM_OK;
+}
+
+void lym_gauge_destroy(lym_gauge* g) {
+    if (!g) return;
+    if (g->ctx && g->ctx->stream) (void)hipStreamSynchronize(g->ctx->stream);
+    if (g->U) (void)hipFree(g->U);
+    if (g->B) (void)hipFree(g->B);
+    delete g;
+}
+
+// ---------------------------------------------------------------- spinors
+int lym_spinor_create(lym_ctx* c, lym_spinor** out) {
+    if (!c || !out) return fail(LYM_ERR_INVALID, "null argument");
+    *out = nullptr;
+    lym_spinor* s = new lym_spinor();
+    s->ctx = c;
+    s->d = nullptr;
+    int rc = dmalloc((void**)&s->d, sizeof(zd) * full_n(c));
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return LYM_OK;
+}
+
+void lym_spinor_destroy(lym_spinor* s) {
+    if (!s) return;
+    if (s->ctx && s->ctx->stream) (void)hipStreamSynchronize(s->ctx->stream);
+    if (s->d) (void)hipFree(s->d);
+    delete s;
+}
+
+int lym_spinor_upload(lym_spinor* s, const double* host) {
+    if (!s || !host) return fail(LYM_ERR_INVALID, "null argument");
+    lym_ctx* c = s->ctx;
+    size_t bytes = sizeof(zd) * 12 * (size_t)c->V;
+    int rc = ensure_staging(c, bytes);
+    if (rc) return rc;
+    HIPCHECK(hipMemcpyAsync(c->staging, host, bytes, hipMemcpyHostToDevice, c->stream));
+    lym::launch_spinor_to_eo(c->g, reinterpret_cast<const zd*>(c->staging), reinterpret_cast<zd*>(s->d), c->stream);
+    CHECK_LAUNCH();
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return LYM_OK;
+}
+
+int lym_spinor_download(const lym_spinor* s, double* host) {
+    if (!s || !host) return fail(LYM_ERR_INVALID, "null argument");
+    lym_ctx* c = s->ctx;
+    size_t bytes = sizeof(zd) * 12 * (size_t)c->V;
+    int rc = ensure_staging(c, bytes);
+    if (rc) return rc;
+    lym::launch_spinor_from_eo(c->g, reinterpret_cast<const zd*>(s->d), reinterpret_cast<zd*>(c->staging), c->stream);
+    CHECK_LAUNCH();
+    HIPCHECK(hipMemcpyAsync(host, c->staging, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return LYM_OK;
+}
+
+int lym_spinor_copy(lym_spinor* dst, const lym_spinor* src) {
+    if (!dst || !src || dst->ctx != src->ctx) return fail(LYM_ERR_INVALID, "spinors of different contexts");
+    lym::launch_copy(dst->d ? reinterpret_cast<zd*>(dst->d) : nullptr, reinterpret_cast<const zd*>(src->d),
+                     full_n(dst->ctx), dst->ctx->stream);
+    CHECK_LAUNCH();
+    return LYM_OK;
+}
+
+int lym_spinor_set_zero(lym_spinor* s) {
+    if (!s) return fail(LYM_ERR_INVALID, "null spinor");
+    HIPCHECK(hipMemsetAsync(s->d, 0, sizeof(zd) * full_n(s->ctx), s->ctx->stream));
+    return LYM_OK;
+}
+
+int lym_spinor_gamma5(lym_spinor* s) {
+    if (!s) return fail(LYM_ERR_INVALID, "null spinor");
+    lym::launch_scale_g5(reinterpret_cast<zd*>(s->d), 2, s->ctx->g.stride, s->ctx->stream);
+    CHECK_LAUNCH();
+    return LYM_OK;
+}
+
+static int read_result(lym_ctx* c, int n, double* out) {
+    HIPCHECK(hipMemcpyAsync(c->host_pinned, c->red_result, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; ++i) out[i] = c->host_pinned[i];
+    return LYM_OK;
+}
+
+int lym_spinor_squared_norm(const lym_spinor* s, double* out) {
+    if (!s || !out) return fail(LYM_ERR_INVALID, "null argument");
+    lym_ctx* c = s->ctx;
+    lym::launch_norm2(reinterpret_cast<const zd*>(s->d), full_n(c), c->red_partials, c->red_result, c->stream);
+    CHECK_LAUNCH();
+    return read_result(c, 1, out);
+}
+
+int lym_spinor_dot(const lym_spinor* a, const lym_spinor* b, double* re, double* im) {
+    if (!a || !b || !re || !im || a->ctx != b->ctx) return fail(LYM_ERR_INVALID, "bad arguments");
+    lym_ctx* c = a->ctx;
+    lym::launch_dot(reinterpret_cast<const zd*>(a->d), reinterpret_cast<const zd*>(b->d), full_n(c), c->red_partials,
+                    c->red_result, c->stream);
+    CHECK_LAUNCH();
+    double r[2];
+    int rc = read_result(c, 2, r);
+    *re = r[0];
+    *im = r[1];
+    return rc;
+}
+
+int lym_spinor_gamma5dot(const lym_spinor* a, const lym_spinor* b, double* re, double* im) {
+    if (!a || !b || !re || !im || a->ctx != b->ctx) return fail(LYM_ERR_INVALID, "bad arguments");
+    lym_ctx* c = a->ctx;
+    lym::launch_gamma5dot(reinterpret_cast<const zd*>(a->d), reinterpret_cast<const zd*>(b->d), full_n(c), 2,
+                          c->g.stride, c->red_partials, c->red_result, c->stream);
+    CHECK_LAUNCH();
+    double r[2];
+    int rc = read_result(c, 2, r);
+    *re = r[0];
+    *im = r[1];
+    return rc;
+}
+
+int lym_spinor_difference_norm(const lym_spinor* a, const lym_spinor* b, double* out) {
+    if (!a || !b || !out || a->ctx != b->ctx) return fail(LYM_ERR_INVALID, "bad arguments");
+    lym_ctx* c = a->ctx;
+    lym::launch_diffnorm(reinterpret_cast<const zd*>(a->d), reinterpret_cast<const zd*>(b->d), full_n(c),
+                         c->red_partials, c->red_result, c->stream);
+    CHECK_LAUNCH();
+    return read_result(c, 1, out);
+}
+
+int lym_spinor_axpy(lym_spinor* y, double are, double aim, const lym_spinor* x) {
+    if (!y || !x || y->ctx != x->ctx) return fail(LYM_ERR_INVALID, "bad arguments");
+    lym::launch_axpy(reinterpret_cast<zd*>(y->d), lym::mk(are, aim), reinterpret_cast<const zd*>(x->d),
+                     full_n(y->ctx), y->ctx->stream);
+    CHECK_LAUNCH();
+    return LYM_OK;
+}
+
+// ---------------------------------------------------------------- operators
+int lym_op_create(lym_ctx* c, int kind, unsigned power, double kappa, double csw, int gamma5, lym_op** out) {
+    if (!c || !out) return fail(LYM_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (kind < LYM_OP_WILSON || kind > LYM_OP_EO_CLOVER) return fail(LYM_ERR_INVALID, "unknown operator kind %d", kind);
+    if (power != 1 && power != 2) return fail(LYM_ERR_INVALID, "Power of the Dirac Wilson Operator not supported: %u", power);
+    if (kind == LYM_OP_EO_CLOVER)
+        for (int i = 0; i < 4; ++i)
+            if (c->dims[i] % 2) return fail(LYM_ERR_INVALID, "even-odd operator needs even lattice extents");
+    lym_op* op = new lym_op();
+    std::memset(op, 0, sizeof(*op));
+    op->ctx = c;
+    op->kind = kind;
+    op->power = power;
+    op->kappa = kappa;
+    op->csw = (kind == LYM_OP_CLOVER || kind == LYM_OP_EO_CLOVER) ? csw : 0.0;
+    op->gamma5 = gamma5 ? 1 : 0;
+    op->diag_dirty = 1;
+    *out = op;
+    return LYM_OK;
+}
+
+int lym_op_create_by_name(lym_ctx* c, const char* name, unsigned power, double kappa, double csw, int gamma5,
+                          lym_op** out) {
+    if (!name) return fail(LYM_ERR_INVALID, "null name");
+    int kind;
+    if (!std::strcmp(name, "DiracWilson")) kind = LYM_OP_WILSON;
+    else if (!std::strcmp(name, "BasicDiracWilson")) kind = LYM_OP_BASIC_WILSON;
+    else if (!std::strcmp(name, "Improved")) kind = LYM_OP_CLOVER;
+    else if (!std::strcmp(name, "EvenOddImproved")) kind = LYM_OP_EO_CLOVER;
+    else return fail(LYM_ERR_UNSUPPORTED, "Dirac Wilson Operator %s not supported!", name);
+    return lym_op_create(c, kind, power, kappa, csw, gamma5, out);
+}
+
+void lym_op_destroy(lym_op* op) {
+    if (!op) return;
+    if (op->ctx && op->ctx->stream) (void)hipStreamSynchronize(op->ctx->stream);
+    if (op->A) (void)hipFree(op->A);
+    if (op->Ainv) (void)hipFree(op->Ainv);
+    if (op->tmp) (void)hipFree(op->tmp);
+    if (op->tmp2) (void)hipFree(op->tmp2);
+    delete op;
+}
+
+int lym_op_set_gauge(lym_op* op, const lym_gauge* g) {
+    if (!op || !g || g->ctx != op->ctx) return fail(LYM_ERR_INVALID, "gauge field of another context");
+    op->gauge = g;
+    op->diag_dirty = 1;
+    return LYM_OK;
+}
+
+int lym_op_set_kappa(lym_op* op, double kappa) {
+    if (!op) return fail(LYM_ERR_INVALID, "null operator");
+    if (op->kappa != kappa) op->diag_dirty = 1;
+    op->kappa = kappa;
+    return LYM_OK;
+}
+
+int lym_op_set_csw(lym_op* op, double csw) {
+    if (!op) return fail(LYM_ERR_INVALID, "null operator");
+    if (op->kind != LYM_OP_CLOVER && op->kind != LYM_OP_EO_CLOVER)
+        return fail(LYM_ERR_INVALID, "csw only applies to clover operators");
+    if (op->csw != csw) op->diag_dirty = 1;
+    op->csw = csw;
+    return LYM_OK;
+}
+
+int lym_op_set_gamma5(lym_op* op, int gamma5) {
+    if (!op) return fail(LYM_ERR_INVALID, "null operator");
+    op->gamma5 = gamma5 ? 1 : 0;
+    return LYM_OK;
+}
+
+int lym_op_get_params(const lym_op* op, double* kappa, double* csw, int* gamma5, int* kind, unsigned* power) {
+    if (!op) return fail(LYM_ERR_INVALID, "null operator");
+    if (kappa) *kappa = op->kappa;
+    if (csw) *csw = op->csw;
+    if (gamma5) *gamma5 = op->gamma5;
+    if (kind) *kind = op->kind;
+    if (power) *power = op->power;
+    return LYM_OK;
+}
+
+int lym_op_multiply(lym_op* op, lym_spinor* out, const lym_spinor* in) {
+    int rc = check_op_ready(op);
+    if (rc) return rc;
+    if (!out || !in || out->ctx != op->ctx || in->ctx != op->ctx) return fail(LYM_ERR_INVALID, "bad spinors");
+    if (out == in) return fail(LYM_ERR_INVALID, "multiply needs distinct input and output");
+    lym_ctx* c = op->ctx;
+    zd* o = reinterpret_cast<zd*>(out->d);
+    const zd* i = reinterpret_cast<const zd*>(in->d);
+    if (op->kind == LYM_OP_EO_CLOVER) {
+        // even sites copied through, odd sites get the Schur complement
+        HIPCHECK(hipMemcpyAsync(o, i, sizeof(zd) * half_n(c), hipMemcpyDeviceToDevice, c->stream));
+        apply_power(op, par(o, c, 1), par(i, c, 1), nullptr);
+    } else {
+        apply_power(op, o, i, nullptr);
+    }
+    CHECK_LAUNCH();
+    return LYM_OK;
+}
+
+int lym_op_multiply_add(lym_op* op, lym_spinor* out, const lym_spinor* v1, const lym_spinor* v2, double are,
+                        double aim) {
+    int rc = check_op_ready(op);
+    if (rc) return rc;
+    if (!out || !v1 || !v2) return fail(LYM_ERR_INVALID, "null spinor");
+    if (out == v1 || out == v2) return fail(LYM_ERR_INVALID, "multiplyAdd output must not alias an input");
+    lym_ctx* c = op->ctx;
+    zd* o = reinterpret_cast<zd*>(out->d);
+    const zd* a = reinterpret_cast<const zd*>(v1->d);
+    const zd* b = reinterpret_cast<const zd*>(v2->d);
+    zd al = lym::mk(are, aim);
+    if (op->kind == LYM_OP_EO_CLOVER) {
+        if (op->power == 1)
+            return fail(LYM_ERR_UNSUPPORTED, "EvenOddImprovedDiracWilsonOperator::multiplyAdd not implemented");
+        // SquareEvenOdd multiplyAdd: Q^2 v1 + alpha v2 on all sites (even part of Q^2 is the identity)
+        HIPCHECK(hipMemcpyAsync(o, a, sizeof(zd) * half_n(c), hipMemcpyDeviceToDevice, c->stream));
+        apply_power(op, par(o, c, 1), par(a, c, 1), nullptr);
+        lym::launch_axpy(o, al, b, full_n(c), c->stream);
+    } else if (op->power == 1) {
+        apply_full1(op, o, a, b, al, op->gamma5, nullptr);
+    } else {
+        zd* t = reinterpret_cast<zd*>(op->tmp);
+        apply_full1(op, t, a, nullptr, lym::mk(0.0, 0.0), op->gamma5, nullptr);
+        apply_full1(op, o, t, b, al, op->gamma5, nullptr);
+    }
+    CHECK_LAUNCH();
+    return LYM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,517 @@
+// blas.hip -- spinor BLAS-1 and the device-resident CG / BiCGStab state machines.
+//
+// Reductions (AlgebraUtils::squaredNorm, dot, gamma5dot, differenceNorm,
+// source/algebra_utils/AlgebraUtils.h:17-27,182-254) run on a fixed grid of
+// RED_BLOCKS workgroups: each wavefront folds its lanes with xor-shuffles,
+// the workgroup folds its waves through LDS and writes one partial per
+// workgroup; a single-workgroup second pass folds the partials in a fixed
+// tree.  The order of every addition is therefore fixed, so a result does not
+// depend on what else runs on the device (the reference sums in long double;
+// here every sum is FP64).
+//
+// The solver kernels keep all scalars (alpha, beta, rho, omega, |r|^2) in a
+// device state block, so an iteration needs no host round trip; every kernel
+// returns at once when the state says `done`, which makes the iteration
+// count and stopping point those of the reference loops
+// (source/inverters/ConjugateGradient.cpp:33-67, BiConjugateGradient.cpp:364-501)
+// however many iterations the host has queued.
+#include "lym_kernels.h"
+
+namespace lym {
+
+constexpr int RB_THREADS = 256;
+
+template <int NV>
+__device__ __forceinline__ void wave_sum(double v[NV]) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] += __shfl_xor(v[k], off, 64);
+}
+
+// block of RB_THREADS threads -> partials[k * RED_BLOCKS + blockIdx.x]
+template <int NV>
+__device__ __forceinline__ void block_store(double v[NV], double* partials) {
+    __shared__ double sh[NV][RB_THREADS / 64];
+    wave_sum<NV>(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) sh[k][w] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double s = 0;
+#pragma unroll
+            for (int j = 0; j < RB_THREADS / 64; ++j) s += sh[k][j];
+            partials[k * RED_BLOCKS + blockIdx.x] = s;
+        }
+    }
+}
+
+// one block of RED_BLOCKS threads folds the partials; thread 0 returns totals
+template <int NV>
+__device__ __forceinline__ void final_sum(const double* partials, double out[NV]) {
+    __shared__ double sh[NV][RED_BLOCKS / 64];
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = partials[k * RED_BLOCKS + threadIdx.x];
+    wave_sum<NV>(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) sh[k][w] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double s = 0;
+#pragma unroll
+            for (int j = 0; j < RED_BLOCKS / 64; ++j) s += sh[k][j];
+            out[k] = s;
+        }
+}
+
+#define GRID_LOOP(i, n) for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (long)gridDim.x * blockDim.x)
+
+// ---------------------------------------------------------------- plain reductions
+__global__ __launch_bounds__(RB_THREADS) void k_norm2(const zd* __restrict__ a, long n, double* partials) {
+    double v[1] = {0};
+    GRID_LOOP(i, n) {
+        zd x = a[i];
+        v[0] += x.re * x.re + x.im * x.im;
+    }
+    block_store<1>(v, partials);
+}
+
+__global__ __launch_bounds__(RB_THREADS) void k_dot(const zd* __restrict__ a, const zd* __restrict__ b, long n,
+                                                    double* partials) {
+    double v[2] = {0, 0};
+    GRID_LOOP(i, n) {
+        zd x = a[i], y = b[i];
+        v[0] += x.re * y.re + x.im * y.im;
+        v[1] += x.re * y.im - x.im * y.re;
+    }
+    block_store<2>(v, partials);
+}
+
+__global__ __launch_bounds__(RB_THREADS) void k_gamma5dot(const zd* __restrict__ a, const zd* __restrict__ b, long n,
+                                                          int stride, double* partials) {
+    double v[2] = {0, 0};
+    GRID_LOOP(i, n) {
+        zd x = a[i], y = b[i];
+        double s = ((i / stride) % 12) < 6 ? 1.0 : -1.0;
+        v[0] += s * (x.re * y.re + x.im * y.im);
+        v[1] += s * (x.re * y.im - x.im * y.re);
+    }
+    block_store<2>(v, partials);
+}
+
+__global__ __launch_bounds__(RB_THREADS) void k_diffnorm(const zd* __restrict__ a, const zd* __restrict__ b, long n,
+                                                         double* partials) {
+    double v[1] = {0};
+    GRID_LOOP(i, n) {
+        zd d = a[i] - b[i];
+        v[0] += d.re * d.re + d.im * d.im;
+    }
+    block_store<1>(v, partials);
+}
+
+template <int NV>
+__global__ __launch_bounds__(RED_BLOCKS) void k_final(const double* partials, double* result) {
+    double out[NV];
+    final_sum<NV>(partials, out);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NV; ++k) result[k] = out[k];
+}
+
+void launch_norm2(const zd* a, long n, double* partials, double* result, hipStream_t st) {
+    hipLaunchKernelGGL(k_norm2, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, a, n, partials);
+    hipLaunchKernelGGL(k_final<1>, dim3(1), dim3(RED_BLOCKS), 0, st, partials, result);
+}
+void launch_dot(const zd* a, const zd* b, long n, double* partials, double* result, hipStream_t st) {
+    hipLaunchKernelGGL(k_dot, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, a, b, n, partials);
+    hipLaunchKernelGGL(k_final<2>, dim3(1), dim3(RED_BLOCKS), 0, st, partials, result);
+}
+void launch_gamma5dot(const zd* a, const zd* b, long n, int nparity, int stride, double* partials, double* result,
+                      hipStream_t st) {
+    (void)nparity;
+    hipLaunchKernelGGL(k_gamma5dot, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, a, b, n, stride, partials);
+    hipLaunchKernelGGL(k_final<2>, dim3(1), dim3(RED_BLOCKS), 0, st, partials, result);
+}
+void launch_diffnorm(const zd* a, const zd* b, long n, double* partials, double* result, hipStream_t st) {
+    hipLaunchKernelGGL(k_diffnorm, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, a, b, n, partials);
+    hipLaunchKernelGGL(k_final<1>, dim3(1), dim3(RED_BLOCKS), 0, st, partials, result);
+}
+
+// ---------------------------------------------------------------- elementwise
+static inline int ew_blocks(long n) {
+    long b = (n + 255) / 256;
+    return (int)(b < 8192 ? b : 8192);
+}
+
+__global__ void k_axpy(zd* __restrict__ y, zd a, const zd* __restrict__ x, long n) {
+    GRID_LOOP(i, n) { y[i] = y[i] + a * x[i]; }
+}
+__global__ void k_xpay(zd* __restrict__ y, const zd* __restrict__ x, zd a, long n) {
+    GRID_LOOP(i, n) { y[i] = x[i] + a * y[i]; }
+}
+__global__ void k_g5(zd* __restrict__ y, long n, int stride) {
+    GRID_LOOP(i, n) {
+        if (((i / stride) % 12) >= 6) y[i] = -y[i];
+    }
+}
+__global__ void k_copy(zd* __restrict__ y, const zd* __restrict__ x, long n) {
+    GRID_LOOP(i, n) { y[i] = x[i]; }
+}
+__global__ void k_sub(zd* __restrict__ o, const zd* __restrict__ a, const zd* __restrict__ b, long n) {
+    GRID_LOOP(i, n) { o[i] = a[i] - b[i]; }
+}
+
+void launch_axpy(zd* y, zd a, const zd* x, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_axpy, dim3(ew_blocks(n)), dim3(256), 0, st, y, a, x, n);
+}
+void launch_xpay(zd* y, const zd* x, zd a, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_xpay, dim3(ew_blocks(n)), dim3(256), 0, st, y, x, a, n);
+}
+void launch_scale_g5(zd* y, int nparity, int stride, hipStream_t st) {
+    long n = (long)nparity * 12 * stride;
+    hipLaunchKernelGGL(k_g5, dim3(ew_blocks(n)), dim3(256), 0, st, y, n, stride);
+}
+void launch_copy(zd* y, const zd* x, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_copy, dim3(ew_blocks(n)), dim3(256), 0, st, y, x, n);
+}
+void launch_sub(zd* o, const zd* a, const zd* b, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_sub, dim3(ew_blocks(n)), dim3(256), 0, st, o, a, b, n);
+}
+
+// ---------------------------------------------------------------- CG
+__global__ __launch_bounds__(RED_BLOCKS) void k_cg_init_final(CGState* s, const double* partials, double eps,
+                                                              int max_steps) {
+    double out[1];
+    final_sum<1>(partials, out);
+    if (threadIdx.x == 0) {
+        s->rr = out[0];
+        s->rr_new = out[0];
+        s->eps = eps;
+        s->max_steps = max_steps;
+        s->step = 0;
+        s->converged = 0;
+        s->done = max_steps <= 0 ? 1 : 0;
+        s->alpha_re = s->alpha_im = s->beta = 0;
+    }
+}
+
+void launch_cg_init(CGState* s, const zd* r, long n, double* partials, double eps, int max_steps, hipStream_t st) {
+    hipLaunchKernelGGL(k_norm2, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, r, n, partials);
+    hipLaunchKernelGGL(k_cg_init_final, dim3(1), dim3(RED_BLOCKS), 0, st, s, partials, eps, max_steps);
+}
+
+__global__ __launch_bounds__(RB_THREADS) void k_cg_dot(const CGState* s, const zd* __restrict__ p,
+                                                       const zd* __restrict__ Ap, long n, double* partials) {
+    if (s->done) return;
+    double v[2] = {0, 0};
+    GRID_LOOP(i, n) {
+        zd x = p[i], y = Ap[i];
+        v[0] += x.re * y.re + x.im * y.im;
+        v[1] += x.re * y.im - x.im * y.re;
+    }
+    block_store<2>(v, partials);
+}
+
+// alpha = |r|^2 / <p, A p>  (complex, ConjugateGradient.cpp:36)
+__global__ __launch_bounds__(RED_BLOCKS) void k_cg_alpha_final(CGState* s, const double* partials) {
+    if (s->done) return;
+    double out[2];
+    final_sum<2>(partials, out);
+    if (threadIdx.x == 0) {
+        double den = out[0] * out[0] + out[1] * out[1];
+        s->alpha_re = s->rr * out[0] / den;
+        s->alpha_im = -s->rr * out[1] / den;
+    }
+}
+
+void launch_cg_alpha(CGState* s, const zd* p, const zd* Ap, long n, double* partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_dot, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, s, p, Ap, n, partials);
+    hipLaunchKernelGGL(k_cg_alpha_final, dim3(1), dim3(RED_BLOCKS), 0, st, s, partials);
+}
+
+// x += alpha p, r -= alpha Ap, partial |r|^2
+__global__ __launch_bounds__(RB_THREADS) void k_cg_update(const CGState* s, zd* __restrict__ x, zd* __restrict__ r,
+                                                          const zd* __restrict__ p, const zd* __restrict__ Ap, long n,
+                                                          double* partials) {
+    if (s->done) return;
+    const zd al = mk(s->alpha_re, s->alpha_im);
+    double v[1] = {0};
+    GRID_LOOP(i, n) {
+        x[i] = x[i] + al * p[i];
+        zd rn = r[i] - al * Ap[i];
+        r[i] = rn;
+        v[0] += rn.re * rn.re + rn.im * rn.im;
+    }
+    block_store<1>(v, partials);
+}
+
+__global__ __launch_bounds__(RED_BLOCKS) void k_cg_update_final(CGState* s, const double* partials) {
+    if (s->done) return;
+    double out[1];
+    final_sum<1>(partials, out);
+    if (threadIdx.x == 0) {
+        double rn = out[0];
+        s->rr_new = rn;
+        if (rn < s->eps) {
+            s->done = 1;
+            s->converged = 1;
+        } else {
+            s->beta = rn / s->rr;
+            s->rr = rn;
+            s->step += 1;
+            if (s->step >= s->max_steps) s->done = 1;
+        }
+    }
+}
+
+void launch_cg_update(CGState* s, zd* x, zd* r, const zd* p, const zd* Ap, long n, double* partials,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_update, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, s, x, r, p, Ap, n, partials);
+    hipLaunchKernelGGL(k_cg_update_final, dim3(1), dim3(RED_BLOCKS), 0, st, s, partials);
+}
+
+__global__ void k_cg_p(const CGState* s, zd* __restrict__ p, const zd* __restrict__ r, long n) {
+    if (s->done) return;
+    const double b = s->beta;
+    GRID_LOOP(i, n) { p[i] = r[i] + b * p[i]; }
+}
+
+void launch_cg_p(CGState* s, zd* p, const zd* r, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_p, dim3(ew_blocks(n)), dim3(256), 0, st, s, p, r, n);
+}
+
+// ---------------------------------------------------------------- BiCGStab
+__device__ __forceinline__ zd zdiv(zd a, zd b) {
+    double den = b.re * b.re + b.im * b.im;
+    return mk((a.re * b.re + a.im * b.im) / den, (a.im * b.re - a.re * b.im) / den);
+}
+
+__global__ void k_bicg_init(BiCGState* s, double precision, int max_steps) {
+    s->rho_re = 1; s->rho_im = 0;
+    s->alpha_re = 1; s->alpha_im = 0;
+    s->omega_re = 1; s->omega_im = 0;
+    s->rhon_re = s->rhon_im = s->beta_re = s->beta_im = 0;
+    s->norm = 0;
+    s->precision = precision;
+    s->max_steps = max_steps;
+    s->step = 0;
+    s->converged = 0;
+    s->breakdown = 0;
+    s->copy_b = 0;
+    s->done = max_steps <= 0 ? 1 : 0;
+}
+
+void launch_bicg_init(BiCGState* s, double precision, int max_steps, hipStream_t st) {
+    hipLaunchKernelGGL(k_bicg_init, dim3(1), dim3(1), 0, st, s, precision, max_steps);
+}
+
+__global__ void k_bicg_setup(zd* __restrict__ r, zd* __restrict__ rh, zd* __restrict__ p, zd* __restrict__ nu,
+                             const zd* __restrict__ b, const zd* __restrict__ Ax, long n) {
+    GRID_LOOP(i, n) {
+        zd bb = b[i], ax = Ax[i];
+        r[i] = bb - ax;
+        rh[i] = bb + ax;
+        p[i] = mk(0.0, 0.0);
+        nu[i] = mk(0.0, 0.0);
+    }
+}
+
+void launch_bicg_setup(zd* r, zd* rh, zd* p, zd* nu, const zd* b, const zd* Ax, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_bicg_setup, dim3(ew_blocks(n)), dim3(256), 0, st, r, rh, p, nu, b, Ax, n);
+}
+
+template <class S>
+__global__ __launch_bounds__(RB_THREADS) void k_state_dot(const S* s, const zd* __restrict__ a,
+                                                          const zd* __restrict__ b, long n, double* partials) {
+    if (s->done) return;
+    double v[2] = {0, 0};
+    GRID_LOOP(i, n) {
+        zd x = a[i], y = b[i];
+        v[0] += x.re * y.re + x.im * y.im;
+        v[1] += x.re * y.im - x.im * y.re;
+    }
+    block_store<2>(v, partials);
+}
+
+// rho_next = <rh, r>; breakdown if zero; beta = (rho_next / rho)(alpha / omega)
+__global__ __launch_bounds__(RED_BLOCKS) void k_bicg_rho_final(BiCGState* s, const double* partials) {
+    if (s->done) return;
+    double out[2];
+    final_sum<2>(partials, out);
+    if (threadIdx.x == 0) {
+        zd rn = mk(out[0], out[1]);
+        s->rhon_re = rn.re;
+        s->rhon_im = rn.im;
+        if (rn.re * rn.re + rn.im * rn.im == 0.0) {
+            s->done = 1;
+            s->breakdown = 1;
+            s->converged = 0;
+            return;
+        }
+        zd beta = zdiv(rn, mk(s->rho_re, s->rho_im)) * zdiv(mk(s->alpha_re, s->alpha_im), mk(s->omega_re, s->omega_im));
+        s->beta_re = beta.re;
+        s->beta_im = beta.im;
+    }
+}
+
+void launch_bicg_rho(BiCGState* s, const zd* rh, const zd* r, long n, double* partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_state_dot<BiCGState>, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, s, rh, r, n, partials);
+    hipLaunchKernelGGL(k_bicg_rho_final, dim3(1), dim3(RED_BLOCKS), 0, st, s, partials);
+}
+
+// p = r + beta (p - omega nu)
+__global__ void k_bicg_p(const BiCGState* s, zd* __restrict__ p, const zd* __restrict__ r, const zd* __restrict__ nu,
+                         long n) {
+    if (s->done) return;
+    const zd beta = mk(s->beta_re, s->beta_im), om = mk(s->omega_re, s->omega_im);
+    GRID_LOOP(i, n) { p[i] = r[i] + beta * (p[i] - om * nu[i]); }
+}
+
+void launch_bicg_p(BiCGState* s, zd* p, const zd* r, const zd* nu, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_bicg_p, dim3(ew_blocks(n)), dim3(256), 0, st, s, p, r, nu, n);
+}
+
+__global__ __launch_bounds__(RED_BLOCKS) void k_bicg_alpha_final(BiCGState* s, const double* partials) {
+    if (s->done) return;
+    double out[2];
+    final_sum<2>(partials, out);
+    if (threadIdx.x == 0) {
+        zd a = zdiv(mk(s->rhon_re, s->rhon_im), mk(out[0], out[1]));
+        s->alpha_re = a.re;
+        s->alpha_im = a.im;
+    }
+}
+
+void launch_bicg_alpha(BiCGState* s, const zd* rh, const zd* nu, long n, double* partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_state_dot<BiCGState>, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, s, rh, nu, n, partials);
+    hipLaunchKernelGGL(k_bicg_alpha_final, dim3(1), dim3(RED_BLOCKS), 0, st, s, partials);
+}
+
+__global__ void k_bicg_s(const BiCGState* s, zd* __restrict__ sv, const zd* __restrict__ r, const zd* __restrict__ nu,
+                         long n) {
+    if (s->done) return;
+    const zd al = mk(s->alpha_re, s->alpha_im);
+    GRID_LOOP(i, n) { sv[i] = r[i] - al * nu[i]; }
+}
+
+void launch_bicg_s(BiCGState* s, zd* sv, const zd* r, const zd* nu, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_bicg_s, dim3(ew_blocks(n)), dim3(256), 0, st, s, sv, r, nu, n);
+}
+
+__global__ __launch_bounds__(RB_THREADS) void k_bicg_ts(const BiCGState* s, const zd* __restrict__ t,
+                                                        const zd* __restrict__ sv, long n, double* partials) {
+    if (s->done) return;
+    double v[4] = {0, 0, 0, 0};
+    GRID_LOOP(i, n) {
+        zd x = t[i], y = sv[i];
+        v[0] += x.re * y.re + x.im * y.im;
+        v[1] += x.re * y.im - x.im * y.re;
+        v[2] += x.re * x.re + x.im * x.im;
+    }
+    block_store<4>(v, partials);
+}
+
+// omega = <t,s>/<t,t>; <t,t> == 0 -> solution = source (BiConjugateGradient.cpp:449-458)
+__global__ __launch_bounds__(RED_BLOCKS) void k_bicg_omega_final(BiCGState* s, const double* partials) {
+    if (s->done) return;
+    double out[4];
+    final_sum<4>(partials, out);
+    if (threadIdx.x == 0) {
+        if (out[2] == 0.0) {
+            s->copy_b = 1;
+            s->done = 1;
+            s->converged = 1;
+            return;
+        }
+        s->omega_re = out[0] / out[2];
+        s->omega_im = out[1] / out[2];
+    }
+}
+
+void launch_bicg_omega(BiCGState* s, const zd* t, const zd* sv, long n, double* partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_bicg_ts, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, s, t, sv, n, partials);
+    hipLaunchKernelGGL(k_bicg_omega_final, dim3(1), dim3(RED_BLOCKS), 0, st, s, partials);
+}
+
+// x += alpha p + omega s, r = s - omega t, partial |r|^2
+__global__ __launch_bounds__(RB_THREADS) void k_bicg_update(const BiCGState* s, zd* __restrict__ x,
+                                                            zd* __restrict__ r, const zd* __restrict__ p,
+                                                            const zd* __restrict__ sv, const zd* __restrict__ t,
+                                                            const zd* __restrict__ b, long n, double* partials) {
+    if (s->copy_b) {
+        GRID_LOOP(i, n) { x[i] = b[i]; }
+        return;
+    }
+    if (s->done) return;
+    const zd al = mk(s->alpha_re, s->alpha_im), om = mk(s->omega_re, s->omega_im);
+    double v[1] = {0};
+    GRID_LOOP(i, n) {
+        zd sv_i = sv[i];
+        x[i] = x[i] + al * p[i] + om * sv_i;
+        zd rn = sv_i - om * t[i];
+        r[i] = rn;
+        v[0] += rn.re * rn.re + rn.im * rn.im;
+    }
+    block_store<1>(v, partials);
+}
+
+__global__ __launch_bounds__(RED_BLOCKS) void k_bicg_update_final(BiCGState* s, const double* partials) {
+    if (s->done) return;
+    double out[1];
+    final_sum<1>(partials, out);
+    if (threadIdx.x == 0) {
+        s->norm = out[0];
+        if (out[0] < s->precision) {
+            s->done = 1;
+            s->converged = 1;
+            return;
+        }
+        s->rho_re = s->rhon_re;
+        s->rho_im = s->rhon_im;
+        s->step += 1;
+        if (s->step >= s->max_steps) s->done = 1;
+    }
+}
+
+void launch_bicg_update(BiCGState* s, zd* x, zd* r, const zd* p, const zd* sv, const zd* t, const zd* b, long n,
+                        double* partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_bicg_update, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, s, x, r, p, sv, t, b, n, partials);
+    hipLaunchKernelGGL(k_bicg_update_final, dim3(1), dim3(RED_BLOCKS), 0, st, s, partials);
+}
+
+}  // namespace lym
+
+namespace lym {
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__global__ void k_fill_uniform(zd* x, long n, int stride, int Vh, uint64_t seed) {
+    GRID_LOOP(i, n) {
+        if ((int)(i % stride) >= Vh) {
+            x[i] = mk(0.0, 0.0);
+        } else {
+            uint64_t a = mix64(seed ^ mix64(2 * (uint64_t)i)), b = mix64(seed ^ mix64(2 * (uint64_t)i + 1));
+            x[i] = mk((a >> 11) * 0x1.0p-53, (b >> 11) * 0x1.0p-53);
+        }
+    }
+}
+
+void launch_fill_uniform(zd* x, int nparity, int stride, int Vh, uint64_t seed, hipStream_t st) {
+    long n = (long)nparity * 12 * stride;
+    hipLaunchKernelGGL(k_fill_uniform, dim3(ew_blocks(n)), dim3(256), 0, st, x, n, stride, Vh, seed);
+}
+
+}  // namespace lym

@@ -1,0 +1,295 @@
+// clover.hip -- clover field strength, site-diagonal blocks and their inverses.
+//
+// F_k = (Q_k - Q_k^dagger)/8 from the four clover leaves of each plane
+// k = (01, 02, 03, 12, 13, 23) (source/dirac_operators/ImprovedDiracWilsonOperator.cpp:502-523),
+// no trace subtraction.  The kappa-independent part B of the clover term is
+// stored as two packed hermitian 6x6 blocks per site (the chiral combination of
+// ImprovedDiracWilsonOperator.cpp:200-243); the operator's diagonal blocks are
+// A = 1 + kappa csw gamma5 B, and R_ee^-1 is their blockwise inverse
+// (EvenOddImprovedDiracWilsonOperator.cpp:285-317 inverts the full 12x12 instead).
+// Links are the unsigned ones: each leaf that crosses the time boundary holds
+// two time links, so the antiperiodic sign cancels in F.
+#include "lym_device.h"
+
+namespace lym {
+
+struct M3 {
+    zd a[9];  // row-major
+};
+
+__device__ __forceinline__ M3 m3mul(const M3& A, const M3& B) {
+    M3 C;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            zd s = A.a[3 * i] * B.a[j];
+            cfma(s, A.a[3 * i + 1], B.a[3 + j]);
+            cfma(s, A.a[3 * i + 2], B.a[6 + j]);
+            C.a[3 * i + j] = s;
+        }
+    return C;
+}
+
+__device__ __forceinline__ M3 m3dag(const M3& A) {
+    M3 C;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) C.a[3 * i + j] = conj(A.a[3 * j + i]);
+    return C;
+}
+
+template <int LK>
+__device__ __forceinline__ M3 get_link(const Geom& g, const void* links, const int c[4], int mu) {
+    int p = (c[0] + c[1] + c[2] + c[3]) & 1;
+    int idx = eo_index(g, c[0], c[1], c[2], c[3]);
+    LinkMat<LK> L;
+    load_link<LK>(links, g.stride, p, mu, idx, L);
+    M3 m;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        if constexpr (LK <= 1) m.a[k] = L.m[k];
+        else m.a[k] = mk(L.m[k], 0.0);
+    }
+    return m;
+}
+
+__device__ __forceinline__ void shift(const Geom& g, const int c[4], int mu, int d, int o[4]) {
+    for (int k = 0; k < 4; ++k) o[k] = c[k];
+    int v = c[mu] + d;
+    o[mu] = v < 0 ? v + g.L[mu] : (v >= g.L[mu] ? v - g.L[mu] : v);
+}
+
+// F for plane (m, n) at coordinates c
+template <int LK>
+__device__ M3 field_strength(const Geom& g, const void* links, const int c[4], int m, int n) {
+    int xm[4], xn[4], xmn[4], xpm[4], xpn[4], xnpm[4], xmpn[4];
+    shift(g, c, m, -1, xm);
+    shift(g, c, n, -1, xn);
+    shift(g, xm, n, -1, xmn);
+    shift(g, c, m, +1, xpm);
+    shift(g, c, n, +1, xpn);
+    shift(g, xn, m, +1, xnpm);
+    shift(g, xm, n, +1, xmpn);
+    M3 Um_x = get_link<LK>(g, links, c, m), Un_x = get_link<LK>(g, links, c, n);
+    M3 Um_xm = get_link<LK>(g, links, xm, m), Un_xm = get_link<LK>(g, links, xm, n);
+    M3 Q;
+    {
+        M3 l = m3mul(m3mul(m3dag(Um_xm), m3dag(get_link<LK>(g, links, xmn, n))),
+                     m3mul(get_link<LK>(g, links, xmn, m), get_link<LK>(g, links, xn, n)));
+        Q = l;
+    }
+    {
+        M3 Un_xn = get_link<LK>(g, links, xn, n);
+        M3 l = m3mul(m3mul(m3dag(Un_xn), get_link<LK>(g, links, xn, m)),
+                     m3mul(get_link<LK>(g, links, xnpm, n), m3dag(Um_x)));
+        for (int k = 0; k < 9; ++k) Q.a[k] = Q.a[k] + l.a[k];
+    }
+    {
+        M3 l = m3mul(m3mul(Um_x, get_link<LK>(g, links, xpm, n)),
+                     m3mul(m3dag(get_link<LK>(g, links, xpn, m)), m3dag(Un_x)));
+        for (int k = 0; k < 9; ++k) Q.a[k] = Q.a[k] + l.a[k];
+    }
+    {
+        M3 l = m3mul(m3mul(Un_x, m3dag(get_link<LK>(g, links, xmpn, m))), m3mul(m3dag(Un_xm), Um_xm));
+        for (int k = 0; k < 9; ++k) Q.a[k] = Q.a[k] + l.a[k];
+    }
+    M3 F;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) F.a[3 * i + j] = 0.125 * (Q.a[3 * i + j] - conj(Q.a[3 * j + i]));
+    return F;
+}
+
+__device__ __forceinline__ zd ti(zd z) { return mulph<1>(z); }
+
+// store one hermitian 6x6 (full, row-major h[6][6]) packed
+__device__ __forceinline__ void store_herm6(zd* base, int stride, int idx, const zd h[36]) {
+    for (int k = 0; k < 3; ++k) base[(size_t)k * stride + idx] = mk(h[(2 * k) * 7].re, h[(2 * k + 1) * 7].re);
+    int k = 3;
+    for (int i = 1; i < 6; ++i)
+        for (int j = 0; j < i; ++j) {
+            base[(size_t)k * stride + idx] = h[6 * i + j];
+            ++k;
+        }
+}
+
+__device__ __forceinline__ void load_herm6(const zd* base, int stride, int idx, zd h[36]) {
+    for (int k = 0; k < 3; ++k) {
+        zd d = base[(size_t)k * stride + idx];
+        h[(2 * k) * 7] = mk(d.re, 0.0);
+        h[(2 * k + 1) * 7] = mk(d.im, 0.0);
+    }
+    int k = 3;
+    for (int i = 1; i < 6; ++i)
+        for (int j = 0; j < i; ++j) {
+            zd a = base[(size_t)k * stride + idx];
+            h[6 * i + j] = a;
+            h[6 * j + i] = conj(a);
+            ++k;
+        }
+}
+
+template <int LK>
+__global__ __launch_bounds__(128) void k_clover_B(Geom g, const void* links, zd* B) {
+    const int nbp = (g.Vh + 127) / 128;
+    int p = blockIdx.x / nbp;
+    int idx = (blockIdx.x % nbp) * 128 + threadIdx.x;
+    if (idx >= g.Vh) return;
+    int c[4];
+    eo_coords(g, p, idx, c);
+    const int pm[6] = {0, 0, 0, 1, 1, 2}, pn[6] = {1, 2, 3, 2, 3, 3};
+    zd hu[36], hl[36];
+    for (int k = 0; k < 36; ++k) hu[k] = hl[k] = mk(0.0, 0.0);
+    for (int pl = 0; pl < 6; ++pl) {
+        M3 F = field_strength<LK>(g, links, c, pm[pl], pn[pl]);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                zd f = F.a[3 * i + j];
+                // upper block: rows/cols 3*s + colour, s = spin 0, 1
+                // lower block: s = spin 2, 3 (local 0, 1)
+                switch (pl) {
+                    case 0:  // F0: i(-F0) on (0,0), i(+F0) on (1,1); lower i(+F0) (2,2), i(-F0) (3,3)
+                        hu[6 * i + j] = hu[6 * i + j] - ti(f);
+                        hu[6 * (3 + i) + 3 + j] = hu[6 * (3 + i) + 3 + j] + ti(f);
+                        hl[6 * i + j] = hl[6 * i + j] + ti(f);
+                        hl[6 * (3 + i) + 3 + j] = hl[6 * (3 + i) + 3 + j] - ti(f);
+                        break;
+                    case 5:  // F5: +i on (0,0), -i on (1,1); lower +i (2,2), -i (3,3)
+                        hu[6 * i + j] = hu[6 * i + j] + ti(f);
+                        hu[6 * (3 + i) + 3 + j] = hu[6 * (3 + i) + 3 + j] - ti(f);
+                        hl[6 * i + j] = hl[6 * i + j] + ti(f);
+                        hl[6 * (3 + i) + 3 + j] = hl[6 * (3 + i) + 3 + j] - ti(f);
+                        break;
+                    case 1:  // F1: +1 (0,1), -1 (1,0); lower -1 (2,3), +1 (3,2)
+                        hu[6 * i + 3 + j] = hu[6 * i + 3 + j] + f;
+                        hu[6 * (3 + i) + j] = hu[6 * (3 + i) + j] - f;
+                        hl[6 * i + 3 + j] = hl[6 * i + 3 + j] - f;
+                        hl[6 * (3 + i) + j] = hl[6 * (3 + i) + j] + f;
+                        break;
+                    case 4:  // F4: +1 (0,1), -1 (1,0); lower +1 (2,3), -1 (3,2)
+                        hu[6 * i + 3 + j] = hu[6 * i + 3 + j] + f;
+                        hu[6 * (3 + i) + j] = hu[6 * (3 + i) + j] - f;
+                        hl[6 * i + 3 + j] = hl[6 * i + 3 + j] + f;
+                        hl[6 * (3 + i) + j] = hl[6 * (3 + i) + j] - f;
+                        break;
+                    case 2:  // F2: +i on all four off-diagonal spin blocks
+                        hu[6 * i + 3 + j] = hu[6 * i + 3 + j] + ti(f);
+                        hu[6 * (3 + i) + j] = hu[6 * (3 + i) + j] + ti(f);
+                        hl[6 * i + 3 + j] = hl[6 * i + 3 + j] + ti(f);
+                        hl[6 * (3 + i) + j] = hl[6 * (3 + i) + j] + ti(f);
+                        break;
+                    default:  // F3: -i upper off-diagonal, +i lower off-diagonal
+                        hu[6 * i + 3 + j] = hu[6 * i + 3 + j] - ti(f);
+                        hu[6 * (3 + i) + j] = hu[6 * (3 + i) + j] - ti(f);
+                        hl[6 * i + 3 + j] = hl[6 * i + 3 + j] + ti(f);
+                        hl[6 * (3 + i) + j] = hl[6 * (3 + i) + j] + ti(f);
+                        break;
+                }
+            }
+    }
+    zd* base = B + (size_t)p * 36 * g.stride;
+    store_herm6(base, g.stride, idx, hu);
+    store_herm6(base + (size_t)18 * g.stride, g.stride, idx, hl);
+}
+
+// A = 1 + kc gamma5 B: upper block 1 + kc B_u, lower block 1 - kc B_l
+__global__ void k_clover_A(Geom g, const zd* __restrict__ B, double kc, zd* __restrict__ A) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long n = 2L * 36 * g.stride;
+    if (i >= n) return;
+    int comp = (int)((i / g.stride) % 36);
+    double s = comp < 18 ? kc : -kc;
+    int k = comp % 18;
+    zd b = B[i];
+    zd a = mk(s * b.re, s * b.im);
+    if (k < 3) {
+        // diagonal pairs
+        a.re += 1.0;
+        a.im += 1.0;
+    }
+    A[i] = a;
+}
+
+// blockwise inverse on parity-0 sites: Gauss-Jordan with partial pivoting
+__global__ __launch_bounds__(64) void k_clover_inv(Geom g, const zd* __restrict__ A, zd* __restrict__ Ainv,
+                                                   int* flag) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.Vh) return;
+    for (int blk = 0; blk < 2; ++blk) {
+        zd M[36], R[36];
+        load_herm6(A + (size_t)18 * blk * g.stride, g.stride, idx, M);
+        for (int k = 0; k < 36; ++k) R[k] = mk(k % 7 == 0 ? 1.0 : 0.0, 0.0);
+        bool ok = true;
+        for (int col = 0; col < 6; ++col) {
+            int piv = col;
+            double best = M[6 * col + col].re * M[6 * col + col].re + M[6 * col + col].im * M[6 * col + col].im;
+            for (int r = col + 1; r < 6; ++r) {
+                double v = M[6 * r + col].re * M[6 * r + col].re + M[6 * r + col].im * M[6 * r + col].im;
+                if (v > best) { best = v; piv = r; }
+            }
+            if (best == 0.0) { ok = false; break; }
+            if (piv != col)
+                for (int j = 0; j < 6; ++j) {
+                    zd t = M[6 * col + j]; M[6 * col + j] = M[6 * piv + j]; M[6 * piv + j] = t;
+                    t = R[6 * col + j]; R[6 * col + j] = R[6 * piv + j]; R[6 * piv + j] = t;
+                }
+            zd d = M[6 * col + col];
+            double den = d.re * d.re + d.im * d.im;
+            zd inv = mk(d.re / den, -d.im / den);
+            for (int j = 0; j < 6; ++j) { M[6 * col + j] = inv * M[6 * col + j]; R[6 * col + j] = inv * R[6 * col + j]; }
+            for (int r = 0; r < 6; ++r) {
+                if (r == col) continue;
+                zd f = M[6 * r + col];
+                for (int j = 0; j < 6; ++j) {
+                    M[6 * r + j] = M[6 * r + j] - f * M[6 * col + j];
+                    R[6 * r + j] = R[6 * r + j] - f * R[6 * col + j];
+                }
+            }
+        }
+        if (!ok) atomicAdd(flag, 1);
+        store_herm6(Ainv + (size_t)18 * blk * g.stride, g.stride, idx, R);
+    }
+}
+
+// out = blocks * in on one parity (half fields), optional gamma5
+__global__ void k_clover_apply(Geom g, const zd* __restrict__ blocks, const zd* __restrict__ in, zd* __restrict__ out,
+                               int gamma5) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.Vh) return;
+    zd s[12], t[12];
+    load_spinor(in, g.stride, idx, s);
+    herm6(blocks, g.stride, idx, s, t);
+    herm6(blocks + (size_t)18 * g.stride, g.stride, idx, s + 6, t + 6);
+    for (int k = 0; k < 12; ++k) {
+        zd v = (gamma5 && k >= 6) ? -t[k] : t[k];
+        out[(size_t)k * g.stride + idx] = v;
+    }
+}
+
+void launch_clover_B(const Geom& g, const void* links, int link_kind, zd* B, hipStream_t st) {
+    int nb = 2 * ((g.Vh + 127) / 128);
+    switch (link_kind) {
+        case 0: hipLaunchKernelGGL(k_clover_B<0>, dim3(nb), dim3(128), 0, st, g, links, B); break;
+        case 1: hipLaunchKernelGGL(k_clover_B<1>, dim3(nb), dim3(128), 0, st, g, links, B); break;
+        case 2: hipLaunchKernelGGL(k_clover_B<2>, dim3(nb), dim3(128), 0, st, g, links, B); break;
+        default: hipLaunchKernelGGL(k_clover_B<3>, dim3(nb), dim3(128), 0, st, g, links, B); break;
+    }
+}
+
+void launch_clover_A(const Geom& g, const zd* B, double kc, zd* A, hipStream_t st) {
+    long n = 2L * 36 * g.stride;
+    hipLaunchKernelGGL(k_clover_A, dim3((n + 255) / 256), dim3(256), 0, st, g, B, kc, A);
+}
+
+void launch_clover_inv(const Geom& g, const zd* A, zd* Ainv, int* flag, hipStream_t st) {
+    hipLaunchKernelGGL(k_clover_inv, dim3((g.Vh + 63) / 64), dim3(64), 0, st, g, A, Ainv, flag);
+}
+
+void launch_clover_apply(const Geom& g, const zd* blocks, const zd* in, zd* out, int gamma5, hipStream_t st) {
+    hipLaunchKernelGGL(k_clover_apply, dim3((g.Vh + 255) / 256), dim3(256), 0, st, g, blocks, in, out, gamma5);
+}
+
+}  // namespace lym

@@ -1,0 +1,96 @@
+// lym_kernels.h -- kernel parameter blocks and host-side launchers.
+#pragma once
+
+#include "lym_common.h"
+
+namespace lym {
+
+typedef cplx<double> zd;
+
+struct DslashParams {
+    Geom g;
+    const zd* in[2];       // per parity: the field read at neighbours is in[1 - p]
+    const zd* diag_in[2];  // per parity: the field under the site-diagonal term
+    zd* out[2];
+    const zd* v2[2];       // optional axpy operand
+    const void* links;
+    const zd* A;           // [2][36][stride] clover blocks of D
+    const zd* Ainv;        // [36][stride] inverse blocks (even sites)
+    double kappa;
+    double hop_sign;
+    zd alpha;
+    int gamma5;
+    int parity;            // 0 or 1: that parity only; -1: both
+    int nblocks_per_parity;
+    const int* skip;       // optional device flag: nonzero -> the launch does nothing (solver done)
+};
+
+// link_kind: 0 complex 18, 1 complex 12, 2 real 9, 3 real 6; diag: 0 none, 1 identity, 2 clover
+void launch_dslash(DslashParams a, int link_kind, int diag, bool inv, hipStream_t st);
+
+// ---------------------------------------------------------------- layout (layout.hip)
+void launch_spinor_to_eo(const Geom& g, const zd* aos, zd* eo, hipStream_t st);
+void launch_spinor_from_eo(const Geom& g, const zd* eo, zd* aos, hipStream_t st);
+// links_aos: complex [V][4][3x3 col-major] (real_links = 0) or real (real_links = 1)
+void launch_links_to_eo(const Geom& g, const void* links_aos, int real_links, int ncomp, int unfold_bc, void* eo,
+                        hipStream_t st);
+
+// ---------------------------------------------------------------- clover (clover.hip)
+void launch_clover_B(const Geom& g, const void* links, int link_kind, zd* B, hipStream_t st);
+void launch_clover_A(const Geom& g, const zd* B, double kc, zd* A, hipStream_t st);
+// Ainv on parity 0 from A; returns via *flag (device int) nonzero when a block is singular
+void launch_clover_inv(const Geom& g, const zd* A, zd* Ainv, int* flag, hipStream_t st);
+void launch_clover_apply(const Geom& g, const zd* blocks, const zd* in, zd* out, int gamma5, hipStream_t st);
+
+// ---------------------------------------------------------------- BLAS-1 (blas.hip)
+constexpr int RED_BLOCKS = 512;
+
+// reductions over n complex elements; results land in `result` (device) as doubles
+void launch_norm2(const zd* a, long n, double* partials, double* result, hipStream_t st);
+void launch_dot(const zd* a, const zd* b, long n, double* partials, double* result, hipStream_t st);
+void launch_gamma5dot(const zd* a, const zd* b, long half_stride_n, int nparity, int stride, double* partials,
+                      double* result, hipStream_t st);
+void launch_diffnorm(const zd* a, const zd* b, long n, double* partials, double* result, hipStream_t st);
+
+void launch_axpy(zd* y, zd a, const zd* x, long n, hipStream_t st);         // y += a x
+void launch_xpay(zd* y, const zd* x, zd a, long n, hipStream_t st);         // y = x + a y
+void launch_scale_g5(zd* y, int nparity, int stride, hipStream_t st);       // lower spins negated
+void launch_copy(zd* y, const zd* x, long n, hipStream_t st);
+void launch_sub(zd* out, const zd* a, const zd* b, long n, hipStream_t st); // out = a - b
+
+// ---------------------------------------------------------------- solver state machines
+struct CGState {
+    int done, converged, step, max_steps;
+    double rr, rr_new, alpha_re, alpha_im, beta, eps;
+};
+
+struct BiCGState {
+    int done, converged, step, max_steps, breakdown, copy_b;
+    double rho_re, rho_im, rhon_re, rhon_im;
+    double alpha_re, alpha_im, omega_re, omega_im, beta_re, beta_im;
+    double norm, precision;
+};
+
+void launch_cg_init(CGState* s, const zd* r, long n, double* partials, double eps, int max_steps, hipStream_t st);
+void launch_cg_alpha(CGState* s, const zd* p, const zd* Ap, long n, double* partials, hipStream_t st);
+void launch_cg_update(CGState* s, zd* x, zd* r, const zd* p, const zd* Ap, long n, double* partials,
+                      hipStream_t st);
+void launch_cg_p(CGState* s, zd* p, const zd* r, long n, hipStream_t st);
+
+void launch_bicg_init(BiCGState* s, double precision, int max_steps, hipStream_t st);
+// r = b - Ax, rh = b + Ax, p = nu = 0
+void launch_bicg_setup(zd* r, zd* rh, zd* p, zd* nu, const zd* b, const zd* Ax, long n, hipStream_t st);
+void launch_bicg_rho(BiCGState* s, const zd* rh, const zd* r, long n, double* partials, hipStream_t st);
+void launch_bicg_p(BiCGState* s, zd* p, const zd* r, const zd* nu, long n, hipStream_t st);
+void launch_bicg_alpha(BiCGState* s, const zd* rh, const zd* nu, long n, double* partials, hipStream_t st);
+void launch_bicg_s(BiCGState* s, zd* sv, const zd* r, const zd* nu, long n, hipStream_t st);
+void launch_bicg_omega(BiCGState* s, const zd* t, const zd* sv, long n, double* partials, hipStream_t st);
+void launch_bicg_update(BiCGState* s, zd* x, zd* r, const zd* p, const zd* sv, const zd* t, const zd* b, long n,
+                        double* partials, hipStream_t st);
+
+}  // namespace lym
+
+namespace lym {
+// x = uniform [0,1) complex on real sites, zero on padding (BiCGStab start for a zero source)
+void launch_fill_uniform(zd* x, int nparity, int stride, int Vh, uint64_t seed, hipStream_t st);
+}  // namespace lym

@@ -1,0 +1,342 @@
+// solvers.cpp -- CG, BiCGStab and the even-odd BiCGStab driver on the device.
+//
+// ConjugateGradient::solve        source/inverters/ConjugateGradient.cpp:10-74
+// BiConjugateGradient::solve      source/inverters/BiConjugateGradient.cpp:321-507
+// PreconditionedBiCGStab::solve   source/inverters/PreconditionedBiCGStab.cpp:33-100
+//
+// The host queues iterations in chunks on the context stream and reads the
+// device state block once per chunk; kernels of iterations past the stopping
+// point return immediately (see blas.hip), so the result is the one the
+// reference's loop stops at.  Even-odd operators are solved on the odd half
+// only: on the even half the reference's Q^2 is the identity, its residual
+// starts and stays zero there, and the even part of the solution is the even
+// part of the source.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/lym.h"
+#include "lym_kernels.h"
+
+using lym::zd;
+
+// provided by api.cpp
+namespace lym_internal {
+int fail(int code, const char* fmt, ...);
+int op_ready(lym_op* op, bool force_blocks);
+void apply_schur(lym_op* op, zd* out_o, const zd* in_o, int gamma5, const int* skip);
+void apply_full(lym_op* op, zd* out, const zd* in, int gamma5, const int* skip);
+int link_kind_of(const lym_gauge* g);
+}  // namespace lym_internal
+
+using lym_internal::fail;
+
+struct lym_solver {
+    lym_ctx* ctx;
+    int kind;
+    double precision;
+    unsigned max_steps;
+    int even_odd;
+    zd* w[8];
+    void* state;
+    void* host_state;
+};
+
+namespace {
+
+#define HIPCHECK(expr)                                                                             \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) return fail(LYM_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+inline long half_n(const lym_ctx* c) { return 12L * c->g.stride; }
+inline long full_n(const lym_ctx* c) { return 24L * c->g.stride; }
+inline zd* par(zd* f, const lym_ctx* c, int p) { return f + (size_t)p * 12 * c->g.stride; }
+inline const zd* par(const zd* f, const lym_ctx* c, int p) { return f + (size_t)p * 12 * c->g.stride; }
+
+// A linear map for a Krylov solver: the operator to the power `power` on full
+// fields, or the Schur complement (gamma5 flag given) to `power` on odd halves.
+struct LinOp {
+    lym_op* op;
+    bool half;
+    int gamma5;
+    unsigned power;
+    zd* scratch;  // one vector of the map's domain
+    void apply(zd* out, const zd* in, const int* skip) const {
+        if (power == 1) {
+            if (half) lym_internal::apply_schur(op, out, in, gamma5, skip);
+            else lym_internal::apply_full(op, out, in, gamma5, skip);
+        } else {
+            if (half) {
+                lym_internal::apply_schur(op, scratch, in, gamma5, skip);
+                lym_internal::apply_schur(op, out, scratch, gamma5, skip);
+            } else {
+                lym_internal::apply_full(op, scratch, in, gamma5, skip);
+                lym_internal::apply_full(op, out, scratch, gamma5, skip);
+            }
+        }
+    }
+};
+
+int ensure_work(lym_solver* s) {
+    if (s->w[0]) return LYM_OK;
+    const size_t bytes = sizeof(zd) * full_n(s->ctx);
+    for (int i = 0; i < 8; ++i) {
+        if (hipMalloc((void**)&s->w[i], bytes) != hipSuccess) return fail(LYM_ERR_NOMEM, "solver work vectors");
+        if (hipMemset(s->w[i], 0, bytes) != hipSuccess) return fail(LYM_ERR_DEVICE, "hipMemset");
+    }
+    if (hipMalloc(&s->state, 256) != hipSuccess) return fail(LYM_ERR_NOMEM, "solver state");
+    if (hipHostMalloc(&s->host_state, 256, hipHostMallocDefault) != hipSuccess)
+        return fail(LYM_ERR_DEVICE, "hipHostMalloc");
+    return LYM_OK;
+}
+
+// iterations to queue before the next look at the state
+unsigned next_chunk(double r0, double r_now, unsigned done_iters, double target, unsigned left) {
+    unsigned c = 8;
+    if (done_iters > 0 && r_now > 0 && r0 > r_now && target > 0) {
+        double rate = std::log(r_now / r0) / done_iters;  // < 0
+        double need = std::log(target / r_now) / rate;
+        if (need > 0) c = (unsigned)std::ceil(need * 0.9) + 1;
+    }
+    if (c < 2) c = 2;
+    if (c > 64) c = 64;
+    if (c > left) c = left;
+    return c;
+}
+
+double read_norm2(lym_ctx* c, const zd* v, long n) {
+    lym::launch_norm2(v, n, c->red_partials, c->red_result, c->stream);
+    double out = 0;
+    (void)hipMemcpyAsync(c->host_pinned, c->red_result, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    out = c->host_pinned[0];
+    return out;
+}
+
+// CG on L (ConjugateGradient.cpp:10-74): vectors of length n
+int run_cg(lym_solver* s, const LinOp& L, const zd* b, zd* xout, const zd* x0, long n, lym_solve_info* info) {
+    lym_ctx* c = s->ctx;
+    hipStream_t st = c->stream;
+    zd *x = s->w[0], *r = s->w[1], *p = s->w[2], *Ap = s->w[3];
+    auto* S = reinterpret_cast<lym::CGState*>(s->state);
+    auto* H = reinterpret_cast<lym::CGState*>(s->host_state);
+    lym::launch_copy(x, x0 ? x0 : b, n, st);
+    L.apply(Ap, x, nullptr);
+    lym::launch_sub(r, b, Ap, n, st);
+    lym::launch_copy(p, r, n, st);
+    lym::launch_cg_init(S, r, n, c->red_partials, s->precision, (int)s->max_steps, st);
+    HIPCHECK(hipMemcpyAsync(H, S, sizeof(lym::CGState), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    const double r0 = H->rr;
+    unsigned queued = 0;
+    while (!H->done && queued <= s->max_steps) {
+        unsigned k = next_chunk(r0, H->rr, (unsigned)H->step, s->precision, s->max_steps + 1 - queued);
+        for (unsigned i = 0; i < k; ++i) {
+            L.apply(Ap, p, &S->done);
+            lym::launch_cg_alpha(S, p, Ap, n, c->red_partials, st);
+            lym::launch_cg_update(S, x, r, p, Ap, n, c->red_partials, st);
+            lym::launch_cg_p(S, p, r, n, st);
+        }
+        queued += k;
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(H, S, sizeof(lym::CGState), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+    lym::launch_copy(xout, x, n, st);
+    HIPCHECK(hipGetLastError());
+    info->converged = H->converged;
+    info->steps = H->converged ? (unsigned)H->step : s->max_steps;
+    info->last_error = H->rr_new;
+    info->applies = 1 + (H->converged ? (unsigned)H->step + 1 : s->max_steps);
+    return LYM_OK;
+}
+
+// BiCGStab on L (BiConjugateGradient.cpp:321-507); extra_norm is added to
+// |b|^2 for the zero-source test when b is the odd half of a longer vector.
+int run_bicgstab(lym_solver* s, const LinOp& L, const zd* b, zd* xout, const zd* x0, long n, double extra_norm,
+                 lym_solve_info* info) {
+    lym_ctx* c = s->ctx;
+    hipStream_t st = c->stream;
+    zd *x = s->w[0], *r = s->w[1], *rh = s->w[2], *p = s->w[3], *nu = s->w[4], *sv = s->w[5], *t = s->w[6];
+    auto* S = reinterpret_cast<lym::BiCGState*>(s->state);
+    auto* H = reinterpret_cast<lym::BiCGState*>(s->host_state);
+    if (x0) {
+        lym::launch_copy(x, x0, n, st);
+    } else if (read_norm2(c, b, n) + extra_norm > s->precision) {
+        lym::launch_copy(x, b, n, st);
+    } else {
+        // the reference draws a time-seeded random start here; a fixed seed is used instead
+        lym::launch_fill_uniform(x, (int)(n / half_n(c)), c->g.stride, c->g.Vh, 0x5eedull, st);
+    }
+    L.apply(t, x, nullptr);
+    lym::launch_bicg_setup(r, rh, p, nu, b, t, n, st);
+    lym::launch_bicg_init(S, s->precision, (int)s->max_steps, st);
+    HIPCHECK(hipMemcpyAsync(H, S, sizeof(lym::BiCGState), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    const double r0 = read_norm2(c, r, n);
+    double rnow = r0;
+    unsigned queued = 0;
+    while (!H->done && queued <= s->max_steps) {
+        unsigned k = next_chunk(r0, rnow, (unsigned)H->step, s->precision, s->max_steps + 1 - queued);
+        for (unsigned i = 0; i < k; ++i) {
+            lym::launch_bicg_rho(S, rh, r, n, c->red_partials, st);
+            lym::launch_bicg_p(S, p, r, nu, n, st);
+            L.apply(nu, p, &S->done);
+            lym::launch_bicg_alpha(S, rh, nu, n, c->red_partials, st);
+            lym::launch_bicg_s(S, sv, r, nu, n, st);
+            L.apply(t, sv, &S->done);
+            lym::launch_bicg_omega(S, t, sv, n, c->red_partials, st);
+            lym::launch_bicg_update(S, x, r, p, sv, t, b, n, c->red_partials, st);
+        }
+        queued += k;
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(H, S, sizeof(lym::BiCGState), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        rnow = H->norm;
+    }
+    lym::launch_copy(xout, x, n, st);
+    HIPCHECK(hipGetLastError());
+    info->converged = H->converged;
+    unsigned iters = H->done ? (unsigned)H->step + (H->converged ? 1 : 0) : s->max_steps;
+    info->steps = H->converged ? (unsigned)H->step : (H->breakdown ? (unsigned)H->step : s->max_steps);
+    info->last_error = H->norm;
+    info->applies = 1 + 2 * iters;
+    return LYM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lym_solver_create(lym_ctx* c, int kind, lym_solver** out) {
+    if (!c || !out) return fail(LYM_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (kind < LYM_SOLVER_CG || kind > LYM_SOLVER_PRECOND_BICGSTAB) return fail(LYM_ERR_INVALID, "unknown solver %d", kind);
+    lym_solver* s = new lym_solver();
+    std::memset(s, 0, sizeof(*s));
+    s->ctx = c;
+    s->kind = kind;
+    // defaults: ConjugateGradient.cpp:6 and Solver.h:10
+    s->precision = kind == LYM_SOLVER_CG ? 1e-11 : 1e-10;
+    s->max_steps = kind == LYM_SOLVER_CG ? 3000 : 1000;
+    s->even_odd = 1;
+    *out = s;
+    return LYM_OK;
+}
+
+void lym_solver_destroy(lym_solver* s) {
+    if (!s) return;
+    if (s->ctx && s->ctx->stream) (void)hipStreamSynchronize(s->ctx->stream);
+    for (int i = 0; i < 8; ++i)
+        if (s->w[i]) (void)hipFree(s->w[i]);
+    if (s->state) (void)hipFree(s->state);
+    if (s->host_state) (void)hipHostFree(s->host_state);
+    delete s;
+}
+
+int lym_solver_set_precision(lym_solver* s, double precision) {
+    if (!s) return fail(LYM_ERR_INVALID, "null solver");
+    s->precision = precision;
+    return LYM_OK;
+}
+
+int lym_solver_set_max_steps(lym_solver* s, unsigned max_steps) {
+    if (!s) return fail(LYM_ERR_INVALID, "null solver");
+    s->max_steps = max_steps;
+    return LYM_OK;
+}
+
+int lym_solver_set_even_odd(lym_solver* s, int use) {
+    if (!s) return fail(LYM_ERR_INVALID, "null solver");
+    s->even_odd = use ? 1 : 0;
+    return LYM_OK;
+}
+
+int lym_solver_solve(lym_solver* s, lym_op* op, const lym_spinor* source, lym_spinor* solution,
+                     const lym_spinor* guess, lym_solve_info* info) {
+    if (!s || !op || !source || !solution) return fail(LYM_ERR_INVALID, "null argument");
+    lym_ctx* c = s->ctx;
+    if (op->ctx != c || source->ctx != c || solution->ctx != c || (guess && guess->ctx != c))
+        return fail(LYM_ERR_INVALID, "objects of different contexts");
+    lym_solve_info local;
+    if (!info) info = &local;
+    std::memset(info, 0, sizeof(*info));
+    bool eo_driver = s->kind == LYM_SOLVER_PRECOND_BICGSTAB && s->even_odd && op->power == 1 &&
+                     (op->kind == LYM_OP_CLOVER || op->kind == LYM_OP_EO_CLOVER) && c->dims[0] % 2 == 0 &&
+                     c->dims[1] % 2 == 0 && c->dims[2] % 2 == 0 && c->dims[3] % 2 == 0;
+    int rc = lym_internal::op_ready(op, eo_driver);
+    if (rc) return rc;
+    rc = ensure_work(s);
+    if (rc) return rc;
+    hipStream_t st = c->stream;
+    HIPCHECK(hipEventRecord(c->ev_start, st));
+    const zd* b = reinterpret_cast<const zd*>(source->d);
+    zd* x = reinterpret_cast<zd*>(solution->d);
+    const zd* x0 = guess ? reinterpret_cast<const zd*>(guess->d) : nullptr;
+    const bool half = op->kind == LYM_OP_EO_CLOVER;
+    zd* scratch = s->w[7];
+
+    if (eo_driver) {
+        // PreconditionedBiCGStab.cpp:56-86 with the operator's kappa and csw
+        lym::DslashParams a;
+        zd* te = s->w[7];
+        zd* rhs = s->w[7] + half_n(c);
+        zd* src = s->w[1];
+        zd* xo = par(x, c, 1);
+        lym::launch_clover_apply(c->g, reinterpret_cast<const zd*>(op->Ainv), par(b, c, 0), te, 0, st);
+        std::memset(&a, 0, sizeof(a));
+        a.g = c->g;
+        a.links = op->gauge->U;
+        a.kappa = op->kappa;
+        a.hop_sign = -1.0;
+        a.parity = 1;
+        a.in[0] = te;
+        a.diag_in[1] = par(b, c, 1);
+        a.out[1] = src;
+        lym::launch_dslash(a, lym_internal::link_kind_of(op->gauge), 1, false, st);
+        lym::launch_scale_g5(src, 1, c->g.stride, st);
+        lym_internal::apply_schur(op, rhs, src, 1, nullptr);
+        double be = read_norm2(c, par(b, c, 0), half_n(c));
+        // inner BiCGStab on Qhat^2 with gamma5 = 1 (the driver's own EO operators);
+        // it uses w[0..6], the Q^2 intermediate lives in the operator's scratch
+        LinOp L{op, true, 1, 2, reinterpret_cast<zd*>(op->tmp)};
+        rc = run_bicgstab(s, L, rhs, xo, x0 ? par(x0, c, 1) : nullptr, half_n(c), be, info);
+        if (rc) return rc;
+        std::memset(&a, 0, sizeof(a));
+        a.g = c->g;
+        a.links = op->gauge->U;
+        a.Ainv = reinterpret_cast<const zd*>(op->Ainv);
+        a.kappa = op->kappa;
+        a.hop_sign = -1.0;
+        a.parity = 0;
+        a.in[1] = xo;
+        a.diag_in[0] = par(b, c, 0);
+        a.out[0] = par(x, c, 0);
+        lym::launch_dslash(a, lym_internal::link_kind_of(op->gauge), 1, true, st);
+    } else {
+        LinOp L{op, half, op->gamma5, op->power, half ? reinterpret_cast<zd*>(op->tmp) : scratch};
+        long n = half ? half_n(c) : full_n(c);
+        const zd* bb = half ? par(b, c, 1) : b;
+        zd* xx = half ? par(x, c, 1) : x;
+        const zd* g0 = x0 ? (half ? par(x0, c, 1) : x0) : nullptr;
+        if (s->kind == LYM_SOLVER_CG) rc = run_cg(s, L, bb, xx, g0, n, info);
+        else rc = run_bicgstab(s, L, bb, xx, g0, n, half ? read_norm2(c, b, half_n(c)) : 0.0, info);
+        if (rc) return rc;
+        if (half) HIPCHECK(hipMemcpyAsync(x, b, sizeof(zd) * half_n(c), hipMemcpyDeviceToDevice, st));
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventRecord(c->ev_stop, st));
+    HIPCHECK(hipEventSynchronize(c->ev_stop));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev_start, c->ev_stop);
+    info->seconds = ms * 1e-3;
+    return LYM_OK;
+}
+
+}  // extern "C"
