@@ -98,6 +98,13 @@ def gpu_bench(args):
 
     if not args.no_solve:
         res["solves"] = solve_bench(lym, ctx, sets, V)
+    if args.calibrate:
+        # k_copy over a full 16^3x32 spinor: 24 * stride * 16 B read and written
+        g, op, src, out, eo = sets[0]
+        for _ in range(20):
+            out.copy_from(src)
+        ctx.synchronize()
+        log("calibrate: 20 copies of %d bytes each" % (24 * (V // 2) * 16))
     return res
 
 
@@ -178,6 +185,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run 20 plain spinor copies (known bytes) to scale the HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X; "

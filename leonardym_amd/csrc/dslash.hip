@@ -20,6 +20,19 @@
 // (PreconditionedBiCGStab.cpp:56-86).
 #include "lym_device.h"
 
+// Launch shape of the stencil (swept on the device; defaults are the measured best)
+#ifndef LYM_DSLASH_THREADS
+#define LYM_DSLASH_THREADS 256
+#endif
+#ifndef LYM_XCD_REMAP
+#define LYM_XCD_REMAP 1
+#endif
+#ifdef LYM_WAVES_PER_EU
+#define LYM_OCC __attribute__((amdgpu_waves_per_eu(LYM_WAVES_PER_EU, LYM_WAVES_PER_EU)))
+#else
+#define LYM_OCC
+#endif
+
 namespace lym {
 
 template <int MU>
@@ -153,12 +166,16 @@ __device__ __forceinline__ void hopping(const Geom& g, const void* links, const 
 }
 
 template <int LK, int DIAG, bool INV>
-__global__ __launch_bounds__(256) void k_dslash(DslashParams a) {
+__global__ __launch_bounds__(LYM_DSLASH_THREADS) LYM_OCC void k_dslash(DslashParams a) {
     if (a.skip && *a.skip) return;
     const int nbp = a.nblocks_per_parity;
+#if LYM_XCD_REMAP
     int lb = xcd_block(blockIdx.x, gridDim.x);
+#else
+    int lb = blockIdx.x;
+#endif
     int p = a.parity >= 0 ? a.parity : lb / nbp;
-    int idx = (lb % nbp) * 256 + threadIdx.x;
+    int idx = (lb % nbp) * LYM_DSLASH_THREADS + threadIdx.x;
     if (idx >= a.g.Vh) return;
     const int S = a.g.stride;
     int c[4];
@@ -208,7 +225,7 @@ __global__ __launch_bounds__(256) void k_dslash(DslashParams a) {
 
 template <int LK, int DIAG, bool INV>
 static void launch_one(const DslashParams& a, int nblocks, hipStream_t st) {
-    hipLaunchKernelGGL((k_dslash<LK, DIAG, INV>), dim3(nblocks), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_dslash<LK, DIAG, INV>), dim3(nblocks), dim3(LYM_DSLASH_THREADS), 0, st, a);
 }
 
 template <int LK>
@@ -225,7 +242,7 @@ static void launch_lk(const DslashParams& a, int diag, bool inv, int nblocks, hi
 }
 
 void launch_dslash(DslashParams a, int link_kind, int diag, bool inv, hipStream_t st) {
-    a.nblocks_per_parity = (a.g.Vh + 255) / 256;
+    a.nblocks_per_parity = (a.g.Vh + LYM_DSLASH_THREADS - 1) / LYM_DSLASH_THREADS;
     int nblocks = a.nblocks_per_parity * (a.parity >= 0 ? 1 : 2);
     switch (link_kind) {
         case 0: launch_lk<0>(a, diag, inv, nblocks, st); break;
