@@ -54,14 +54,18 @@ namespace {
 inline long half_n(const lym_ctx* c) { return 12L * c->g.stride; }
 inline long full_n(const lym_ctx* c) { return 24L * c->g.stride; }
 
-int dmalloc(void** p, size_t bytes) {
+// Allocates and zeroes on the context's stream: every later use of the buffer
+// is on that stream too, so the zero fill is ordered before it (a plain
+// hipMemset runs on the null stream, which a non-blocking stream does not
+// wait for).
+int dmalloc(lym_ctx* c, void** p, size_t bytes) {
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) {
         *p = nullptr;
         return fail(LYM_ERR_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
     }
-    e = hipMemset(*p, 0, bytes);
-    if (e != hipSuccess) return fail(LYM_ERR_DEVICE, "hipMemset: %s", hipGetErrorString(e));
+    e = hipMemsetAsync(*p, 0, bytes, c->stream);
+    if (e != hipSuccess) return fail(LYM_ERR_DEVICE, "hipMemsetAsync: %s", hipGetErrorString(e));
     return LYM_OK;
 }
 
@@ -75,7 +79,7 @@ int ensure_staging(lym_ctx* c, size_t bytes) {
     if (c->staging) (void)hipFree(c->staging);
     c->staging = nullptr;
     c->staging_bytes = 0;
-    int rc = dmalloc(&c->staging, bytes);
+    int rc = dmalloc(c, &c->staging, bytes);
     if (rc) return rc;
     c->staging_bytes = bytes;
     return LYM_OK;
@@ -95,17 +99,17 @@ int op_prepare(lym_op* op, bool force_blocks) {
     lym_gauge* g = const_cast<lym_gauge*>(op->gauge);
     const size_t blk = sizeof(zd) * 36 * (size_t)c->g.stride;
     if (!g->B) {
-        int rc = dmalloc((void**)&g->B, 2 * blk);
+        int rc = dmalloc(c, (void**)&g->B, 2 * blk);
         if (rc) return rc;
         lym::launch_clover_B(c->g, g->U, link_kind(g), reinterpret_cast<zd*>(g->B), c->stream);
         CHECK_LAUNCH();
     }
     if (!op->A) {
-        int rc = dmalloc((void**)&op->A, 2 * blk);
+        int rc = dmalloc(c, (void**)&op->A, 2 * blk);
         if (rc) return rc;
     }
     if (!op->Ainv) {
-        int rc = dmalloc((void**)&op->Ainv, blk);
+        int rc = dmalloc(c, (void**)&op->Ainv, blk);
         if (rc) return rc;
     }
     lym::launch_clover_A(c->g, reinterpret_cast<const zd*>(g->B), op->kappa * op->csw, reinterpret_cast<zd*>(op->A),
@@ -181,11 +185,11 @@ void apply_schur1(lym_op* op, zd* out_o, const zd* in_o, int gamma5, const int* 
 int ensure_op_scratch(lym_op* op) {
     lym_ctx* c = op->ctx;
     if (!op->tmp) {
-        int rc = dmalloc((void**)&op->tmp, sizeof(zd) * full_n(c));
+        int rc = dmalloc(c, (void**)&op->tmp, sizeof(zd) * full_n(c));
         if (rc) return rc;
     }
     if (!op->tmp2) {
-        int rc = dmalloc((void**)&op->tmp2, sizeof(zd) * half_n(c));
+        int rc = dmalloc(c, (void**)&op->tmp2, sizeof(zd) * half_n(c));
         if (rc) return rc;
     }
     return LYM_OK;
@@ -281,8 +285,8 @@ int lym_ctx_create(const int dims[4], int rep, int bc_t, lym_ctx** out) {
     (void)hipGetDevice(&c->device);
     int rc = LYM_OK;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = fail(LYM_ERR_DEVICE, "stream");
-    if (!rc) rc = dmalloc((void**)&c->red_partials, sizeof(double) * 8 * lym::RED_BLOCKS);
-    if (!rc) rc = dmalloc((void**)&c->red_result, 256);
+    if (!rc) rc = dmalloc(c, (void**)&c->red_partials, sizeof(double) * 8 * lym::RED_BLOCKS);
+    if (!rc) rc = dmalloc(c, (void**)&c->red_result, 256);
     if (!rc && hipHostMalloc((void**)&c->host_pinned, 4096, hipHostMallocDefault) != hipSuccess)
         rc = fail(LYM_ERR_DEVICE, "hipHostMalloc");
     if (!rc && (hipEventCreate(&c->ev_start) != hipSuccess || hipEventCreate(&c->ev_stop) != hipSuccess))
@@ -343,7 +347,7 @@ int lym_gauge_create(lym_ctx* c, const double* links, int folded, int storage, l
     const size_t elem = g->real_links ? sizeof(double) : sizeof(zd);
     const int ncomp = g->real_links ? g->compression : g->compression / 2;
     const size_t host_bytes = (size_t)c->V * 4 * 9 * elem;
-    int rc = dmalloc(&g->U, (size_t)2 * 4 * ncomp * c->g.stride * elem);
+    int rc = dmalloc(c, &g->U, (size_t)2 * 4 * ncomp * c->g.stride * elem);
     if (!rc) rc = ensure_staging(c, host_bytes);
     if (rc) {
         lym_gauge_destroy(g);
@@ -379,7 +383,7 @@ int lym_spinor_create(lym_ctx* c, lym_spinor** out) {
     lym_spinor* s = new lym_spinor();
     s->ctx = c;
     s->d = nullptr;
-    int rc = dmalloc((void**)&s->d, sizeof(zd) * full_n(c));
+    int rc = dmalloc(c, (void**)&s->d, sizeof(zd) * full_n(c));
     if (rc) {
         delete s;
         return rc;

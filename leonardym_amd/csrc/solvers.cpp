@@ -88,7 +88,7 @@ int ensure_work(lym_solver* s) {
     const size_t bytes = sizeof(zd) * full_n(s->ctx);
     for (int i = 0; i < 8; ++i) {
         if (hipMalloc((void**)&s->w[i], bytes) != hipSuccess) return fail(LYM_ERR_NOMEM, "solver work vectors");
-        if (hipMemset(s->w[i], 0, bytes) != hipSuccess) return fail(LYM_ERR_DEVICE, "hipMemset");
+        if (hipMemsetAsync(s->w[i], 0, bytes, s->ctx->stream) != hipSuccess) return fail(LYM_ERR_DEVICE, "hipMemsetAsync");
     }
     if (hipMalloc(&s->state, 256) != hipSuccess) return fail(LYM_ERR_NOMEM, "solver state");
     if (hipHostMalloc(&s->host_state, 256, hipHostMallocDefault) != hipSuccess)

@@ -260,3 +260,21 @@ def test_errors_are_codes_not_exits(case):
     by_name = lym.Operator(case.ctx, 0, name="Improved", kappa=case.kappa, csw=case.csw, gauge=case.gauge)
     psi = case.gauss()
     assert rel_err_per_site(case.apply(by_name, psi), case.orc.apply(K_CLOVER, psi)) < SITE_TOL
+
+
+def test_fresh_buffers_are_ordered_before_first_use():
+    """Buffers are zeroed on the context stream, so objects used right after
+    creation never see a late zero fill (regression: the fill once ran on the
+    null stream, unordered with the non-blocking context stream)."""
+    dims = (8, 8, 8, 8)
+    V = int(np.prod(dims))
+    ctx = lym.Context(dims)
+    for r in range(6):
+        U = fold_bc(dims, lym.gen_su3_links(dims, 0.5, 500 + r))
+        g = lym.Gauge(ctx, U, folded=True)
+        op = lym.Operator(ctx, lym.OP_EO_CLOVER, 1, 0.13, 1.0, True, gauge=g)
+        psi = lym.gaussian_spinor(V, 600 + r)
+        a, b = lym.Spinor(ctx, psi), lym.Spinor(ctx)
+        op.multiply(b, a)
+        orc = OracleField(dims, U, kappa=0.13, csw=1.0, need_clover=True)
+        assert rel_err_per_site(b.download(), orc.apply(K_EO, psi)) < SITE_TOL
