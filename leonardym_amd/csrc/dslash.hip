@@ -24,8 +24,10 @@
 #ifndef LYM_DSLASH_THREADS
 #define LYM_DSLASH_THREADS 256
 #endif
-#ifndef LYM_XCD_REMAP
-#define LYM_XCD_REMAP 1
+// both parities of a block range run on the same XCD (links and spinors of a
+// region are then fetched into one L2 for the even and the odd sites)
+#ifndef LYM_GRID_PER_CU
+#define LYM_GRID_PER_CU 0
 #endif
 #ifdef LYM_WAVES_PER_EU
 #define LYM_OCC __attribute__((amdgpu_waves_per_eu(LYM_WAVES_PER_EU, LYM_WAVES_PER_EU)))
@@ -166,17 +168,7 @@ __device__ __forceinline__ void hopping(const Geom& g, const void* links, const 
 }
 
 template <int LK, int DIAG, bool INV>
-__global__ __launch_bounds__(LYM_DSLASH_THREADS) LYM_OCC void k_dslash(DslashParams a) {
-    if (a.skip && *a.skip) return;
-    const int nbp = a.nblocks_per_parity;
-#if LYM_XCD_REMAP
-    int lb = xcd_block(blockIdx.x, gridDim.x);
-#else
-    int lb = blockIdx.x;
-#endif
-    int p = a.parity >= 0 ? a.parity : lb / nbp;
-    int idx = (lb % nbp) * LYM_DSLASH_THREADS + threadIdx.x;
-    if (idx >= a.g.Vh) return;
+__device__ __forceinline__ void dslash_site(const DslashParams& a, int p, int idx) {
     const int S = a.g.stride;
     int c[4];
     eo_coords(a.g, p, idx, c);
@@ -216,11 +208,40 @@ __global__ __launch_bounds__(LYM_DSLASH_THREADS) LYM_OCC void k_dslash(DslashPar
     if (a.v2[p]) {
         const cplx<double>* w = a.v2[p];
 #pragma unroll
-        for (int k = 0; k < 12; ++k) v[k] = v[k] + a.alpha * w[(size_t)k * S + idx];
+        for (int k = 0; k < 12; ++k) v[k] = v[k] + a.alpha * ld_nt(w + (size_t)k * S + idx);
     }
     cplx<double>* o = a.out[p];
 #pragma unroll
-    for (int k = 0; k < 12; ++k) o[(size_t)k * S + idx] = v[k];
+    for (int k = 0; k < 12; ++k) st_nt(o + (size_t)k * S + idx, v[k]);
+}
+
+// Grid: gridDim.x workgroups dealt round-robin over the 8 XCDs.  XCD x owns a
+// contiguous range of logical site blocks (x-major lattice order, the two
+// parities of a block range interleaved) and its workgroups sweep that range
+// in order, so at any time an XCD works on one compact slab and the
+// neighbour spinors and links of that slab are reused from its L2.
+template <int LK, int DIAG, bool INV>
+__global__ __launch_bounds__(LYM_DSLASH_THREADS) LYM_OCC void k_dslash(DslashParams a) {
+    if (a.skip && *a.skip) return;
+    const int nbt = a.nblocks_per_parity * (a.parity >= 0 ? 1 : 2);
+    const int nx = 8;
+    const int xcd = blockIdx.x % nx, k = blockIdx.x / nx;
+    const int nphys = ((int)gridDim.x - xcd + nx - 1) / nx;  // workgroups on this XCD
+    const int q = nbt / nx, r = nbt % nx;
+    const int lo = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
+    const int hi = lo + (xcd < r ? q + 1 : q);
+    for (int lb = lo + k; lb < hi; lb += nphys) {
+        int p, bi;
+        if (a.parity >= 0) {
+            p = a.parity;
+            bi = lb;
+        } else {
+            p = lb & 1;
+            bi = lb >> 1;
+        }
+        int idx = bi * LYM_DSLASH_THREADS + threadIdx.x;
+        if (idx < a.g.Vh) dslash_site<LK, DIAG, INV>(a, p, idx);
+    }
 }
 
 template <int LK, int DIAG, bool INV>
@@ -244,6 +265,8 @@ static void launch_lk(const DslashParams& a, int diag, bool inv, int nblocks, hi
 void launch_dslash(DslashParams a, int link_kind, int diag, bool inv, hipStream_t st) {
     a.nblocks_per_parity = (a.g.Vh + LYM_DSLASH_THREADS - 1) / LYM_DSLASH_THREADS;
     int nblocks = a.nblocks_per_parity * (a.parity >= 0 ? 1 : 2);
+    // workgroups in flight: LYM_GRID_PER_CU per CU of the 256 CUs, sweeping
+    if (LYM_GRID_PER_CU > 0 && nblocks > 256 * LYM_GRID_PER_CU) nblocks = 256 * LYM_GRID_PER_CU;
     switch (link_kind) {
         case 0: launch_lk<0>(a, diag, inv, nblocks, st); break;
         case 1: launch_lk<1>(a, diag, inv, nblocks, st); break;

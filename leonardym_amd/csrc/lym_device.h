@@ -4,7 +4,24 @@
 
 #include "lym_kernels.h"
 
+#ifndef LYM_NT_LINKS
+#define LYM_NT_LINKS 0
+#endif
+
 namespace lym {
+
+// Non-temporal 16-byte access for data touched once per launch (clover blocks,
+// axpy operands, outputs): it streams past L2 instead of evicting the
+// neighbour spinors and links that other sites of the launch reuse.
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ cplx<double> ld_nt(const cplx<double>* p) {
+    nt_d2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(p));
+    return {v.x, v.y};
+}
+__device__ __forceinline__ void st_nt(cplx<double>* p, cplx<double> z) {
+    nt_d2 v = {z.re, z.im};
+    __builtin_nontemporal_store(v, reinterpret_cast<nt_d2*>(p));
+}
 
 // ---------------------------------------------------------------- link loaders
 // LK: 0 complex 18-real, 1 complex 12-real (row 2 = conj(row0 x row1)),
@@ -26,7 +43,13 @@ __device__ __forceinline__ void load_link(const void* base, int stride, int p, i
         constexpr int NC = (LK == 0) ? 9 : 6;
         const cplx<double>* b = reinterpret_cast<const cplx<double>*>(base) + (size_t)((p * 4 + mu) * NC) * stride + idx;
 #pragma unroll
-        for (int k = 0; k < NC; ++k) L.m[k] = b[(size_t)k * stride];
+        for (int k = 0; k < NC; ++k) {
+#if LYM_NT_LINKS
+            L.m[k] = ld_nt(b + (size_t)k * stride);
+#else
+            L.m[k] = b[(size_t)k * stride];
+#endif
+        }
         if constexpr (LK == 1) {
             // row 2 = conj(row0 x row1)
             L.m[6] = conj(L.m[1] * L.m[5] - L.m[2] * L.m[4]);
@@ -89,7 +112,7 @@ __device__ __forceinline__ void herm6(const cplx<double>* A, int stride, int idx
     double d[6];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        cplx<double> dd = A[(size_t)k * stride + idx];
+        cplx<double> dd = ld_nt(A + (size_t)k * stride + idx);
         d[2 * k] = dd.re;
         d[2 * k + 1] = dd.im;
     }
@@ -100,7 +123,7 @@ __device__ __forceinline__ void herm6(const cplx<double>* A, int stride, int idx
     for (int i = 1; i < 6; ++i) {
 #pragma unroll
         for (int j = 0; j < i; ++j) {
-            cplx<double> a = A[(size_t)k * stride + idx];
+            cplx<double> a = ld_nt(A + (size_t)k * stride + idx);
             cfma(o[i], a, v[j]);
             cfmac(o[j], a, v[i]);
             ++k;
