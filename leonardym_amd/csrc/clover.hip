@@ -44,8 +44,8 @@ template <int LK>
 __device__ __forceinline__ M3 get_link(const Geom& g, const void* links, const int c[4], int mu) {
     int p = (c[0] + c[1] + c[2] + c[3]) & 1;
     int idx = eo_index(g, c[0], c[1], c[2], c[3]);
-    LinkMat<LK> L;
-    load_link<LK>(links, g.stride, p, mu, idx, L);
+    LinkMat<LK, double> L;
+    load_link<0, LK>(links, g.stride, p, mu, idx, L);
     M3 m;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {

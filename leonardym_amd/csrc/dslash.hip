@@ -51,15 +51,14 @@ struct Dir<3> { static constexpr int P0 = 2, C0 = 2, P1 = 3, C1 = 2; };
 
 // forward:  h_a = psi_a + c_a psi_P, acc_a += U h_a, acc_P += conj(c_a) U h_a
 // backward: h_a = psi_a - c_a psi_P, acc_a += U^+ h_a, acc_P -= conj(c_a) U^+ h_a
-template <int MU, bool FWD, int LK>
-__device__ __forceinline__ void hop_dir(cplx<double> acc[12], const cplx<double> psi[12], const LinkMat<LK>& L,
-                                        bool flip) {
+template <int MU, bool FWD, int LK, class T>
+__device__ __forceinline__ void hop_dir(cplx<T> acc[12], const cplx<T> psi[12], const LinkMat<LK, T>& L, bool flip) {
     using D = Dir<MU>;
     constexpr int PH0 = FWD ? D::C0 : D::C0 + 2;
     constexpr int PH1 = FWD ? D::C1 : D::C1 + 2;
     constexpr int Q0 = FWD ? 4 - D::C0 : 6 - D::C0;
     constexpr int Q1 = FWD ? 4 - D::C1 : 6 - D::C1;
-    cplx<double> h0[3], h1[3], r0[3], r1[3];
+    cplx<T> h0[3], h1[3], r0[3], r1[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         h0[c] = psi[c] + mulph<PH0>(psi[3 * D::P0 + c]);
@@ -70,11 +69,11 @@ __device__ __forceinline__ void hop_dir(cplx<double> acc[12], const cplx<double>
         }
     }
     if constexpr (FWD) {
-        matvec<LK>(L, h0, r0);
-        matvec<LK>(L, h1, r1);
+        matvec<LK, T>(L, h0, r0);
+        matvec<LK, T>(L, h1, r1);
     } else {
-        matvec_dag<LK>(L, h0, r0);
-        matvec_dag<LK>(L, h1, r1);
+        matvec_dag<LK, T>(L, h0, r0);
+        matvec_dag<LK, T>(L, h1, r1);
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -89,104 +88,67 @@ __device__ __forceinline__ void hop_dir(cplx<double> acc[12], const cplx<double>
 // hop of one direction must finish before the next direction's loads are
 // issued, which bounds the live registers to one hop (without it every load of
 // all eight hops is hoisted ahead of the arithmetic and the kernel spills).
-__device__ __forceinline__ void pin(cplx<double> acc[12]) {
+template <class T>
+__device__ __forceinline__ void pin(cplx<T> acc[12]) {
 #pragma unroll
     for (int k = 0; k < 12; ++k) asm volatile("" : "+v"(acc[k].re), "+v"(acc[k].im));
     asm volatile("" ::: "memory");
 }
 
-// H psi at site (p, idx) with coordinates c, reading the other parity `src`.
-template <int LK>
-__device__ __forceinline__ void hopping(const Geom& g, const void* links, const cplx<double>* src, int p, int idx,
-                                        const int c[4], cplx<double> acc[12]) {
-    const int S = g.stride;
+// one hop pair (forward and backward) in direction MU for site (p, site), rhs r
+template <int MU, int PR, int LK>
+__device__ __forceinline__ void hop_pair(const DslashParams& a, int p, int site, int r, const int c[4],
+                                         cplx<typename Prec<PR>::T> acc[12]) {
+    typedef typename Prec<PR>::T T;
+    const Geom& g = a.g;
     const int q = p ^ 1;
-#pragma unroll
-    for (int k = 0; k < 12; ++k) acc[k] = mk(0.0, 0.0);
-    cplx<double> psi[12];
-    LinkMat<LK> L;
-    // mu = 0 (x)
-    {
-        int xu = c[0] + 1 == g.L[0] ? 0 : c[0] + 1;
-        int xd = c[0] == 0 ? g.L[0] - 1 : c[0] - 1;
-        int iu = eo_index(g, xu, c[1], c[2], c[3]);
-        int id = eo_index(g, xd, c[1], c[2], c[3]);
-        load_spinor(src, S, iu, psi);
-        load_link<LK>(links, S, p, 0, idx, L);
-        hop_dir<0, true, LK>(acc, psi, L, false);
-        pin(acc);
-        load_spinor(src, S, id, psi);
-        load_link<LK>(links, S, q, 0, id, L);
-        hop_dir<0, false, LK>(acc, psi, L, false);
-        pin(acc);
-    }
-    {
-        int yu = c[1] + 1 == g.L[1] ? 0 : c[1] + 1;
-        int yd = c[1] == 0 ? g.L[1] - 1 : c[1] - 1;
-        int iu = eo_index(g, c[0], yu, c[2], c[3]);
-        int id = eo_index(g, c[0], yd, c[2], c[3]);
-        load_spinor(src, S, iu, psi);
-        load_link<LK>(links, S, p, 1, idx, L);
-        hop_dir<1, true, LK>(acc, psi, L, false);
-        pin(acc);
-        load_spinor(src, S, id, psi);
-        load_link<LK>(links, S, q, 1, id, L);
-        hop_dir<1, false, LK>(acc, psi, L, false);
-        pin(acc);
-    }
-    {
-        int zu = c[2] + 1 == g.L[2] ? 0 : c[2] + 1;
-        int zd = c[2] == 0 ? g.L[2] - 1 : c[2] - 1;
-        int iu = eo_index(g, c[0], c[1], zu, c[3]);
-        int id = eo_index(g, c[0], c[1], zd, c[3]);
-        load_spinor(src, S, iu, psi);
-        load_link<LK>(links, S, p, 2, idx, L);
-        hop_dir<2, true, LK>(acc, psi, L, false);
-        pin(acc);
-        load_spinor(src, S, id, psi);
-        load_link<LK>(links, S, q, 2, id, L);
-        hop_dir<2, false, LK>(acc, psi, L, false);
-        pin(acc);
-    }
-    {
-        const bool last = c[3] + 1 == g.L[3];
-        const bool first = c[3] == 0;
-        int tu = last ? 0 : c[3] + 1;
-        int td = first ? g.L[3] - 1 : c[3] - 1;
-        int iu = eo_index(g, c[0], c[1], c[2], tu);
-        int id = eo_index(g, c[0], c[1], c[2], td);
-        const bool anti = g.bc_t < 0;
-        load_spinor(src, S, iu, psi);
-        load_link<LK>(links, S, p, 3, idx, L);
-        hop_dir<3, true, LK>(acc, psi, L, anti && last);
-        pin(acc);
-        load_spinor(src, S, id, psi);
-        load_link<LK>(links, S, q, 3, id, L);
-        hop_dir<3, false, LK>(acc, psi, L, anti && first);
-        pin(acc);
-    }
+    int cu[4] = {c[0], c[1], c[2], c[3]}, cd[4] = {c[0], c[1], c[2], c[3]};
+    cu[MU] = c[MU] + 1 == g.L[MU] ? 0 : c[MU] + 1;
+    cd[MU] = c[MU] == 0 ? g.L[MU] - 1 : c[MU] - 1;
+    const int iu = eo_index(g, cu[0], cu[1], cu[2], cu[3]);
+    const int id = eo_index(g, cd[0], cd[1], cd[2], cd[3]);
+    const bool anti = MU == 3 && g.bc_t < 0;
+    const SpinorRef<PR> src{reinterpret_cast<const typename Prec<PR>::S*>(a.in[q]), a.in_sc[q]};
+    cplx<T> psi[12];
+    LinkMat<LK, T> L;
+    load_spinor<PR>(src, g.stride, a.nr, iu, r, psi);
+    load_link<PR, LK>(a.links, g.stride, p, MU, site, L);
+    hop_dir<MU, true, LK, T>(acc, psi, L, anti && c[3] == g.L[3] - 1);
+    pin(acc);
+    load_spinor<PR>(src, g.stride, a.nr, id, r, psi);
+    load_link<PR, LK>(a.links, g.stride, q, MU, id, L);
+    hop_dir<MU, false, LK, T>(acc, psi, L, anti && c[3] == 0);
+    pin(acc);
 }
 
-template <int LK, int DIAG, bool INV>
-__device__ __forceinline__ void dslash_site(const DslashParams& a, int p, int idx) {
-    const int S = a.g.stride;
+template <int PR, int LK, int DIAG, bool INV>
+__device__ __forceinline__ void dslash_site(const DslashParams& a, int p, int site, int r) {
+    typedef typename Prec<PR>::T T;
+    typedef typename Prec<PR>::S S;
+    const int ST = a.g.stride;
     int c[4];
-    eo_coords(a.g, p, idx, c);
-    cplx<double> acc[12];
-    hopping<LK>(a.g, a.links, a.in[p ^ 1], p, idx, c, acc);
-    const double hk = -a.kappa * a.hop_sign;
-    cplx<double> v[12];
+    eo_coords(a.g, p, site, c);
+    cplx<T> acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k] = mk((T)0, (T)0);
+    hop_pair<0, PR, LK>(a, p, site, r, c, acc);
+    hop_pair<1, PR, LK>(a, p, site, r, c, acc);
+    hop_pair<2, PR, LK>(a, p, site, r, c, acc);
+    hop_pair<3, PR, LK>(a, p, site, r, c, acc);
+    const T hk = (T)(-a.kappa * a.hop_sign);
+    cplx<T> v[12];
     if constexpr (DIAG == 0) {
 #pragma unroll
         for (int k = 0; k < 12; ++k) v[k] = hk * acc[k];
     } else {
-        cplx<double> s[12];
-        load_spinor(a.diag_in[p], S, idx, s);
+        cplx<T> s[12];
+        load_spinor<PR>(SpinorRef<PR>{reinterpret_cast<const S*>(a.diag_in[p]), a.diag_sc[p]}, ST, a.nr, site, r, s);
         if constexpr (DIAG == 2) {
-            cplx<double> t[12];
-            const cplx<double>* Ab = a.A + (size_t)p * 36 * S;
-            herm6(Ab, S, idx, s, t);
-            herm6(Ab + (size_t)18 * S, S, idx, s + 6, t + 6);
+            cplx<T> t[12];
+            const S* Ab = reinterpret_cast<const S*>(a.A) + (size_t)p * 36 * ST;
+            const T sc = PR == 2 ? (T)a.A_sc[(size_t)p * ST + site] : (T)1;
+            herm6<PR>(Ab, ST, site, sc, s, t);
+            herm6<PR>(Ab + (size_t)18 * ST, ST, site, sc, s + 6, t + 6);
 #pragma unroll
             for (int k = 0; k < 12; ++k) v[k] = t[k] + hk * acc[k];
         } else {
@@ -195,9 +157,11 @@ __device__ __forceinline__ void dslash_site(const DslashParams& a, int p, int id
         }
     }
     if constexpr (INV) {
-        cplx<double> t[12];
-        herm6(a.Ainv, S, idx, v, t);
-        herm6(a.Ainv + (size_t)18 * S, S, idx, v + 6, t + 6);
+        cplx<T> t[12];
+        const S* Ai = reinterpret_cast<const S*>(a.Ainv);
+        const T sc = PR == 2 ? (T)a.Ainv_sc[site] : (T)1;
+        herm6<PR>(Ai, ST, site, sc, v, t);
+        herm6<PR>(Ai + (size_t)18 * ST, ST, site, sc, v + 6, t + 6);
 #pragma unroll
         for (int k = 0; k < 12; ++k) v[k] = t[k];
     }
@@ -206,30 +170,33 @@ __device__ __forceinline__ void dslash_site(const DslashParams& a, int p, int id
         for (int k = 6; k < 12; ++k) v[k] = -v[k];
     }
     if (a.v2[p]) {
-        const cplx<double>* w = a.v2[p];
+        cplx<T> w[12];
+        load_spinor<PR, true>(SpinorRef<PR>{reinterpret_cast<const S*>(a.v2[p]), a.v2_sc[p]}, ST, a.nr, site, r, w);
+        const cplx<T> al = mk((T)a.alpha.re, (T)a.alpha.im);
 #pragma unroll
-        for (int k = 0; k < 12; ++k) v[k] = v[k] + a.alpha * ld_nt(w + (size_t)k * S + idx);
+        for (int k = 0; k < 12; ++k) v[k] = v[k] + al * w[k];
     }
-    cplx<double>* o = a.out[p];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) st_nt(o + (size_t)k * S + idx, v[k]);
+    store_spinor<PR>(SpinorOut<PR>{reinterpret_cast<S*>(a.out[p]), a.out_sc[p]}, ST, a.nr, site, r, v);
 }
 
 // Grid: gridDim.x workgroups dealt round-robin over the 8 XCDs.  XCD x owns a
 // contiguous range of logical site blocks (x-major lattice order, the two
 // parities of a block range interleaved) and its workgroups sweep that range
 // in order, so at any time an XCD works on one compact slab and the
-// neighbour spinors and links of that slab are reused from its L2.
-template <int LK, int DIAG, bool INV>
+// neighbour spinors and links of that slab are reused from its L2.  A
+// workgroup holds `spb` sites x `nr` right-hand sides, rhs fastest.
+template <int PR, int LK, int DIAG, bool INV>
 __global__ __launch_bounds__(LYM_DSLASH_THREADS) LYM_OCC void k_dslash(DslashParams a) {
     if (a.skip && *a.skip) return;
     const int nbt = a.nblocks_per_parity * (a.parity >= 0 ? 1 : 2);
     const int nx = 8;
     const int xcd = blockIdx.x % nx, k = blockIdx.x / nx;
     const int nphys = ((int)gridDim.x - xcd + nx - 1) / nx;  // workgroups on this XCD
-    const int q = nbt / nx, r = nbt % nx;
-    const int lo = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
-    const int hi = lo + (xcd < r ? q + 1 : q);
+    const int q = nbt / nx, rr = nbt % nx;
+    const int lo = xcd < rr ? xcd * (q + 1) : rr * (q + 1) + (xcd - rr) * q;
+    const int hi = lo + (xcd < rr ? q + 1 : q);
+    const int ls = (int)threadIdx.x / a.nr, r = (int)threadIdx.x - ls * a.nr;
+    if (ls >= a.spb) return;
     for (int lb = lo + k; lb < hi; lb += nphys) {
         int p, bi;
         if (a.parity >= 0) {
@@ -239,39 +206,51 @@ __global__ __launch_bounds__(LYM_DSLASH_THREADS) LYM_OCC void k_dslash(DslashPar
             p = lb & 1;
             bi = lb >> 1;
         }
-        int idx = bi * LYM_DSLASH_THREADS + threadIdx.x;
-        if (idx < a.g.Vh) dslash_site<LK, DIAG, INV>(a, p, idx);
+        const int site = bi * a.spb + ls;
+        if (site < a.g.Vh) dslash_site<PR, LK, DIAG, INV>(a, p, site, r);
     }
 }
 
-template <int LK, int DIAG, bool INV>
+template <int PR, int LK, int DIAG, bool INV>
 static void launch_one(const DslashParams& a, int nblocks, hipStream_t st) {
-    hipLaunchKernelGGL((k_dslash<LK, DIAG, INV>), dim3(nblocks), dim3(LYM_DSLASH_THREADS), 0, st, a);
+    hipLaunchKernelGGL((k_dslash<PR, LK, DIAG, INV>), dim3(nblocks), dim3(LYM_DSLASH_THREADS), 0, st, a);
 }
 
-template <int LK>
+template <int PR, int LK>
 static void launch_lk(const DslashParams& a, int diag, bool inv, int nblocks, hipStream_t st) {
     if (diag == 0) {
-        if (inv) launch_one<LK, 0, true>(a, nblocks, st);
-        else launch_one<LK, 0, false>(a, nblocks, st);
+        if (inv) launch_one<PR, LK, 0, true>(a, nblocks, st);
+        else launch_one<PR, LK, 0, false>(a, nblocks, st);
     } else if (diag == 1) {
-        if (inv) launch_one<LK, 1, true>(a, nblocks, st);
-        else launch_one<LK, 1, false>(a, nblocks, st);
+        if (inv) launch_one<PR, LK, 1, true>(a, nblocks, st);
+        else launch_one<PR, LK, 1, false>(a, nblocks, st);
     } else {
-        launch_one<LK, 2, false>(a, nblocks, st);
+        launch_one<PR, LK, 2, false>(a, nblocks, st);
+    }
+}
+
+template <int PR>
+static void launch_pr(const DslashParams& a, int link_kind, int diag, bool inv, int nblocks, hipStream_t st) {
+    switch (link_kind) {
+        case 0: launch_lk<PR, 0>(a, diag, inv, nblocks, st); break;
+        case 1: launch_lk<PR, 1>(a, diag, inv, nblocks, st); break;
+        case 2: launch_lk<PR, 2>(a, diag, inv, nblocks, st); break;
+        default: launch_lk<PR, 3>(a, diag, inv, nblocks, st); break;
     }
 }
 
 void launch_dslash(DslashParams a, int link_kind, int diag, bool inv, hipStream_t st) {
-    a.nblocks_per_parity = (a.g.Vh + LYM_DSLASH_THREADS - 1) / LYM_DSLASH_THREADS;
+    if (a.nr < 1) a.nr = 1;
+    a.spb = LYM_DSLASH_THREADS / a.nr;
+    if (a.spb < 1) a.spb = 1;
+    a.nblocks_per_parity = (a.g.Vh + a.spb - 1) / a.spb;
     int nblocks = a.nblocks_per_parity * (a.parity >= 0 ? 1 : 2);
     // workgroups in flight: LYM_GRID_PER_CU per CU of the 256 CUs, sweeping
     if (LYM_GRID_PER_CU > 0 && nblocks > 256 * LYM_GRID_PER_CU) nblocks = 256 * LYM_GRID_PER_CU;
-    switch (link_kind) {
-        case 0: launch_lk<0>(a, diag, inv, nblocks, st); break;
-        case 1: launch_lk<1>(a, diag, inv, nblocks, st); break;
-        case 2: launch_lk<2>(a, diag, inv, nblocks, st); break;
-        default: launch_lk<3>(a, diag, inv, nblocks, st); break;
+    switch (a.prec) {
+        case 0: launch_pr<0>(a, link_kind, diag, inv, nblocks, st); break;
+        case 1: launch_pr<1>(a, link_kind, diag, inv, nblocks, st); break;
+        default: launch_pr<2>(a, link_kind, diag, inv, nblocks, st); break;
     }
 }
 

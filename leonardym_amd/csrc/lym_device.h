@@ -1,5 +1,18 @@
-// lym_device.h -- device-side link loaders, colour matvecs and spinor loads
-// shared by the stencil and the clover builder.
+// lym_device.h -- device-side field access shared by the stencil, the clover
+// builder and the solver kernels, generic over the storage precision and the
+// number of right-hand sides.
+//
+// Storage precisions (PR):
+//   0  FP64: spinor double2, links double2 / double, clover double2
+//   1  FP32: float2 / float, computed in FP32
+//   2  FP16: _Float16 pairs, computed in FP32; spinors and clover blocks carry
+//      one FP32 scale per site (and right-hand side) = the largest |real|
+//      component, so the stored values lie in [-1, 1]; links are unitary
+//      (or orthogonal), their entries already lie in [-1, 1] and need no scale.
+// Multi-RHS fields store the right-hand side fastest:
+//   element (component k, site s, rhs r) at (k * stride + s) * NR + r,
+// so the NR lanes that handle one site read one link (a single broadcast
+// transaction) and NR consecutive spinor elements.
 #pragma once
 
 #include "lym_kernels.h"
@@ -10,57 +23,184 @@
 
 namespace lym {
 
-// Non-temporal 16-byte access for data touched once per launch (clover blocks,
-// axpy operands, outputs): it streams past L2 instead of evicting the
-// neighbour spinors and links that other sites of the launch reuse.
-typedef double nt_d2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ cplx<double> ld_nt(const cplx<double>* p) {
-    nt_d2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(p));
-    return {v.x, v.y};
+typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
+
+template <int PR>
+struct Prec;
+template <>
+struct Prec<0> {
+    typedef double T;
+    typedef cplx<double> S;  // complex storage element
+    typedef double R;        // real storage element
+};
+template <>
+struct Prec<1> {
+    typedef float T;
+    typedef cplx<float> S;
+    typedef float R;
+};
+template <>
+struct Prec<2> {
+    typedef float T;
+    typedef h2_t S;
+    typedef _Float16 R;
+};
+
+// Non-temporal access for data touched once per launch (clover blocks, axpy
+// operands, outputs): it streams past L2 instead of evicting the neighbour
+// spinors and links that other sites of the launch reuse.
+template <class E>
+__device__ __forceinline__ E ld_nt_raw(const E* p) {
+    if constexpr (sizeof(E) == 16) {
+        typedef double v2 __attribute__((ext_vector_type(2)));
+        v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+        E e;
+        __builtin_memcpy(&e, &v, 16);
+        return e;
+    } else if constexpr (sizeof(E) == 8) {
+        typedef float v2 __attribute__((ext_vector_type(2)));
+        v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+        E e;
+        __builtin_memcpy(&e, &v, 8);
+        return e;
+    } else {
+        static_assert(sizeof(E) == 4, "element size");
+        unsigned v = __builtin_nontemporal_load(reinterpret_cast<const unsigned*>(p));
+        E e;
+        __builtin_memcpy(&e, &v, 4);
+        return e;
+    }
 }
-__device__ __forceinline__ void st_nt(cplx<double>* p, cplx<double> z) {
-    nt_d2 v = {z.re, z.im};
-    __builtin_nontemporal_store(v, reinterpret_cast<nt_d2*>(p));
+template <class E>
+__device__ __forceinline__ void st_nt_raw(E* p, E e) {
+    if constexpr (sizeof(E) == 16) {
+        typedef double v2 __attribute__((ext_vector_type(2)));
+        v2 v;
+        __builtin_memcpy(&v, &e, 16);
+        __builtin_nontemporal_store(v, reinterpret_cast<v2*>(p));
+    } else if constexpr (sizeof(E) == 8) {
+        typedef float v2 __attribute__((ext_vector_type(2)));
+        v2 v;
+        __builtin_memcpy(&v, &e, 8);
+        __builtin_nontemporal_store(v, reinterpret_cast<v2*>(p));
+    } else {
+        unsigned v;
+        __builtin_memcpy(&v, &e, 4);
+        __builtin_nontemporal_store(v, reinterpret_cast<unsigned*>(p));
+    }
+}
+
+__device__ __forceinline__ cplx<double> ld_nt(const cplx<double>* p) { return ld_nt_raw(p); }
+__device__ __forceinline__ void st_nt(cplx<double>* p, cplx<double> z) { st_nt_raw(p, z); }
+
+// storage element -> compute complex
+template <int PR>
+__device__ __forceinline__ cplx<typename Prec<PR>::T> to_c(typename Prec<PR>::S s) {
+    if constexpr (PR == 2) return {(float)s.x, (float)s.y};
+    else return s;
+}
+template <int PR>
+__device__ __forceinline__ typename Prec<PR>::S from_c(cplx<typename Prec<PR>::T> z) {
+    if constexpr (PR == 2) {
+        h2_t h;
+        h.x = (_Float16)z.re;
+        h.y = (_Float16)z.im;
+        return h;
+    } else {
+        return z;
+    }
+}
+
+// ---------------------------------------------------------------- spinors
+// One parity of a (multi-RHS) spinor field.
+template <int PR>
+struct SpinorRef {
+    const typename Prec<PR>::S* base;
+    const float* scale;  // PR == 2 only: [stride][NR]
+};
+template <int PR>
+struct SpinorOut {
+    typename Prec<PR>::S* base;
+    float* scale;
+};
+
+template <int PR, bool NT = false>
+__device__ __forceinline__ void load_spinor(SpinorRef<PR> f, int stride, int nr, int site, int r,
+                                            cplx<typename Prec<PR>::T> s[12]) {
+    const size_t o = (size_t)site * nr + r;
+    const size_t ks = (size_t)stride * nr;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+        typename Prec<PR>::S e = NT ? ld_nt_raw(f.base + k * ks + o) : f.base[k * ks + o];
+        s[k] = to_c<PR>(e);
+    }
+    if constexpr (PR == 2) {
+        const float sc = f.scale[o];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) s[k] = sc * s[k];
+    }
+}
+
+template <int PR>
+__device__ __forceinline__ void store_spinor(SpinorOut<PR> f, int stride, int nr, int site, int r,
+                                             const cplx<typename Prec<PR>::T> v[12]) {
+    const size_t o = (size_t)site * nr + r;
+    const size_t ks = (size_t)stride * nr;
+    if constexpr (PR == 2) {
+        float m = 0.f;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) m = fmaxf(m, fmaxf(fabsf(v[k].re), fabsf(v[k].im)));
+        const float inv = m > 0.f ? 1.0f / m : 0.f;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) st_nt_raw(f.base + k * ks + o, from_c<PR>(inv * v[k]));
+        f.scale[o] = m;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) st_nt_raw(f.base + k * ks + o, v[k]);
+    }
 }
 
 // ---------------------------------------------------------------- link loaders
 // LK: 0 complex 18-real, 1 complex 12-real (row 2 = conj(row0 x row1)),
 //     2 real 9 (adjoint), 3 real 6 (row 2 = row0 x row1)
-template <int LK>
+template <int LK, class T>
 struct LinkMat;
-template <>
-struct LinkMat<0> { cplx<double> m[9]; };
-template <>
-struct LinkMat<1> { cplx<double> m[9]; };
-template <>
-struct LinkMat<2> { double m[9]; };
-template <>
-struct LinkMat<3> { double m[9]; };
+template <class T>
+struct LinkMat<0, T> { cplx<T> m[9]; };
+template <class T>
+struct LinkMat<1, T> { cplx<T> m[9]; };
+template <class T>
+struct LinkMat<2, T> { T m[9]; };
+template <class T>
+struct LinkMat<3, T> { T m[9]; };
 
-template <int LK>
-__device__ __forceinline__ void load_link(const void* base, int stride, int p, int mu, int idx, LinkMat<LK>& L) {
+template <int PR, int LK>
+__device__ __forceinline__ void load_link(const void* base, int stride, int p, int mu, int idx,
+                                          LinkMat<LK, typename Prec<PR>::T>& L) {
+    typedef typename Prec<PR>::T T;
     if constexpr (LK == 0 || LK == 1) {
         constexpr int NC = (LK == 0) ? 9 : 6;
-        const cplx<double>* b = reinterpret_cast<const cplx<double>*>(base) + (size_t)((p * 4 + mu) * NC) * stride + idx;
+        const typename Prec<PR>::S* b =
+            reinterpret_cast<const typename Prec<PR>::S*>(base) + (size_t)((p * 4 + mu) * NC) * stride + idx;
 #pragma unroll
         for (int k = 0; k < NC; ++k) {
 #if LYM_NT_LINKS
-            L.m[k] = ld_nt(b + (size_t)k * stride);
+            L.m[k] = to_c<PR>(ld_nt_raw(b + (size_t)k * stride));
 #else
-            L.m[k] = b[(size_t)k * stride];
+            L.m[k] = to_c<PR>(b[(size_t)k * stride]);
 #endif
         }
         if constexpr (LK == 1) {
-            // row 2 = conj(row0 x row1)
             L.m[6] = conj(L.m[1] * L.m[5] - L.m[2] * L.m[4]);
             L.m[7] = conj(L.m[2] * L.m[3] - L.m[0] * L.m[5]);
             L.m[8] = conj(L.m[0] * L.m[4] - L.m[1] * L.m[3]);
         }
     } else {
         constexpr int NC = (LK == 2) ? 9 : 6;
-        const double* b = reinterpret_cast<const double*>(base) + (size_t)((p * 4 + mu) * NC) * stride + idx;
+        const typename Prec<PR>::R* b =
+            reinterpret_cast<const typename Prec<PR>::R*>(base) + (size_t)((p * 4 + mu) * NC) * stride + idx;
 #pragma unroll
-        for (int k = 0; k < NC; ++k) L.m[k] = b[(size_t)k * stride];
+        for (int k = 0; k < NC; ++k) L.m[k] = (T)b[(size_t)k * stride];
         if constexpr (LK == 3) {
             L.m[6] = L.m[1] * L.m[5] - L.m[2] * L.m[4];
             L.m[7] = L.m[2] * L.m[3] - L.m[0] * L.m[5];
@@ -69,12 +209,12 @@ __device__ __forceinline__ void load_link(const void* base, int stride, int p, i
     }
 }
 
-template <int LK>
-__device__ __forceinline__ void matvec(const LinkMat<LK>& L, const cplx<double> h[3], cplx<double> o[3]) {
+template <int LK, class T>
+__device__ __forceinline__ void matvec(const LinkMat<LK, T>& L, const cplx<T> h[3], cplx<T> o[3]) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         if constexpr (LK <= 1) {
-            cplx<double> s = L.m[3 * i] * h[0];
+            cplx<T> s = L.m[3 * i] * h[0];
             cfma(s, L.m[3 * i + 1], h[1]);
             cfma(s, L.m[3 * i + 2], h[2]);
             o[i] = s;
@@ -85,12 +225,12 @@ __device__ __forceinline__ void matvec(const LinkMat<LK>& L, const cplx<double> 
     }
 }
 
-template <int LK>
-__device__ __forceinline__ void matvec_dag(const LinkMat<LK>& L, const cplx<double> h[3], cplx<double> o[3]) {
+template <int LK, class T>
+__device__ __forceinline__ void matvec_dag(const LinkMat<LK, T>& L, const cplx<T> h[3], cplx<T> o[3]) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         if constexpr (LK <= 1) {
-            cplx<double> s = cmulc(L.m[i], h[0]);
+            cplx<T> s = cmulc(L.m[i], h[0]);
             cfmac(s, L.m[3 + i], h[1]);
             cfmac(s, L.m[6 + i], h[2]);
             o[i] = s;
@@ -101,20 +241,26 @@ __device__ __forceinline__ void matvec_dag(const LinkMat<LK>& L, const cplx<doub
     }
 }
 
+// FP64 single-RHS spinor load (layout.hip, clover.hip)
 __device__ __forceinline__ void load_spinor(const cplx<double>* f, int stride, int idx, cplx<double> s[12]) {
 #pragma unroll
     for (int k = 0; k < 12; ++k) s[k] = f[(size_t)k * stride + idx];
 }
 
-// o = A v for one packed hermitian 6x6 block (base points at the block's first component)
-__device__ __forceinline__ void herm6(const cplx<double>* A, int stride, int idx, const cplx<double> v[6],
-                                      cplx<double> o[6]) {
-    double d[6];
+// ---------------------------------------------------------------- clover blocks
+// o = A v for one packed hermitian 6x6 block; A points at the block's first
+// component of the parity, sc is the per-site scale (PR == 2) or 1.
+template <int PR>
+__device__ __forceinline__ void herm6(const typename Prec<PR>::S* A, int stride, int idx,
+                                      typename Prec<PR>::T sc, const cplx<typename Prec<PR>::T> v[6],
+                                      cplx<typename Prec<PR>::T> o[6]) {
+    typedef typename Prec<PR>::T T;
+    T d[6];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        cplx<double> dd = ld_nt(A + (size_t)k * stride + idx);
-        d[2 * k] = dd.re;
-        d[2 * k + 1] = dd.im;
+        cplx<T> dd = to_c<PR>(ld_nt_raw(A + (size_t)k * stride + idx));
+        d[2 * k] = sc * dd.re;
+        d[2 * k + 1] = sc * dd.im;
     }
 #pragma unroll
     for (int i = 0; i < 6; ++i) o[i] = d[i] * v[i];
@@ -123,12 +269,18 @@ __device__ __forceinline__ void herm6(const cplx<double>* A, int stride, int idx
     for (int i = 1; i < 6; ++i) {
 #pragma unroll
         for (int j = 0; j < i; ++j) {
-            cplx<double> a = ld_nt(A + (size_t)k * stride + idx);
+            cplx<T> a = sc * to_c<PR>(ld_nt_raw(A + (size_t)k * stride + idx));
             cfma(o[i], a, v[j]);
             cfmac(o[j], a, v[i]);
             ++k;
         }
     }
+}
+
+// FP64 form used by the clover builder and the EO driver
+__device__ __forceinline__ void herm6(const cplx<double>* A, int stride, int idx, const cplx<double> v[6],
+                                      cplx<double> o[6]) {
+    herm6<0>(A, stride, idx, 1.0, v, o);
 }
 
 }  // namespace lym

@@ -7,15 +7,27 @@ namespace lym {
 
 typedef cplx<double> zd;
 
+// Parameters of one stencil launch.  Field pointers are per parity; the
+// storage element type follows `prec` (0 FP64, 1 FP32, 2 FP16 + per-site
+// scales); multi-RHS fields hold `nr` right-hand sides, rhs fastest.
 struct DslashParams {
     Geom g;
-    const zd* in[2];       // per parity: the field read at neighbours is in[1 - p]
-    const zd* diag_in[2];  // per parity: the field under the site-diagonal term
-    zd* out[2];
-    const zd* v2[2];       // optional axpy operand
+    int prec;              // storage precision of spinors, links and clover blocks
+    int nr;                // right-hand sides per field
+    int spb;               // sites per workgroup (set by the launcher)
+    const void* in[2];     // per parity: the field read at neighbours is in[1 - p]
+    const float* in_sc[2];
+    const void* diag_in[2];  // per parity: the field under the site-diagonal term
+    const float* diag_sc[2];
+    void* out[2];
+    float* out_sc[2];
+    const void* v2[2];     // optional axpy operand
+    const float* v2_sc[2];
     const void* links;
-    const zd* A;           // [2][36][stride] clover blocks of D
-    const zd* Ainv;        // [36][stride] inverse blocks (even sites)
+    const void* A;         // [2][36][stride] clover blocks of D
+    const float* A_sc;     // [2][stride] (FP16)
+    const void* Ainv;      // [36][stride] inverse blocks (even sites)
+    const float* Ainv_sc;  // [stride] (FP16)
     double kappa;
     double hop_sign;
     zd alpha;
