@@ -47,6 +47,16 @@ extern "C" {
 #define LYM_SOLVER_CG 0                 /* ConjugateGradient.cpp:10-74 */
 #define LYM_SOLVER_BICGSTAB 1           /* BiConjugateGradient.cpp:321-507 */
 #define LYM_SOLVER_PRECOND_BICGSTAB 2   /* PreconditionedBiCGStab.cpp:33-100 */
+/* Forms beyond the reference (no counterpart in it; their parity is against
+ * the FP64 oracle): */
+#define LYM_SOLVER_MIXED_CG 3           /* CG on Q^2 with an FP32/FP16 sloppy operator and FP64 reliable updates */
+#define LYM_SOLVER_PRECOND_CG 4         /* the even-odd driver of PreconditionedBiCGStab.cpp:43-91 with FP64 CG on Qhat^2 */
+#define LYM_SOLVER_PRECOND_MIXED_CG 5   /* the even-odd driver with the mixed-precision CG on Qhat^2 */
+
+/* storage precision of a sloppy operator */
+#define LYM_PREC_DOUBLE 0
+#define LYM_PREC_SINGLE 1  /* FP32 storage and arithmetic */
+#define LYM_PREC_HALF 2    /* FP16 storage (per-site FP32 scale on spinors and clover), FP32 arithmetic */
 
 /* link storage on the device */
 #define LYM_LINKS_FULL 0          /* 18 reals (9 for adjoint) */
@@ -64,6 +74,7 @@ typedef struct {
     double last_error;  /* final |r|^2 of the Krylov recursion (getLastError()) */
     unsigned applies;   /* operator applications performed */
     double seconds;     /* wall time of the solve on the device stream */
+    unsigned reliable_updates; /* FP64 true-residual recomputations (mixed-precision solvers) */
 } lym_solve_info;
 
 /* ------------------------------------------------------------------ errors / device */
@@ -142,6 +153,11 @@ void lym_solver_destroy(lym_solver* s);
 int lym_solver_set_precision(lym_solver* s, double precision);
 int lym_solver_set_max_steps(lym_solver* s, unsigned max_steps);
 int lym_solver_set_even_odd(lym_solver* s, int use_even_odd);   /* setUseEvenOddPreconditioning */
+/* mixed-precision solvers (no counterpart in the reference): sloppy storage
+ * precision (LYM_PREC_SINGLE or LYM_PREC_HALF, default HALF) and the factor by
+ * which the sloppy residual must fall before an FP64 reliable update (default 0.1) */
+int lym_solver_set_sloppy_precision(lym_solver* s, int prec);
+int lym_solver_set_reliable_delta(lym_solver* s, double delta);
 /* solve(dirac, source, solution, initial_guess) -- x0 may be NULL */
 int lym_solver_solve(lym_solver* s, lym_op* op, const lym_spinor* source, lym_spinor* solution,
                      const lym_spinor* initial_guess, lym_solve_info* info);

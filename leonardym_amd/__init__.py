@@ -15,6 +15,8 @@ LIB_PATH = os.path.join(HERE, "liblym.so")
 REP_SU3_FUND, REP_SU2_ADJ = 0, 1
 OP_WILSON, OP_BASIC_WILSON, OP_CLOVER, OP_EO_CLOVER = 0, 1, 2, 3
 SOLVER_CG, SOLVER_BICGSTAB, SOLVER_PRECOND_BICGSTAB = 0, 1, 2
+SOLVER_MIXED_CG, SOLVER_PRECOND_CG, SOLVER_PRECOND_MIXED_CG = 3, 4, 5
+PREC_DOUBLE, PREC_SINGLE, PREC_HALF = 0, 1, 2
 LINKS_FULL, LINKS_RECONSTRUCT = 0, 1
 
 # C ABI: name -> (restype, argtypes)
@@ -30,7 +32,7 @@ _PI = ctypes.POINTER(ctypes.c_int)
 
 class SolveInfo(ctypes.Structure):
     _fields_ = [("converged", ctypes.c_int), ("steps", ctypes.c_uint), ("last_error", ctypes.c_double),
-                ("applies", ctypes.c_uint), ("seconds", ctypes.c_double)]
+                ("applies", ctypes.c_uint), ("seconds", ctypes.c_double), ("reliable_updates", ctypes.c_uint)]
 
 
 SIGNATURES = {
@@ -71,6 +73,8 @@ SIGNATURES = {
     "lym_solver_set_precision": (_I, [_P, _D]),
     "lym_solver_set_max_steps": (_I, [_P, _U]),
     "lym_solver_set_even_odd": (_I, [_P, _I]),
+    "lym_solver_set_sloppy_precision": (_I, [_P, _I]),
+    "lym_solver_set_reliable_delta": (_I, [_P, _D]),
     "lym_solver_solve": (_I, [_P, _P, _P, _P, _P, ctypes.POINTER(SolveInfo)]),
     "lym_gen_su3_links": (_I, [_P, _D, _U64, _P]),
     "lym_gen_su2_links": (_I, [_P, _D, _U64, _P]),
@@ -327,11 +331,15 @@ class Operator:
 
 
 class Solver:
-    def __init__(self, ctx, kind, precision=None, max_steps=None, even_odd=True):
+    def __init__(self, ctx, kind, precision=None, max_steps=None, even_odd=True, sloppy=None, delta=None):
         self.ctx = ctx
         h = ctypes.c_void_p()
         _check(lib().lym_solver_create(ctx.h, kind, ctypes.byref(h)))
         self.h = h
+        if sloppy is not None:
+            _check(lib().lym_solver_set_sloppy_precision(self.h, sloppy))
+        if delta is not None:
+            _check(lib().lym_solver_set_reliable_delta(self.h, delta))
         if precision is not None:
             _check(lib().lym_solver_set_precision(self.h, precision))
         if max_steps is not None:

@@ -125,6 +125,46 @@ int op_prepare(lym_op* op, bool force_blocks) {
     HIPCHECK(hipStreamSynchronize(c->stream));
     if (hflag) return fail(LYM_ERR_SINGULAR, "%d clover blocks are singular", hflag);
     op->diag_dirty = 0;
+    op->sloppy_prec = 0;
+    return LYM_OK;
+}
+
+void free_sloppy(lym_op* op) {
+    void* ptrs[5] = {op->Us, op->As, op->As_sc, op->Ainvs, op->Ainvs_sc};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    op->Us = op->As = op->Ainvs = nullptr;
+    op->As_sc = op->Ainvs_sc = nullptr;
+    op->sloppy_prec = 0;
+}
+
+// low-precision copies of links and blocks for a mixed-precision solve
+int op_sloppy_data(lym_op* op, int prec) {
+    lym_ctx* c = op->ctx;
+    if (prec != 1 && prec != 2) return fail(LYM_ERR_INVALID, "sloppy precision must be 1 (FP32) or 2 (FP16)");
+    if (op->sloppy_prec == prec) return LYM_OK;
+    free_sloppy(op);
+    const lym_gauge* g = op->gauge;
+    const long S = c->g.stride;
+    const int ncomp = g->real_links ? g->compression : g->compression / 2;
+    const long nl = 2L * 4 * ncomp * S;
+    const size_t le = g->real_links ? (prec == 1 ? 4 : 2) : (prec == 1 ? 8 : 4);
+    const size_t se = prec == 1 ? 8 : 4;  // spinor/block element
+    int rc = dmalloc(c, &op->Us, nl * le);
+    if (rc) return rc;
+    lym::launch_convert_links(prec, g->U, op->Us, nl, g->real_links, c->stream);
+    if (op->A) {
+        rc = dmalloc(c, &op->As, 2L * 36 * S * se);
+        if (!rc) rc = dmalloc(c, (void**)&op->As_sc, 2L * S * sizeof(float));
+        if (!rc) rc = dmalloc(c, &op->Ainvs, 36L * S * se);
+        if (!rc) rc = dmalloc(c, (void**)&op->Ainvs_sc, S * sizeof(float));
+        if (rc) return rc;
+        lym::launch_convert_blocks(prec, reinterpret_cast<const zd*>(op->A), op->As, op->As_sc, 2, (int)S, c->stream);
+        lym::launch_convert_blocks(prec, reinterpret_cast<const zd*>(op->Ainv), op->Ainvs, op->Ainvs_sc, 1, (int)S,
+                                   c->stream);
+    }
+    CHECK_LAUNCH();
+    op->sloppy_prec = prec;
     return LYM_OK;
 }
 
@@ -240,6 +280,68 @@ void apply_full(lym_op* op, zd* out, const zd* in, int gamma5, const int* skip) 
     apply_full1(op, out, in, nullptr, lym::mk(0.0, 0.0), gamma5, skip);
 }
 int link_kind_of(const lym_gauge* g) { return link_kind(g); }
+
+int op_sloppy(lym_op* op, int prec) { return op_sloppy_data(op, prec); }
+
+static size_t selem(int prec) { return prec == 1 ? 8 : 4; }
+
+// full lattice, sloppy precision: out = Op in
+void apply_full_sloppy(lym_op* op, int prec, void* out, float* out_sc, const void* in, const float* in_sc, int gamma5,
+                       const int* skip) {
+    lym_ctx* c = op->ctx;
+    const size_t po = 12 * (size_t)c->g.stride * selem(prec);
+    lym::DslashParams a = base_params(op);
+    a.prec = prec;
+    a.links = op->Us;
+    a.A = op->As;
+    a.A_sc = op->As_sc;
+    for (int p = 0; p < 2; ++p) {
+        a.in[p] = (const char*)in + p * po;
+        a.diag_in[p] = a.in[p];
+        a.out[p] = (char*)out + p * po;
+        if (prec == 2) {
+            a.in_sc[p] = in_sc + (size_t)p * c->g.stride;
+            a.diag_sc[p] = a.in_sc[p];
+            a.out_sc[p] = out_sc + (size_t)p * c->g.stride;
+        }
+    }
+    a.gamma5 = gamma5;
+    a.parity = -1;
+    a.skip = skip;
+    lym::launch_dslash(a, link_kind(op->gauge), op_has_clover(op) ? 2 : 1, false, c->stream);
+}
+
+// Schur complement, sloppy precision, odd half fields
+void apply_schur_sloppy(lym_op* op, int prec, void* out_o, float* out_sc, const void* in_o, const float* in_sc,
+                        void* tmp_e, float* tmp_sc, int gamma5, const int* skip) {
+    lym_ctx* c = op->ctx;
+    int lk = link_kind(op->gauge);
+    lym::DslashParams a = base_params(op);
+    a.prec = prec;
+    a.links = op->Us;
+    a.A = op->As;
+    a.A_sc = op->As_sc;
+    a.Ainv = op->Ainvs;
+    a.Ainv_sc = op->Ainvs_sc;
+    a.skip = skip;
+    lym::DslashParams b = a;
+    a.parity = 0;
+    a.in[1] = in_o;
+    a.in_sc[1] = in_sc;
+    a.out[0] = tmp_e;
+    a.out_sc[0] = tmp_sc;
+    lym::launch_dslash(a, lk, 0, true, c->stream);
+    b.parity = 1;
+    b.in[0] = tmp_e;
+    b.in_sc[0] = tmp_sc;
+    b.diag_in[1] = in_o;
+    b.diag_sc[1] = in_sc;
+    b.out[1] = out_o;
+    b.out_sc[1] = out_sc;
+    b.hop_sign = -1.0;
+    b.gamma5 = gamma5;
+    lym::launch_dslash(b, lk, op->csw != 0.0 ? 2 : 1, false, c->stream);
+}
 }  // namespace lym_internal
 
 // ====================================================================== C ABI
@@ -541,6 +643,7 @@ int lym_op_create_by_name(lym_ctx* c, const char* name, unsigned power, double k
 void lym_op_destroy(lym_op* op) {
     if (!op) return;
     if (op->ctx && op->ctx->stream) (void)hipStreamSynchronize(op->ctx->stream);
+    free_sloppy(op);
     if (op->A) (void)hipFree(op->A);
     if (op->Ainv) (void)hipFree(op->Ainv);
     if (op->tmp) (void)hipFree(op->tmp);
@@ -552,6 +655,7 @@ int lym_op_set_gauge(lym_op* op, const lym_gauge* g) {
     if (!op || !g || g->ctx != op->ctx) return fail(LYM_ERR_INVALID, "gauge field of another context");
     op->gauge = g;
     op->diag_dirty = 1;
+    op->sloppy_prec = 0;
     return LYM_OK;
 }
 

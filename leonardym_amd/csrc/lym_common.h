@@ -154,4 +154,11 @@ struct lym_op {
     int diag_dirty;
     double2* tmp;      // full-spinor scratch for squared operators
     double2* tmp2;     // half-spinor scratch for the Schur operator
+    // low-precision copies for mixed-precision solvers (sloppy_prec 1 FP32, 2 FP16; 0 none)
+    int sloppy_prec;
+    void* Us;          // links
+    void* As;          // [2][36][stride] blocks
+    float* As_sc;      // [2][stride]
+    void* Ainvs;       // [36][stride]
+    float* Ainvs_sc;   // [stride]
 };

@@ -103,6 +103,28 @@ void launch_bicg_update(BiCGState* s, zd* x, zd* r, const zd* p, const zd* sv, c
 }  // namespace lym
 
 namespace lym {
+struct MCGState {
+    int done, converged, step, max_steps, rel, skip_rel, nrel, pad;
+    double rr_s, rr_true, maxrr, alpha, beta, eps, delta2;
+};
+
+// low-precision copies of operator data (mixed.hip); prec 1 FP32, 2 FP16
+void launch_convert_links(int prec, const void* src, void* dst, long n, int real_links, hipStream_t st);
+void launch_convert_blocks(int prec, const zd* src, void* dst, float* sc, int nparity, int stride, hipStream_t st);
+// mixed-precision CG with reliable updates; vectors span nsites = nparity * stride sites
+void launch_mcg_init(int prec, MCGState* s, const zd* r, cplx<float>* rs, cplx<float>* xs, void* p, float* p_sc,
+                     long nsites, int stride, double* partials, double eps, double delta, int max_steps, hipStream_t st);
+void launch_mcg_alpha(int prec, MCGState* s, void* p, float* p_sc, void* Ap, float* Ap_sc, long nsites, int stride,
+                      double* partials, hipStream_t st);
+void launch_mcg_update(int prec, MCGState* s, cplx<float>* xs, cplx<float>* rs, void* p, float* p_sc, void* Ap,
+                       float* Ap_sc, long nsites, int stride, double* partials, hipStream_t st);
+void launch_rel_accum(MCGState* s, zd* x, cplx<float>* xs, long nsites, int stride, hipStream_t st);
+void launch_accum_xs(zd* x, const cplx<float>* xs, long nsites, int stride, hipStream_t st);
+void launch_rel_resid(MCGState* s, zd* r, cplx<float>* rs, const zd* b, const zd* Ax, long nsites, int stride,
+                      double* partials, hipStream_t st);
+void launch_mcg_p(int prec, MCGState* s, void* p, float* p_sc, const cplx<float>* rs, long nsites, int stride,
+                  hipStream_t st);
+
 // x = uniform [0,1) complex on real sites, zero on padding (BiCGStab start for a zero source)
 void launch_fill_uniform(zd* x, int nparity, int stride, int Vh, uint64_t seed, hipStream_t st);
 }  // namespace lym

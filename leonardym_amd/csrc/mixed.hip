@@ -1,0 +1,457 @@
+// mixed.hip -- low-precision copies of the operator data and the kernels of the
+// mixed-precision CG with reliable updates.
+//
+// The sloppy operator computes in FP32 on FP32 or FP16 storage (FP16 spinors
+// and clover blocks carry one FP32 scale per site; links are unitary or
+// orthogonal and are stored unscaled).  The Krylov recursion keeps x_s, r_s in
+// FP32 and p, Ap in the sloppy storage; every dot product is accumulated in
+// FP64.  When the sloppy residual has dropped by `delta` since the last reliable
+// update (or looks converged), x += x_s in FP64 and the true residual
+// r = b - A x is recomputed with the FP64 operator; convergence is declared
+// only on that true residual, so the FP64 stopping rule |r|^2 < eps of the
+// reference CG (source/inverters/ConjugateGradient.cpp:49-50) holds exactly.
+// The search direction p is kept across updates (reliable updates in the sense
+// of Sleijpen and van der Vorst, 1996).
+#include "lym_device.h"
+
+namespace lym {
+
+constexpr int MR_THREADS = 256;
+
+template <int NV>
+__device__ __forceinline__ void m_block_store(double v[NV], double* partials) {
+    __shared__ double sh[NV][MR_THREADS / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] += __shfl_xor(v[k], off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) sh[k][w] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double s = 0;
+#pragma unroll
+            for (int j = 0; j < MR_THREADS / 64; ++j) s += sh[k][j];
+            partials[k * RED_BLOCKS + blockIdx.x] = s;
+        }
+}
+
+template <int NV>
+__device__ __forceinline__ void m_final_sum(const double* partials, double out[NV]) {
+    __shared__ double sh[NV][RED_BLOCKS / 64];
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = partials[k * RED_BLOCKS + threadIdx.x];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] += __shfl_xor(v[k], off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) sh[k][w] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double s = 0;
+#pragma unroll
+            for (int j = 0; j < RED_BLOCKS / 64; ++j) s += sh[k][j];
+            out[k] = s;
+        }
+}
+
+#define SITE_LOOP(j, n) for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < (n); j += (long)gridDim.x * blockDim.x)
+
+// A sloppy spinor vector over nsites = nparity * stride sites: site j lives in
+// parity j / stride at index j % stride.
+template <int PR>
+struct SVec {
+    typename Prec<PR>::S* d;  // [nparity][12][stride]
+    float* sc;                // [nparity][stride] (FP16)
+};
+
+template <int PR>
+__device__ __forceinline__ void sv_load(const SVec<PR>& v, int stride, long j, cplx<float> s[12]) {
+    const long p = j / stride, i = j % stride;
+    const typename Prec<PR>::S* b = v.d + p * 12 * stride + i;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+        cplx<typename Prec<PR>::T> z = to_c<PR>(b[(long)k * stride]);
+        s[k] = mk((float)z.re, (float)z.im);
+    }
+    if constexpr (PR == 2) {
+        const float sc = v.sc[j];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) s[k] = sc * s[k];
+    }
+}
+
+template <int PR>
+__device__ __forceinline__ void sv_store(const SVec<PR>& v, int stride, long j, const cplx<float> s[12]) {
+    const long p = j / stride, i = j % stride;
+    typename Prec<PR>::S* b = v.d + p * 12 * stride + i;
+    if constexpr (PR == 2) {
+        float m = 0.f;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) m = fmaxf(m, fmaxf(fabsf(s[k].re), fabsf(s[k].im)));
+        const float inv = m > 0.f ? 1.0f / m : 0.f;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) b[(long)k * stride] = from_c<PR>(inv * s[k]);
+        v.sc[j] = m;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) b[(long)k * stride] = from_c<PR>(s[k]);
+    }
+}
+
+__device__ __forceinline__ void f_load(const cplx<float>* v, int stride, long j, cplx<float> s[12]) {
+    const long p = j / stride, i = j % stride;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) s[k] = v[p * 12 * stride + (long)k * stride + i];
+}
+__device__ __forceinline__ void f_store(cplx<float>* v, int stride, long j, const cplx<float> s[12]) {
+    const long p = j / stride, i = j % stride;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[p * 12 * stride + (long)k * stride + i] = s[k];
+}
+__device__ __forceinline__ void d_load(const zd* v, int stride, long j, zd s[12]) {
+    const long p = j / stride, i = j % stride;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) s[k] = v[p * 12 * stride + (long)k * stride + i];
+}
+__device__ __forceinline__ void d_store(zd* v, int stride, long j, const zd s[12]) {
+    const long p = j / stride, i = j % stride;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[p * 12 * stride + (long)k * stride + i] = s[k];
+}
+
+// ---------------------------------------------------------------- operator data conversion
+template <int PR>
+__global__ void k_convert_links(const void* src, void* dst, long n, int real_links) {
+    SITE_LOOP(i, n) {
+        if (real_links) {
+            reinterpret_cast<typename Prec<PR>::R*>(dst)[i] =
+                (typename Prec<PR>::R)reinterpret_cast<const double*>(src)[i];
+        } else {
+            zd z = reinterpret_cast<const zd*>(src)[i];
+            reinterpret_cast<typename Prec<PR>::S*>(dst)[i] =
+                from_c<PR>(mk((typename Prec<PR>::T)z.re, (typename Prec<PR>::T)z.im));
+        }
+    }
+}
+
+// blocks [nparity][36][stride] -> PR storage with one scale per site
+template <int PR>
+__global__ void k_convert_blocks(const zd* src, typename Prec<PR>::S* dst, float* sc, int nparity, int stride) {
+    SITE_LOOP(j, (long)nparity * stride) {
+        const long p = j / stride, i = j % stride;
+        const zd* s = src + p * 36 * stride + i;
+        typename Prec<PR>::S* d = dst + p * 36 * stride + i;
+        float m = 1.f;
+        if constexpr (PR == 2) {
+            m = 0.f;
+            for (int k = 0; k < 36; ++k) {
+                zd z = s[(long)k * stride];
+                m = fmaxf(m, fmaxf(fabsf((float)z.re), fabsf((float)z.im)));
+            }
+            sc[j] = m;
+        }
+        const float inv = m > 0.f ? 1.0f / m : 0.f;
+        for (int k = 0; k < 36; ++k) {
+            zd z = s[(long)k * stride];
+            d[(long)k * stride] = from_c<PR>(mk((typename Prec<PR>::T)(inv * z.re), (typename Prec<PR>::T)(inv * z.im)));
+        }
+    }
+}
+
+static inline int site_blocks(long n) {
+    long b = (n + 255) / 256;
+    return (int)(b < 4096 ? b : 4096);
+}
+
+void launch_convert_links(int prec, const void* src, void* dst, long n, int real_links, hipStream_t st) {
+    if (prec == 1) hipLaunchKernelGGL(k_convert_links<1>, dim3(site_blocks(n)), dim3(256), 0, st, src, dst, n, real_links);
+    else hipLaunchKernelGGL(k_convert_links<2>, dim3(site_blocks(n)), dim3(256), 0, st, src, dst, n, real_links);
+}
+
+void launch_convert_blocks(int prec, const zd* src, void* dst, float* sc, int nparity, int stride, hipStream_t st) {
+    long n = (long)nparity * stride;
+    if (prec == 1)
+        hipLaunchKernelGGL(k_convert_blocks<1>, dim3(site_blocks(n)), dim3(256), 0, st, src,
+                           (typename Prec<1>::S*)dst, sc, nparity, stride);
+    else
+        hipLaunchKernelGGL(k_convert_blocks<2>, dim3(site_blocks(n)), dim3(256), 0, st, src,
+                           (typename Prec<2>::S*)dst, sc, nparity, stride);
+}
+
+// ---------------------------------------------------------------- mixed CG
+// r_s = r (FP32), p = r (sloppy), x_s = 0, partial |r|^2
+template <int PR>
+__global__ __launch_bounds__(MR_THREADS) void k_mcg_init(const zd* r, cplx<float>* rs, cplx<float>* xs, SVec<PR> p,
+                                                         long nsites, int stride, double* partials) {
+    double v[1] = {0};
+    SITE_LOOP(j, nsites) {
+        zd a[12];
+        cplx<float> f[12], z[12];
+        d_load(r, stride, j, a);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            v[0] += a[k].re * a[k].re + a[k].im * a[k].im;
+            f[k] = mk((float)a[k].re, (float)a[k].im);
+            z[k] = mk(0.f, 0.f);
+        }
+        f_store(rs, stride, j, f);
+        f_store(xs, stride, j, z);
+        sv_store<PR>(p, stride, j, f);
+    }
+    m_block_store<1>(v, partials);
+}
+
+__global__ __launch_bounds__(RED_BLOCKS) void k_mcg_init_final(MCGState* s, const double* partials, double eps,
+                                                               double delta, int max_steps) {
+    double out[1];
+    m_final_sum<1>(partials, out);
+    if (threadIdx.x == 0) {
+        s->rr_s = out[0];
+        s->rr_true = out[0];
+        s->maxrr = out[0];
+        s->eps = eps;
+        s->delta2 = delta * delta;
+        s->max_steps = max_steps;
+        s->step = 0;
+        s->nrel = 0;
+        s->rel = 0;
+        s->skip_rel = 1;
+        s->converged = 0;
+        s->done = max_steps <= 0 ? 1 : 0;
+        s->alpha = s->beta = 0;
+    }
+}
+
+// <p, Ap> accumulated in FP64
+template <int PR>
+__global__ __launch_bounds__(MR_THREADS) void k_mcg_dot(const MCGState* s, SVec<PR> p, SVec<PR> Ap, long nsites,
+                                                        int stride, double* partials) {
+    if (s->done) return;
+    double v[1] = {0};
+    SITE_LOOP(j, nsites) {
+        cplx<float> a[12], b[12];
+        sv_load<PR>(p, stride, j, a);
+        sv_load<PR>(Ap, stride, j, b);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) v[0] += (double)a[k].re * b[k].re + (double)a[k].im * b[k].im;
+    }
+    m_block_store<1>(v, partials);
+}
+
+__global__ __launch_bounds__(RED_BLOCKS) void k_mcg_alpha_final(MCGState* s, const double* partials) {
+    if (s->done) return;
+    double out[1];
+    m_final_sum<1>(partials, out);
+    if (threadIdx.x == 0) s->alpha = s->rr_s / out[0];
+}
+
+// x_s += alpha p, r_s -= alpha Ap, partial |r_s|^2
+template <int PR>
+__global__ __launch_bounds__(MR_THREADS) void k_mcg_update(const MCGState* s, cplx<float>* xs, cplx<float>* rs,
+                                                           SVec<PR> p, SVec<PR> Ap, long nsites, int stride,
+                                                           double* partials) {
+    if (s->done) return;
+    const float al = (float)s->alpha;
+    double v[1] = {0};
+    SITE_LOOP(j, nsites) {
+        cplx<float> x[12], r[12], a[12], b[12];
+        f_load(xs, stride, j, x);
+        f_load(rs, stride, j, r);
+        sv_load<PR>(p, stride, j, a);
+        sv_load<PR>(Ap, stride, j, b);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            x[k] = x[k] + al * a[k];
+            r[k] = r[k] - al * b[k];
+            v[0] += (double)r[k].re * r[k].re + (double)r[k].im * r[k].im;
+        }
+        f_store(xs, stride, j, x);
+        f_store(rs, stride, j, r);
+    }
+    m_block_store<1>(v, partials);
+}
+
+__global__ __launch_bounds__(RED_BLOCKS) void k_mcg_update_final(MCGState* s, const double* partials) {
+    if (s->done) return;
+    double out[1];
+    m_final_sum<1>(partials, out);
+    if (threadIdx.x == 0) {
+        const double rn = out[0];
+        if (rn > s->maxrr) s->maxrr = rn;
+        if (rn < s->delta2 * s->maxrr || rn < s->eps) {
+            s->rel = 1;
+            s->skip_rel = 0;
+            return;  // the reliable update finishes the iteration
+        }
+        s->beta = rn / s->rr_s;
+        s->rr_s = rn;
+        s->step += 1;
+        if (s->step >= s->max_steps) s->done = 1;
+    }
+}
+
+// reliable update, part 1: x += x_s, x_s = 0
+__global__ void k_rel_accum(const MCGState* s, zd* x, cplx<float>* xs, long nsites, int stride) {
+    if (s->skip_rel || s->done) return;
+    SITE_LOOP(j, nsites) {
+        zd a[12];
+        cplx<float> f[12];
+        d_load(x, stride, j, a);
+        f_load(xs, stride, j, f);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            a[k] = a[k] + mk((double)f[k].re, (double)f[k].im);
+            f[k] = mk(0.f, 0.f);
+        }
+        d_store(x, stride, j, a);
+        f_store(xs, stride, j, f);
+    }
+}
+
+// x += x_s unconditionally (end of a solve that stopped between reliable updates)
+__global__ void k_accum_xs(zd* x, const cplx<float>* xs, long nsites, int stride) {
+    SITE_LOOP(j, nsites) {
+        zd a[12];
+        cplx<float> f[12];
+        d_load(x, stride, j, a);
+        f_load(xs, stride, j, f);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) a[k] = a[k] + mk((double)f[k].re, (double)f[k].im);
+        d_store(x, stride, j, a);
+    }
+}
+
+// reliable update, part 2: r = b - Ax (FP64), r_s = r, partial |r|^2
+__global__ __launch_bounds__(MR_THREADS) void k_rel_resid(const MCGState* s, zd* r, cplx<float>* rs, const zd* b,
+                                                          const zd* Ax, long nsites, int stride, double* partials) {
+    if (s->skip_rel || s->done) return;
+    double v[1] = {0};
+    SITE_LOOP(j, nsites) {
+        zd bb[12], ax[12];
+        cplx<float> f[12];
+        d_load(b, stride, j, bb);
+        d_load(Ax, stride, j, ax);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            bb[k] = bb[k] - ax[k];
+            v[0] += bb[k].re * bb[k].re + bb[k].im * bb[k].im;
+            f[k] = mk((float)bb[k].re, (float)bb[k].im);
+        }
+        d_store(r, stride, j, bb);
+        f_store(rs, stride, j, f);
+    }
+    m_block_store<1>(v, partials);
+}
+
+__global__ __launch_bounds__(RED_BLOCKS) void k_rel_final(MCGState* s, const double* partials) {
+    if (s->skip_rel || s->done) return;
+    double out[1];
+    m_final_sum<1>(partials, out);
+    if (threadIdx.x == 0) {
+        const double rt = out[0];
+        s->rr_true = rt;
+        s->nrel += 1;
+        s->rel = 0;
+        s->skip_rel = 1;
+        s->step += 1;
+        if (rt < s->eps) {
+            s->done = 1;
+            s->converged = 1;
+            return;
+        }
+        s->beta = rt / s->rr_s;
+        s->rr_s = rt;
+        s->maxrr = rt;
+        if (s->step >= s->max_steps) s->done = 1;
+    }
+}
+
+// p = r_s + beta p
+template <int PR>
+__global__ void k_mcg_p(const MCGState* s, SVec<PR> p, const cplx<float>* rs, long nsites, int stride) {
+    if (s->done) return;
+    const float be = (float)s->beta;
+    SITE_LOOP(j, nsites) {
+        cplx<float> a[12], r[12];
+        sv_load<PR>(p, stride, j, a);
+        f_load(rs, stride, j, r);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) a[k] = r[k] + be * a[k];
+        sv_store<PR>(p, stride, j, a);
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+template <int PR>
+static SVec<PR> sv(void* d, float* sc) {
+    return SVec<PR>{reinterpret_cast<typename Prec<PR>::S*>(d), sc};
+}
+
+void launch_mcg_init(int prec, MCGState* s, const zd* r, cplx<float>* rs, cplx<float>* xs, void* p, float* p_sc,
+                     long nsites, int stride, double* partials, double eps, double delta, int max_steps,
+                     hipStream_t st) {
+    if (prec == 1)
+        hipLaunchKernelGGL(k_mcg_init<1>, dim3(RED_BLOCKS), dim3(MR_THREADS), 0, st, r, rs, xs, sv<1>(p, p_sc), nsites,
+                           stride, partials);
+    else
+        hipLaunchKernelGGL(k_mcg_init<2>, dim3(RED_BLOCKS), dim3(MR_THREADS), 0, st, r, rs, xs, sv<2>(p, p_sc), nsites,
+                           stride, partials);
+    hipLaunchKernelGGL(k_mcg_init_final, dim3(1), dim3(RED_BLOCKS), 0, st, s, partials, eps, delta, max_steps);
+}
+
+void launch_mcg_alpha(int prec, MCGState* s, void* p, float* p_sc, void* Ap, float* Ap_sc, long nsites, int stride,
+                      double* partials, hipStream_t st) {
+    if (prec == 1)
+        hipLaunchKernelGGL(k_mcg_dot<1>, dim3(RED_BLOCKS), dim3(MR_THREADS), 0, st, s, sv<1>(p, p_sc), sv<1>(Ap, Ap_sc),
+                           nsites, stride, partials);
+    else
+        hipLaunchKernelGGL(k_mcg_dot<2>, dim3(RED_BLOCKS), dim3(MR_THREADS), 0, st, s, sv<2>(p, p_sc), sv<2>(Ap, Ap_sc),
+                           nsites, stride, partials);
+    hipLaunchKernelGGL(k_mcg_alpha_final, dim3(1), dim3(RED_BLOCKS), 0, st, s, partials);
+}
+
+void launch_mcg_update(int prec, MCGState* s, cplx<float>* xs, cplx<float>* rs, void* p, float* p_sc, void* Ap,
+                       float* Ap_sc, long nsites, int stride, double* partials, hipStream_t st) {
+    if (prec == 1)
+        hipLaunchKernelGGL(k_mcg_update<1>, dim3(RED_BLOCKS), dim3(MR_THREADS), 0, st, s, xs, rs, sv<1>(p, p_sc),
+                           sv<1>(Ap, Ap_sc), nsites, stride, partials);
+    else
+        hipLaunchKernelGGL(k_mcg_update<2>, dim3(RED_BLOCKS), dim3(MR_THREADS), 0, st, s, xs, rs, sv<2>(p, p_sc),
+                           sv<2>(Ap, Ap_sc), nsites, stride, partials);
+    hipLaunchKernelGGL(k_mcg_update_final, dim3(1), dim3(RED_BLOCKS), 0, st, s, partials);
+}
+
+void launch_rel_accum(MCGState* s, zd* x, cplx<float>* xs, long nsites, int stride, hipStream_t st) {
+    hipLaunchKernelGGL(k_rel_accum, dim3(site_blocks(nsites)), dim3(256), 0, st, s, x, xs, nsites, stride);
+}
+
+void launch_accum_xs(zd* x, const cplx<float>* xs, long nsites, int stride, hipStream_t st) {
+    hipLaunchKernelGGL(k_accum_xs, dim3(site_blocks(nsites)), dim3(256), 0, st, x, xs, nsites, stride);
+}
+
+void launch_rel_resid(MCGState* s, zd* r, cplx<float>* rs, const zd* b, const zd* Ax, long nsites, int stride,
+                      double* partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_rel_resid, dim3(RED_BLOCKS), dim3(MR_THREADS), 0, st, s, r, rs, b, Ax, nsites, stride, partials);
+    hipLaunchKernelGGL(k_rel_final, dim3(1), dim3(RED_BLOCKS), 0, st, s, partials);
+}
+
+void launch_mcg_p(int prec, MCGState* s, void* p, float* p_sc, const cplx<float>* rs, long nsites, int stride,
+                  hipStream_t st) {
+    if (prec == 1)
+        hipLaunchKernelGGL(k_mcg_p<1>, dim3(site_blocks(nsites)), dim3(256), 0, st, s, sv<1>(p, p_sc), rs, nsites, stride);
+    else
+        hipLaunchKernelGGL(k_mcg_p<2>, dim3(site_blocks(nsites)), dim3(256), 0, st, s, sv<2>(p, p_sc), rs, nsites, stride);
+}
+
+}  // namespace lym

@@ -31,6 +31,11 @@ int op_ready(lym_op* op, bool force_blocks);
 void apply_schur(lym_op* op, zd* out_o, const zd* in_o, int gamma5, const int* skip);
 void apply_full(lym_op* op, zd* out, const zd* in, int gamma5, const int* skip);
 int link_kind_of(const lym_gauge* g);
+int op_sloppy(lym_op* op, int prec);
+void apply_full_sloppy(lym_op* op, int prec, void* out, float* out_sc, const void* in, const float* in_sc, int gamma5,
+                       const int* skip);
+void apply_schur_sloppy(lym_op* op, int prec, void* out_o, float* out_sc, const void* in_o, const float* in_sc,
+                        void* tmp_e, float* tmp_sc, int gamma5, const int* skip);
 }  // namespace lym_internal
 
 using lym_internal::fail;
@@ -44,6 +49,14 @@ struct lym_solver {
     zd* w[8];
     void* state;
     void* host_state;
+    // mixed precision
+    int sloppy_prec;
+    double delta;
+    int mixed_prec_alloc;   // precision the sloppy buffers below were sized for
+    void* sv[4];            // p, Ap, Q^2 intermediate, Schur intermediate (full-size sloppy fields)
+    float* svsc[4];         // their per-site scales [2][stride]
+    lym::cplx<float>* xs;   // FP32 x_s, r_s [2][12][stride]
+    lym::cplx<float>* rs;
 };
 
 namespace {
@@ -210,6 +223,111 @@ int run_bicgstab(lym_solver* s, const LinOp& L, const zd* b, zd* xout, const zd*
     return LYM_OK;
 }
 
+int ensure_mixed_work(lym_solver* s, int prec) {
+    if (s->mixed_prec_alloc == prec) return LYM_OK;
+    lym_ctx* c = s->ctx;
+    for (int i = 0; i < 4; ++i) {
+        if (s->sv[i]) (void)hipFree(s->sv[i]);
+        if (s->svsc[i]) (void)hipFree(s->svsc[i]);
+        s->sv[i] = nullptr;
+        s->svsc[i] = nullptr;
+    }
+    if (s->xs) (void)hipFree(s->xs);
+    if (s->rs) (void)hipFree(s->rs);
+    s->xs = s->rs = nullptr;
+    const size_t S = c->g.stride;
+    const size_t es = prec == 1 ? 8 : 4;
+    for (int i = 0; i < 4; ++i) {
+        if (hipMalloc(&s->sv[i], 24 * S * es) != hipSuccess) return fail(LYM_ERR_NOMEM, "sloppy vectors");
+        if (hipMalloc((void**)&s->svsc[i], 2 * S * sizeof(float)) != hipSuccess) return fail(LYM_ERR_NOMEM, "scales");
+        if (hipMemsetAsync(s->sv[i], 0, 24 * S * es, c->stream) != hipSuccess ||
+            hipMemsetAsync(s->svsc[i], 0, 2 * S * sizeof(float), c->stream) != hipSuccess)
+            return fail(LYM_ERR_DEVICE, "hipMemsetAsync");
+    }
+    if (hipMalloc((void**)&s->xs, 24 * S * 8) != hipSuccess || hipMalloc((void**)&s->rs, 24 * S * 8) != hipSuccess)
+        return fail(LYM_ERR_NOMEM, "FP32 vectors");
+    if (hipMemsetAsync(s->xs, 0, 24 * S * 8, c->stream) != hipSuccess ||
+        hipMemsetAsync(s->rs, 0, 24 * S * 8, c->stream) != hipSuccess)
+        return fail(LYM_ERR_DEVICE, "hipMemsetAsync");
+    s->mixed_prec_alloc = prec;
+    return LYM_OK;
+}
+
+// the sloppy counterpart of a LinOp
+struct SloppyOp {
+    lym_op* op;
+    bool half;
+    int gamma5;
+    unsigned power;
+    int prec;
+    void* mid;     // Q^2 intermediate
+    float* mid_sc;
+    void* tmp_e;   // Schur intermediate
+    float* tmp_sc;
+    void apply(void* out, float* out_sc, const void* in, const float* in_sc, const int* skip) const {
+        auto one = [&](void* o, float* osc, const void* i, const float* isc) {
+            if (half) lym_internal::apply_schur_sloppy(op, prec, o, osc, i, isc, tmp_e, tmp_sc, gamma5, skip);
+            else lym_internal::apply_full_sloppy(op, prec, o, osc, i, isc, gamma5, skip);
+        };
+        if (power == 1) {
+            one(out, out_sc, in, in_sc);
+        } else {
+            one(mid, mid_sc, in, in_sc);
+            one(out, out_sc, mid, mid_sc);
+        }
+    }
+};
+
+// Mixed-precision CG on L with reliable updates (see mixed.hip); vectors of
+// nparity parities
+int run_mixed_cg(lym_solver* s, const LinOp& L, const SloppyOp& SL, const zd* b, zd* xout, const zd* x0, int nparity,
+                 lym_solve_info* info) {
+    lym_ctx* c = s->ctx;
+    hipStream_t st = c->stream;
+    const long n = (long)nparity * 12 * c->g.stride;
+    const long nsites = (long)nparity * c->g.stride;
+    const int prec = SL.prec;
+    zd *x = s->w[0], *r = s->w[1], *Ax = s->w[3];
+    void *p = s->sv[0], *Ap = s->sv[1];
+    float *p_sc = s->svsc[0], *Ap_sc = s->svsc[1];
+    auto* S = reinterpret_cast<lym::MCGState*>(s->state);
+    auto* H = reinterpret_cast<lym::MCGState*>(s->host_state);
+    lym::launch_copy(x, x0 ? x0 : b, n, st);
+    L.apply(Ax, x, nullptr);
+    lym::launch_sub(r, b, Ax, n, st);
+    lym::launch_mcg_init(prec, S, r, s->rs, s->xs, p, p_sc, nsites, c->g.stride, c->red_partials, s->precision,
+                         s->delta, (int)s->max_steps, st);
+    HIPCHECK(hipMemcpyAsync(H, S, sizeof(lym::MCGState), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    const double r0 = H->rr_s;
+    unsigned queued = 0;
+    while (!H->done && queued <= s->max_steps) {
+        unsigned k = next_chunk(r0, H->rr_s, (unsigned)H->step, s->precision, s->max_steps + 1 - queued);
+        for (unsigned i = 0; i < k; ++i) {
+            SL.apply(Ap, Ap_sc, p, p_sc, &S->done);
+            lym::launch_mcg_alpha(prec, S, p, p_sc, Ap, Ap_sc, nsites, c->g.stride, c->red_partials, st);
+            lym::launch_mcg_update(prec, S, s->xs, s->rs, p, p_sc, Ap, Ap_sc, nsites, c->g.stride, c->red_partials, st);
+            lym::launch_rel_accum(S, x, s->xs, nsites, c->g.stride, st);
+            L.apply(Ax, x, &S->skip_rel);
+            lym::launch_rel_resid(S, r, s->rs, b, Ax, nsites, c->g.stride, c->red_partials, st);
+            lym::launch_mcg_p(prec, S, p, p_sc, s->rs, nsites, c->g.stride, st);
+        }
+        queued += k;
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(H, S, sizeof(lym::MCGState), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+    if (!H->converged) lym::launch_accum_xs(x, s->xs, nsites, c->g.stride, st);
+    lym::launch_copy(xout, x, n, st);
+    HIPCHECK(hipGetLastError());
+    info->converged = H->converged;
+    info->steps = H->converged ? (unsigned)H->step : s->max_steps;
+    info->last_error = H->rr_true;
+    info->applies = 1 + (unsigned)H->step + (unsigned)H->nrel;
+    info->reliable_updates = (unsigned)H->nrel;
+    return LYM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -217,15 +335,19 @@ extern "C" {
 int lym_solver_create(lym_ctx* c, int kind, lym_solver** out) {
     if (!c || !out) return fail(LYM_ERR_INVALID, "null argument");
     *out = nullptr;
-    if (kind < LYM_SOLVER_CG || kind > LYM_SOLVER_PRECOND_BICGSTAB) return fail(LYM_ERR_INVALID, "unknown solver %d", kind);
+    if (kind < LYM_SOLVER_CG || kind > LYM_SOLVER_PRECOND_MIXED_CG) return fail(LYM_ERR_INVALID, "unknown solver %d", kind);
     lym_solver* s = new lym_solver();
     std::memset(s, 0, sizeof(*s));
     s->ctx = c;
     s->kind = kind;
     // defaults: ConjugateGradient.cpp:6 and Solver.h:10
-    s->precision = kind == LYM_SOLVER_CG ? 1e-11 : 1e-10;
-    s->max_steps = kind == LYM_SOLVER_CG ? 3000 : 1000;
+    const bool cg = kind == LYM_SOLVER_CG || kind == LYM_SOLVER_MIXED_CG || kind == LYM_SOLVER_PRECOND_CG ||
+                    kind == LYM_SOLVER_PRECOND_MIXED_CG;
+    s->precision = cg ? 1e-11 : 1e-10;
+    s->max_steps = cg ? 3000 : 1000;
     s->even_odd = 1;
+    s->sloppy_prec = LYM_PREC_HALF;
+    s->delta = 0.1;
     *out = s;
     return LYM_OK;
 }
@@ -236,6 +358,12 @@ void lym_solver_destroy(lym_solver* s) {
     for (int i = 0; i < 8; ++i)
         if (s->w[i]) (void)hipFree(s->w[i]);
     if (s->state) (void)hipFree(s->state);
+    for (int i = 0; i < 4; ++i) {
+        if (s->sv[i]) (void)hipFree(s->sv[i]);
+        if (s->svsc[i]) (void)hipFree(s->svsc[i]);
+    }
+    if (s->xs) (void)hipFree(s->xs);
+    if (s->rs) (void)hipFree(s->rs);
     if (s->host_state) (void)hipHostFree(s->host_state);
     delete s;
 }
@@ -249,6 +377,20 @@ int lym_solver_set_precision(lym_solver* s, double precision) {
 int lym_solver_set_max_steps(lym_solver* s, unsigned max_steps) {
     if (!s) return fail(LYM_ERR_INVALID, "null solver");
     s->max_steps = max_steps;
+    return LYM_OK;
+}
+
+int lym_solver_set_sloppy_precision(lym_solver* s, int prec) {
+    if (!s) return fail(LYM_ERR_INVALID, "null solver");
+    if (prec != LYM_PREC_SINGLE && prec != LYM_PREC_HALF) return fail(LYM_ERR_INVALID, "sloppy precision %d", prec);
+    s->sloppy_prec = prec;
+    return LYM_OK;
+}
+
+int lym_solver_set_reliable_delta(lym_solver* s, double delta) {
+    if (!s) return fail(LYM_ERR_INVALID, "null solver");
+    if (!(delta > 0 && delta < 1)) return fail(LYM_ERR_INVALID, "reliable delta must lie in (0, 1)");
+    s->delta = delta;
     return LYM_OK;
 }
 
@@ -267,13 +409,21 @@ int lym_solver_solve(lym_solver* s, lym_op* op, const lym_spinor* source, lym_sp
     lym_solve_info local;
     if (!info) info = &local;
     std::memset(info, 0, sizeof(*info));
-    bool eo_driver = s->kind == LYM_SOLVER_PRECOND_BICGSTAB && s->even_odd && op->power == 1 &&
+    const bool precond_kind = s->kind == LYM_SOLVER_PRECOND_BICGSTAB || s->kind == LYM_SOLVER_PRECOND_CG ||
+                              s->kind == LYM_SOLVER_PRECOND_MIXED_CG;
+    const bool mixed = s->kind == LYM_SOLVER_MIXED_CG || s->kind == LYM_SOLVER_PRECOND_MIXED_CG;
+    bool eo_driver = precond_kind && s->even_odd && op->power == 1 &&
                      (op->kind == LYM_OP_CLOVER || op->kind == LYM_OP_EO_CLOVER) && c->dims[0] % 2 == 0 &&
                      c->dims[1] % 2 == 0 && c->dims[2] % 2 == 0 && c->dims[3] % 2 == 0;
     int rc = lym_internal::op_ready(op, eo_driver);
     if (rc) return rc;
     rc = ensure_work(s);
     if (rc) return rc;
+    if (mixed) {
+        rc = lym_internal::op_sloppy(op, s->sloppy_prec);
+        if (!rc) rc = ensure_mixed_work(s, s->sloppy_prec);
+        if (rc) return rc;
+    }
     hipStream_t st = c->stream;
     HIPCHECK(hipEventRecord(c->ev_start, st));
     const zd* b = reinterpret_cast<const zd*>(source->d);
@@ -302,16 +452,24 @@ int lym_solver_solve(lym_solver* s, lym_op* op, const lym_spinor* source, lym_sp
         lym::launch_dslash(a, lym_internal::link_kind_of(op->gauge), 1, false, st);
         lym::launch_scale_g5(src, 1, c->g.stride, st);
         lym_internal::apply_schur(op, rhs, src, 1, nullptr);
-        double be = read_norm2(c, par(b, c, 0), half_n(c));
-        // inner BiCGStab on Qhat^2 with gamma5 = 1 (the driver's own EO operators);
+        // inner solve on Qhat^2 with gamma5 = 1 (the driver's own EO operators);
         // it uses w[0..6], the Q^2 intermediate lives in the operator's scratch
         LinOp L{op, true, 1, 2, reinterpret_cast<zd*>(op->tmp)};
-        rc = run_bicgstab(s, L, rhs, xo, x0 ? par(x0, c, 1) : nullptr, half_n(c), be, info);
+        const zd* g0 = x0 ? par(x0, c, 1) : nullptr;
+        if (s->kind == LYM_SOLVER_PRECOND_BICGSTAB) {
+            double be = read_norm2(c, par(b, c, 0), half_n(c));
+            rc = run_bicgstab(s, L, rhs, xo, g0, half_n(c), be, info);
+        } else if (s->kind == LYM_SOLVER_PRECOND_CG) {
+            rc = run_cg(s, L, rhs, xo, g0, half_n(c), info);
+        } else {
+            SloppyOp SL{op, true, 1, 2, s->sloppy_prec, s->sv[2], s->svsc[2], s->sv[3], s->svsc[3]};
+            rc = run_mixed_cg(s, L, SL, rhs, xo, g0, 1, info);
+        }
         if (rc) return rc;
         std::memset(&a, 0, sizeof(a));
         a.g = c->g;
         a.links = op->gauge->U;
-        a.Ainv = reinterpret_cast<const zd*>(op->Ainv);
+        a.Ainv = op->Ainv;
         a.kappa = op->kappa;
         a.hop_sign = -1.0;
         a.parity = 0;
@@ -325,8 +483,14 @@ int lym_solver_solve(lym_solver* s, lym_op* op, const lym_spinor* source, lym_sp
         const zd* bb = half ? par(b, c, 1) : b;
         zd* xx = half ? par(x, c, 1) : x;
         const zd* g0 = x0 ? (half ? par(x0, c, 1) : x0) : nullptr;
-        if (s->kind == LYM_SOLVER_CG) rc = run_cg(s, L, bb, xx, g0, n, info);
-        else rc = run_bicgstab(s, L, bb, xx, g0, n, half ? read_norm2(c, b, half_n(c)) : 0.0, info);
+        if (mixed) {
+            SloppyOp SL{op, half, op->gamma5, op->power, s->sloppy_prec, s->sv[2], s->svsc[2], s->sv[3], s->svsc[3]};
+            rc = run_mixed_cg(s, L, SL, bb, xx, g0, half ? 1 : 2, info);
+        } else if (s->kind == LYM_SOLVER_CG || s->kind == LYM_SOLVER_PRECOND_CG) {
+            rc = run_cg(s, L, bb, xx, g0, n, info);
+        } else {
+            rc = run_bicgstab(s, L, bb, xx, g0, n, half ? read_norm2(c, b, half_n(c)) : 0.0, info);
+        }
         if (rc) return rc;
         if (half) HIPCHECK(hipMemcpyAsync(x, b, sizeof(zd) * half_n(c), hipMemcpyDeviceToDevice, st));
     }

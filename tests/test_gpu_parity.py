@@ -278,3 +278,48 @@ def test_fresh_buffers_are_ordered_before_first_use():
         op.multiply(b, a)
         orc = OracleField(dims, U, kappa=0.13, csw=1.0, need_clover=True)
         assert rel_err_per_site(b.download(), orc.apply(K_EO, psi)) < SITE_TOL
+
+
+@pytest.mark.parametrize("sloppy", [lym.PREC_SINGLE, lym.PREC_HALF])
+def test_mixed_cg_on_square_clover(case, sloppy):
+    """Mixed-precision CG (FP32 arithmetic, FP32/FP16 storage, FP64 reliable
+    updates) reaches the FP64 stopping rule and the FP64 oracle solution."""
+    b = case.gauss()
+    rhs = case.orc.apply(K_CLOVER, g5(b))
+    eps = (1e-11 * np.linalg.norm(rhs)) ** 2
+    solver = lym.Solver(case.ctx, lym.SOLVER_MIXED_CG, precision=eps, max_steps=20000, sloppy=sloppy)
+    X = case.spinor()
+    info = solver.solve(case.op(lym.OP_CLOVER, power=2), case.spinor(rhs), X)
+    assert info.converged and info.reliable_updates >= 1
+    x = X.download()
+    assert true_residual(case, x, b) <= 1e-10
+    xo, oinfo, ok = case.orc.solve("cg", rhs, eps, kind=K_CLOVER, power=2, max_steps=5000)
+    err = np.linalg.norm((x - xo).ravel()) / np.linalg.norm(xo.ravel())
+    print("mixed CG prec %d: %d iterations (FP64 oracle %d), %d reliable updates, error vs oracle %.2e"
+          % (sloppy, info.steps, oinfo.steps, info.reliable_updates, err))
+    assert err < 1e-9
+
+
+@pytest.mark.parametrize("kind,sloppy", [(lym.SOLVER_PRECOND_CG, None), (lym.SOLVER_PRECOND_MIXED_CG, lym.PREC_HALF),
+                                         (lym.SOLVER_PRECOND_MIXED_CG, lym.PREC_SINGLE)])
+def test_even_odd_cg_drivers(case, kind, sloppy):
+    """The even-odd driver with FP64 CG or mixed CG on Qhat^2 returns D^-1 b."""
+    b = case.gauss()
+    eps = (1e-11 * np.linalg.norm(b)) ** 2
+    solver = lym.Solver(case.ctx, kind, precision=eps, max_steps=20000, sloppy=sloppy)
+    X = case.spinor()
+    info = solver.solve(case.op(lym.OP_CLOVER), case.spinor(b), X)
+    assert info.converged
+    x = X.download()
+    assert true_residual(case, x, b) <= 1e-10
+    xo, oinfo, ok = case.orc.solve("precond_bicgstab", b, eps, kind=K_CLOVER, max_steps=3000)
+    assert np.linalg.norm((x - xo).ravel()) / np.linalg.norm(xo.ravel()) < 1e-9
+
+
+def test_mixed_cg_max_steps(case):
+    b = case.gauss()
+    solver = lym.Solver(case.ctx, lym.SOLVER_MIXED_CG, precision=1e-300, max_steps=4)
+    X = case.spinor()
+    info = solver.solve(case.op(lym.OP_CLOVER, power=2), case.spinor(b), X)
+    assert not info.converged and info.steps == 4
+    assert np.all(np.isfinite(X.download()))
