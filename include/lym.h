@@ -161,6 +161,21 @@ int lym_solver_set_reliable_delta(lym_solver* s, double delta);
 /* solve(dirac, source, solution, initial_guess) -- x0 may be NULL */
 int lym_solver_solve(lym_solver* s, lym_op* op, const lym_spinor* source, lym_spinor* solution,
                      const lym_spinor* initial_guess, lym_solve_info* info);
+/* Many right-hand sides against one operator (no counterpart in the reference,
+ * which solves them one call at a time, e.g. MesonCorrelator.cpp loops over 12
+ * point sources).  Mixed-precision CG kinds only (LYM_SOLVER_MIXED_CG,
+ * LYM_SOLVER_PRECOND_MIXED_CG).  RHS are solved in batches that share every
+ * operator apply; each keeps its own recursion, tolerance (precisions[q], or
+ * the solver's precision when precisions is NULL) and stopping point, with
+ * results bit-identical to the same RHS solved in a batch of any other size.
+ * Starting guess x0 = source.  info[q].seconds is the wall time of the call. */
+int lym_solver_solve_batch(lym_solver* s, lym_op* op, unsigned nrhs, const lym_spinor* const* sources,
+                           lym_spinor* const* solutions, const double* precisions, lym_solve_info* infos);
+/* RHS per batch: 0 = autotune (timed per lattice shape, link storage and
+ * precision on first use), else one of 1, 2, 3, 4, 6, 8, 12, 16 */
+int lym_solver_set_rhs_block(lym_solver* s, unsigned nrhs_per_batch);
+/* the batch size and workgroup size in use (threads 0: default) */
+int lym_solver_get_rhs_block(lym_solver* s, unsigned* nrhs_per_batch, unsigned* threads);
 
 /* ------------------------------------------------------------------ host-side input preparation
  * Seeded synthetic inputs (the reference seeds from time(NULL),

@@ -76,6 +76,10 @@ SIGNATURES = {
     "lym_solver_set_sloppy_precision": (_I, [_P, _I]),
     "lym_solver_set_reliable_delta": (_I, [_P, _D]),
     "lym_solver_solve": (_I, [_P, _P, _P, _P, _P, ctypes.POINTER(SolveInfo)]),
+    "lym_solver_solve_batch": (_I, [_P, _P, _U, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_D),
+                                    ctypes.POINTER(SolveInfo)]),
+    "lym_solver_set_rhs_block": (_I, [_P, _U]),
+    "lym_solver_get_rhs_block": (_I, [_P, ctypes.POINTER(_U), ctypes.POINTER(_U)]),
     "lym_gen_su3_links": (_I, [_P, _D, _U64, _P]),
     "lym_gen_su2_links": (_I, [_P, _D, _U64, _P]),
     "lym_su2_to_adjoint": (_I, [_L, _P, _P]),
@@ -351,6 +355,28 @@ class Solver:
         _check(lib().lym_solver_solve(self.h, op.h, source.h, solution.h, guess.h if guess is not None else None,
                                       ctypes.byref(info)))
         return info
+
+    def solve_batch(self, op, sources, solutions, precisions=None):
+        """Solve op x_q = b_q for every q, batched; returns one SolveInfo per RHS."""
+        n = len(sources)
+        if len(solutions) != n:
+            raise ValueError("sources and solutions differ in length")
+        srcs = (_P * max(n, 1))(*[s.h for s in sources])
+        sols = (_P * max(n, 1))(*[s.h for s in solutions])
+        infos = (SolveInfo * max(n, 1))()
+        prec = None
+        if precisions is not None:
+            prec = (_D * max(n, 1))(*[float(p) for p in precisions])
+        _check(lib().lym_solver_solve_batch(self.h, op.h, n, srcs, sols, prec, infos))
+        return [infos[i] for i in range(n)]
+
+    def set_rhs_block(self, n):
+        _check(lib().lym_solver_set_rhs_block(self.h, n))
+
+    def rhs_block(self):
+        a, b = _U(), _U()
+        _check(lib().lym_solver_get_rhs_block(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def close(self):
         if self.h:

@@ -283,26 +283,42 @@ int link_kind_of(const lym_gauge* g) { return link_kind(g); }
 
 int op_sloppy(lym_op* op, int prec) { return op_sloppy_data(op, prec); }
 
-static size_t selem(int prec) { return prec == 1 ? 8 : 4; }
+static size_t selem(int prec) { return prec == 0 ? 16 : (prec == 1 ? 8 : 4); }
 
-// full lattice, sloppy precision: out = Op in
-void apply_full_sloppy(lym_op* op, int prec, void* out, float* out_sc, const void* in, const float* in_sc, int gamma5,
-                       const int* skip) {
-    lym_ctx* c = op->ctx;
-    const size_t po = 12 * (size_t)c->g.stride * selem(prec);
-    lym::DslashParams a = base_params(op);
+// operator data of a storage precision: 0 FP64 (the operator's own), 1/2 the sloppy copies
+static void set_op_data(lym::DslashParams& a, const lym_op* op, int prec) {
     a.prec = prec;
-    a.links = op->Us;
-    a.A = op->As;
-    a.A_sc = op->As_sc;
+    if (prec == 0) {
+        a.links = op->gauge->U;
+        a.A = op->A;
+        a.Ainv = op->Ainv;
+    } else {
+        a.links = op->Us;
+        a.A = op->As;
+        a.A_sc = op->As_sc;
+        a.Ainv = op->Ainvs;
+        a.Ainv_sc = op->Ainvs_sc;
+    }
+}
+
+// full lattice, nr right-hand sides, storage precision prec: out = Op in
+void apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* out_sc, const void* in,
+                   const float* in_sc, int gamma5, const int* skip) {
+    lym_ctx* c = op->ctx;
+    const size_t po = 12 * (size_t)c->g.stride * nr * selem(prec);
+    const size_t so = (size_t)c->g.stride * nr;
+    lym::DslashParams a = base_params(op);
+    set_op_data(a, op, prec);
+    a.nr = nr;
+    a.threads = threads;
     for (int p = 0; p < 2; ++p) {
         a.in[p] = (const char*)in + p * po;
         a.diag_in[p] = a.in[p];
         a.out[p] = (char*)out + p * po;
         if (prec == 2) {
-            a.in_sc[p] = in_sc + (size_t)p * c->g.stride;
+            a.in_sc[p] = in_sc + p * so;
             a.diag_sc[p] = a.in_sc[p];
-            a.out_sc[p] = out_sc + (size_t)p * c->g.stride;
+            a.out_sc[p] = out_sc + p * so;
         }
     }
     a.gamma5 = gamma5;
@@ -311,18 +327,15 @@ void apply_full_sloppy(lym_op* op, int prec, void* out, float* out_sc, const voi
     lym::launch_dslash(a, link_kind(op->gauge), op_has_clover(op) ? 2 : 1, false, c->stream);
 }
 
-// Schur complement, sloppy precision, odd half fields
-void apply_schur_sloppy(lym_op* op, int prec, void* out_o, float* out_sc, const void* in_o, const float* in_sc,
-                        void* tmp_e, float* tmp_sc, int gamma5, const int* skip) {
+// Schur complement on odd-half nr-fields, storage precision prec, explicit even intermediate
+void apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, float* out_sc, const void* in_o,
+                    const float* in_sc, void* tmp_e, float* tmp_sc, int gamma5, const int* skip) {
     lym_ctx* c = op->ctx;
     int lk = link_kind(op->gauge);
     lym::DslashParams a = base_params(op);
-    a.prec = prec;
-    a.links = op->Us;
-    a.A = op->As;
-    a.A_sc = op->As_sc;
-    a.Ainv = op->Ainvs;
-    a.Ainv_sc = op->Ainvs_sc;
+    set_op_data(a, op, prec);
+    a.nr = nr;
+    a.threads = threads;
     a.skip = skip;
     lym::DslashParams b = a;
     a.parity = 0;
@@ -341,6 +354,43 @@ void apply_schur_sloppy(lym_op* op, int prec, void* out_o, float* out_sc, const 
     b.hop_sign = -1.0;
     b.gamma5 = gamma5;
     lym::launch_dslash(b, lk, op->csw != 0.0 ? 2 : 1, false, c->stream);
+}
+
+void apply_full_sloppy(lym_op* op, int prec, void* out, float* out_sc, const void* in, const float* in_sc, int gamma5,
+                       const int* skip) {
+    apply_full_nr(op, prec, 1, 0, out, out_sc, in, in_sc, gamma5, skip);
+}
+
+void apply_schur_sloppy(lym_op* op, int prec, void* out_o, float* out_sc, const void* in_o, const float* in_sc,
+                        void* tmp_e, float* tmp_sc, int gamma5, const int* skip) {
+    apply_schur_nr(op, prec, 1, 0, out_o, out_sc, in_o, in_sc, tmp_e, tmp_sc, gamma5, skip);
+}
+
+// the EO driver's source and reconstruction steps on nr-fields (FP64)
+// src_o = b_o - D_oe Ainv b_e  (te: even scratch)
+void eo_source_nr(lym_op* op, int nr, const zd* b_e, const zd* b_o, zd* te, zd* src_o) {
+    lym_ctx* c = op->ctx;
+    lym::launch_clover_apply_nr(c->g, reinterpret_cast<const zd*>(op->Ainv), b_e, te, nr, c->stream);
+    lym::DslashParams a = base_params(op);
+    a.nr = nr;
+    a.hop_sign = -1.0;
+    a.parity = 1;
+    a.in[0] = te;
+    a.diag_in[1] = b_o;
+    a.out[1] = src_o;
+    lym::launch_dslash(a, link_kind(op->gauge), 1, false, c->stream);
+}
+// x_e = Ainv (b_e - D_eo x_o)
+void eo_reconstruct_nr(lym_op* op, int nr, const zd* b_e, const zd* x_o, zd* x_e) {
+    lym_ctx* c = op->ctx;
+    lym::DslashParams a = base_params(op);
+    a.nr = nr;
+    a.hop_sign = -1.0;
+    a.parity = 0;
+    a.in[1] = x_o;
+    a.diag_in[0] = b_e;
+    a.out[0] = x_e;
+    lym::launch_dslash(a, link_kind(op->gauge), 1, true, c->stream);
 }
 }  // namespace lym_internal
 

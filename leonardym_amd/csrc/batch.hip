@@ -1,0 +1,482 @@
+// batch.hip -- multi-RHS mixed-precision CG with per-RHS reliable updates.
+//
+// NR right-hand sides share every operator apply (rhs-fastest fields, so one
+// link load serves all NR lanes of a site) while each keeps its own CG
+// recursion: alpha, beta, |r|^2, its reliable-update decisions, its tolerance
+// and its stopping point.  A converged RHS is frozen.  Every per-RHS sum runs
+// over the same site partition and in the same order as the single-RHS kernels
+// of mixed.hip (thread per site, 512 x 256 grid-stride, fixed trees), and the
+// per-site arithmetic is the same, so an RHS gives bit-identical results alone
+// and in any batch.
+#include "lym_device.h"
+
+namespace lym {
+
+constexpr int BR_THREADS = 256;
+
+template <int NV>
+__device__ __forceinline__ void b_block_store(double v[NV], double* partials) {
+    __shared__ double sh[NV][BR_THREADS / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] += __shfl_xor(v[k], off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) sh[k][w] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double s = 0;
+#pragma unroll
+            for (int j = 0; j < BR_THREADS / 64; ++j) s += sh[k][j];
+            partials[k * RED_BLOCKS + blockIdx.x] = s;
+        }
+}
+
+template <int NV>
+__device__ __forceinline__ void b_final_sum(const double* partials, double out[NV]) {
+    __shared__ double sh[NV][RED_BLOCKS / 64];
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = partials[k * RED_BLOCKS + threadIdx.x];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] += __shfl_xor(v[k], off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) sh[k][w] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double s = 0;
+#pragma unroll
+            for (int j = 0; j < RED_BLOCKS / 64; ++j) s += sh[k][j];
+            out[k] = s;
+        }
+}
+
+#define BSITE_LOOP(j, n) for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < (n); j += (long)gridDim.x * blockDim.x)
+
+// element (parity p, component k, site i, rhs r) of an nr-field: ((p*12 + k)*stride + i)*nr + r
+__device__ __forceinline__ long bidx(int stride, int nr, long j, int k, int r) {
+    const long p = j / stride, i = j % stride;
+    return ((p * 12 + k) * stride + i) * nr + r;
+}
+
+template <int PR, int NR>
+__device__ __forceinline__ void bs_load(const typename Prec<PR>::S* d, const float* sc, int stride, long j, int r,
+                                        cplx<float> s[12]) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+        cplx<typename Prec<PR>::T> z = to_c<PR>(d[bidx(stride, NR, j, k, r)]);
+        s[k] = mk((float)z.re, (float)z.im);
+    }
+    if constexpr (PR == 2) {
+        const float f = sc[j * NR + r];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) s[k] = f * s[k];
+    }
+}
+
+template <int PR, int NR>
+__device__ __forceinline__ void bs_store(typename Prec<PR>::S* d, float* sc, int stride, long j, int r,
+                                         const cplx<float> s[12]) {
+    if constexpr (PR == 2) {
+        float m = 0.f;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) m = fmaxf(m, fmaxf(fabsf(s[k].re), fabsf(s[k].im)));
+        const float inv = m > 0.f ? 1.0f / m : 0.f;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) d[bidx(stride, NR, j, k, r)] = from_c<PR>(inv * s[k]);
+        sc[j * NR + r] = m;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) d[bidx(stride, NR, j, k, r)] = from_c<PR>(s[k]);
+    }
+}
+
+template <class E, int NR>
+__device__ __forceinline__ void bg_load(const E* d, int stride, long j, int r, E s[12]) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) s[k] = d[bidx(stride, NR, j, k, r)];
+}
+template <class E, int NR>
+__device__ __forceinline__ void bg_store(E* d, int stride, long j, int r, const E s[12]) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) d[bidx(stride, NR, j, k, r)] = s[k];
+}
+
+// ---------------------------------------------------------------- kernels
+template <int PR, int NR>
+__global__ __launch_bounds__(BR_THREADS) void k_bcg_init(const zd* r, cplx<float>* rs, cplx<float>* xs,
+                                                         typename Prec<PR>::S* p, float* p_sc, long nsites, int stride,
+                                                         double* partials) {
+    double v[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) v[q] = 0;
+    BSITE_LOOP(j, nsites) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            zd a[12];
+            cplx<float> f[12], z[12];
+            bg_load<zd, NR>(r, stride, j, q, a);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) {
+                v[q] += a[k].re * a[k].re + a[k].im * a[k].im;
+                f[k] = mk((float)a[k].re, (float)a[k].im);
+                z[k] = mk(0.f, 0.f);
+            }
+            bg_store<cplx<float>, NR>(rs, stride, j, q, f);
+            bg_store<cplx<float>, NR>(xs, stride, j, q, z);
+            bs_store<PR, NR>(p, p_sc, stride, j, q, f);
+        }
+    }
+    b_block_store<NR>(v, partials);
+}
+
+template <int NR>
+__global__ __launch_bounds__(RED_BLOCKS) void k_bcg_init_final(BState* s, const double* partials, int max_steps) {
+    double out[NR];
+    b_final_sum<NR>(partials, out);
+    if (threadIdx.x == 0) {
+        int all = 1;
+        for (int q = 0; q < NR; ++q) {
+            s->rr_s[q] = s->rr_true[q] = s->maxrr[q] = out[q];
+            s->step[q] = 0;
+            s->nrel[q] = 0;
+            s->rel[q] = 0;
+            s->alpha[q] = s->beta[q] = 0;
+            // a zero source has the solution x = x0 = 0: nothing to iterate
+            s->conv[q] = out[q] == 0.0 ? 1 : 0;
+            s->done[q] = (max_steps <= 0 || out[q] == 0.0) ? 1 : 0;
+            all = all && s->done[q];
+        }
+        s->max_steps = max_steps;
+        s->all_done = all;
+        s->any_rel = 0;
+        s->skip_rel = 1;
+    }
+}
+
+template <int PR, int NR>
+__global__ __launch_bounds__(BR_THREADS) void k_bcg_dot(const BState* s, const typename Prec<PR>::S* p,
+                                                        const float* p_sc, const typename Prec<PR>::S* Ap,
+                                                        const float* Ap_sc, long nsites, int stride, double* partials) {
+    if (s->all_done) return;
+    double v[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) v[q] = 0;
+    BSITE_LOOP(j, nsites) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            cplx<float> a[12], b[12];
+            bs_load<PR, NR>(p, p_sc, stride, j, q, a);
+            bs_load<PR, NR>(Ap, Ap_sc, stride, j, q, b);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) v[q] += (double)a[k].re * b[k].re + (double)a[k].im * b[k].im;
+        }
+    }
+    b_block_store<NR>(v, partials);
+}
+
+template <int NR>
+__global__ __launch_bounds__(RED_BLOCKS) void k_bcg_alpha_final(BState* s, const double* partials) {
+    if (s->all_done) return;
+    double out[NR];
+    b_final_sum<NR>(partials, out);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < NR; ++q)
+            if (!s->done[q]) s->alpha[q] = s->rr_s[q] / out[q];
+}
+
+template <int PR, int NR>
+__global__ __launch_bounds__(BR_THREADS) void k_bcg_update(const BState* s, cplx<float>* xs, cplx<float>* rs,
+                                                           const typename Prec<PR>::S* p, const float* p_sc,
+                                                           const typename Prec<PR>::S* Ap, const float* Ap_sc,
+                                                           long nsites, int stride, double* partials) {
+    if (s->all_done) return;
+    double v[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) v[q] = 0;
+    BSITE_LOOP(j, nsites) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            if (s->done[q]) continue;
+            const float al = (float)s->alpha[q];
+            cplx<float> x[12], r[12], a[12], b[12];
+            bg_load<cplx<float>, NR>(xs, stride, j, q, x);
+            bg_load<cplx<float>, NR>(rs, stride, j, q, r);
+            bs_load<PR, NR>(p, p_sc, stride, j, q, a);
+            bs_load<PR, NR>(Ap, Ap_sc, stride, j, q, b);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) {
+                x[k] = x[k] + al * a[k];
+                r[k] = r[k] - al * b[k];
+                v[q] += (double)r[k].re * r[k].re + (double)r[k].im * r[k].im;
+            }
+            bg_store<cplx<float>, NR>(xs, stride, j, q, x);
+            bg_store<cplx<float>, NR>(rs, stride, j, q, r);
+        }
+    }
+    b_block_store<NR>(v, partials);
+}
+
+template <int NR>
+__global__ __launch_bounds__(RED_BLOCKS) void k_bcg_update_final(BState* s, const double* partials) {
+    if (s->all_done) return;
+    double out[NR];
+    b_final_sum<NR>(partials, out);
+    if (threadIdx.x == 0) {
+        int any = 0;
+        for (int q = 0; q < NR; ++q) {
+            if (s->done[q]) continue;
+            const double rn = out[q];
+            if (rn > s->maxrr[q]) s->maxrr[q] = rn;
+            if (rn < s->delta2 * s->maxrr[q] || rn < s->eps[q]) {
+                s->rel[q] = 1;
+                any = 1;
+                continue;
+            }
+            s->beta[q] = rn / s->rr_s[q];
+            s->rr_s[q] = rn;
+            s->step[q] += 1;
+            if (s->step[q] >= s->max_steps) s->done[q] = 1;
+        }
+        s->any_rel = any;
+        s->skip_rel = any ? 0 : 1;
+        int all = 1;
+        for (int q = 0; q < NR; ++q) all = all && s->done[q];
+        s->all_done = all;
+    }
+}
+
+template <int NR>
+__global__ void k_brel_accum(const BState* s, zd* x, cplx<float>* xs, long nsites, int stride) {
+    if (s->skip_rel) return;
+    BSITE_LOOP(j, nsites) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            if (!s->rel[q]) continue;
+            zd a[12];
+            cplx<float> f[12];
+            bg_load<zd, NR>(x, stride, j, q, a);
+            bg_load<cplx<float>, NR>(xs, stride, j, q, f);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) {
+                a[k] = a[k] + mk((double)f[k].re, (double)f[k].im);
+                f[k] = mk(0.f, 0.f);
+            }
+            bg_store<zd, NR>(x, stride, j, q, a);
+            bg_store<cplx<float>, NR>(xs, stride, j, q, f);
+        }
+    }
+}
+
+template <int NR>
+__global__ __launch_bounds__(BR_THREADS) void k_brel_resid(const BState* s, zd* r, cplx<float>* rs, const zd* b,
+                                                           const zd* Ax, long nsites, int stride, double* partials) {
+    if (s->skip_rel) return;
+    double v[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) v[q] = 0;
+    BSITE_LOOP(j, nsites) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            if (!s->rel[q]) continue;
+            zd bb[12], ax[12];
+            cplx<float> f[12];
+            bg_load<zd, NR>(b, stride, j, q, bb);
+            bg_load<zd, NR>(Ax, stride, j, q, ax);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) {
+                bb[k] = bb[k] - ax[k];
+                v[q] += bb[k].re * bb[k].re + bb[k].im * bb[k].im;
+                f[k] = mk((float)bb[k].re, (float)bb[k].im);
+            }
+            bg_store<zd, NR>(r, stride, j, q, bb);
+            bg_store<cplx<float>, NR>(rs, stride, j, q, f);
+        }
+    }
+    b_block_store<NR>(v, partials);
+}
+
+template <int NR>
+__global__ __launch_bounds__(RED_BLOCKS) void k_brel_final(BState* s, const double* partials) {
+    if (s->skip_rel) return;
+    double out[NR];
+    b_final_sum<NR>(partials, out);
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < NR; ++q) {
+            if (!s->rel[q]) continue;
+            const double rt = out[q];
+            s->rr_true[q] = rt;
+            s->nrel[q] += 1;
+            s->rel[q] = 0;
+            s->step[q] += 1;
+            if (rt < s->eps[q]) {
+                s->done[q] = 1;
+                s->conv[q] = 1;
+                continue;
+            }
+            s->beta[q] = rt / s->rr_s[q];
+            s->rr_s[q] = rt;
+            s->maxrr[q] = rt;
+            if (s->step[q] >= s->max_steps) s->done[q] = 1;
+        }
+        s->any_rel = 0;
+        s->skip_rel = 1;
+        int all = 1;
+        for (int q = 0; q < NR; ++q) all = all && s->done[q];
+        s->all_done = all;
+    }
+}
+
+template <int PR, int NR>
+__global__ void k_bcg_p(const BState* s, typename Prec<PR>::S* p, float* p_sc, const cplx<float>* rs, long nsites,
+                        int stride) {
+    if (s->all_done) return;
+    BSITE_LOOP(j, nsites) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            if (s->done[q]) continue;
+            const float be = (float)s->beta[q];
+            cplx<float> a[12], r[12];
+            bs_load<PR, NR>(p, p_sc, stride, j, q, a);
+            bg_load<cplx<float>, NR>(rs, stride, j, q, r);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) a[k] = r[k] + be * a[k];
+            bs_store<PR, NR>(p, p_sc, stride, j, q, a);
+        }
+    }
+}
+
+// x += x_s for the RHS that stopped between reliable updates
+template <int NR>
+__global__ void k_baccum(const BState* s, zd* x, const cplx<float>* xs, long nsites, int stride) {
+    BSITE_LOOP(j, nsites) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            if (s->conv[q]) continue;
+            zd a[12];
+            cplx<float> f[12];
+            bg_load<zd, NR>(x, stride, j, q, a);
+            bg_load<cplx<float>, NR>(xs, stride, j, q, f);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) a[k] = a[k] + mk((double)f[k].re, (double)f[k].im);
+            bg_store<zd, NR>(x, stride, j, q, a);
+        }
+    }
+}
+
+// pack / unpack single full fields <-> an nr-field; also multi-RHS clover block apply
+__global__ void k_pack(const zd* const* srcs, int nr, zd* multi, long n) {
+    BSITE_LOOP(i, n) {
+        for (int r = 0; r < nr; ++r) multi[i * nr + r] = srcs[r][i];
+    }
+}
+__global__ void k_unpack(const zd* multi, int nr, zd* const* dsts, long n) {
+    BSITE_LOOP(i, n) {
+        for (int r = 0; r < nr; ++r) dsts[r][i] = multi[i * nr + r];
+    }
+}
+
+__global__ void k_clover_apply_nr(Geom g, const zd* __restrict__ blocks, const zd* __restrict__ in,
+                                  zd* __restrict__ out, int nr) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)g.Vh * nr) return;
+    const int idx = (int)(t / nr), r = (int)(t % nr);
+    zd s[12], o[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) s[k] = in[((long)k * g.stride + idx) * nr + r];
+    herm6(blocks, g.stride, idx, s, o);
+    herm6(blocks + (size_t)18 * g.stride, g.stride, idx, s + 6, o + 6);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) out[((long)k * g.stride + idx) * nr + r] = o[k];
+}
+
+// ---------------------------------------------------------------- launchers
+static inline int bsite_blocks(long n) {
+    long b = (n + 255) / 256;
+    return (int)(b < 4096 ? b : 4096);
+}
+
+template <int PR, int NR>
+static void bcg_step_impl(int which, BState* s, const BcgArgs& a, hipStream_t st) {
+    typedef typename Prec<PR>::S S;
+    S* p = reinterpret_cast<S*>(a.p);
+    S* Ap = reinterpret_cast<S*>(a.Ap);
+    switch (which) {
+        case 0:
+            hipLaunchKernelGGL((k_bcg_init<PR, NR>), dim3(RED_BLOCKS), dim3(BR_THREADS), 0, st, a.r, a.rs, a.xs, p,
+                               a.p_sc, a.nsites, a.stride, a.partials);
+            hipLaunchKernelGGL((k_bcg_init_final<NR>), dim3(1), dim3(RED_BLOCKS), 0, st, s, a.partials, a.max_steps);
+            break;
+        case 1:
+            hipLaunchKernelGGL((k_bcg_dot<PR, NR>), dim3(RED_BLOCKS), dim3(BR_THREADS), 0, st, s, p, a.p_sc, Ap,
+                               a.Ap_sc, a.nsites, a.stride, a.partials);
+            hipLaunchKernelGGL((k_bcg_alpha_final<NR>), dim3(1), dim3(RED_BLOCKS), 0, st, s, a.partials);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_bcg_update<PR, NR>), dim3(RED_BLOCKS), dim3(BR_THREADS), 0, st, s, a.xs, a.rs, p,
+                               a.p_sc, Ap, a.Ap_sc, a.nsites, a.stride, a.partials);
+            hipLaunchKernelGGL((k_bcg_update_final<NR>), dim3(1), dim3(RED_BLOCKS), 0, st, s, a.partials);
+            break;
+        case 3:
+            hipLaunchKernelGGL((k_brel_accum<NR>), dim3(bsite_blocks(a.nsites)), dim3(256), 0, st, s, a.x, a.xs,
+                               a.nsites, a.stride);
+            break;
+        case 4:
+            hipLaunchKernelGGL((k_brel_resid<NR>), dim3(RED_BLOCKS), dim3(BR_THREADS), 0, st, s, a.r, a.rs, a.b, a.Ax,
+                               a.nsites, a.stride, a.partials);
+            hipLaunchKernelGGL((k_brel_final<NR>), dim3(1), dim3(RED_BLOCKS), 0, st, s, a.partials);
+            break;
+        case 5:
+            hipLaunchKernelGGL((k_bcg_p<PR, NR>), dim3(bsite_blocks(a.nsites)), dim3(256), 0, st, s, p, a.p_sc, a.rs,
+                               a.nsites, a.stride);
+            break;
+        default:
+            hipLaunchKernelGGL((k_baccum<NR>), dim3(bsite_blocks(a.nsites)), dim3(256), 0, st, s, a.x, a.xs,
+                               a.nsites, a.stride);
+            break;
+    }
+}
+
+template <int PR>
+static bool bcg_step_pr(int which, int nr, BState* s, const BcgArgs& a, hipStream_t st) {
+    switch (nr) {
+        case 1: bcg_step_impl<PR, 1>(which, s, a, st); return true;
+        case 2: bcg_step_impl<PR, 2>(which, s, a, st); return true;
+        case 3: bcg_step_impl<PR, 3>(which, s, a, st); return true;
+        case 4: bcg_step_impl<PR, 4>(which, s, a, st); return true;
+        case 6: bcg_step_impl<PR, 6>(which, s, a, st); return true;
+        case 8: bcg_step_impl<PR, 8>(which, s, a, st); return true;
+        case 12: bcg_step_impl<PR, 12>(which, s, a, st); return true;
+        case 16: bcg_step_impl<PR, 16>(which, s, a, st); return true;
+        default: return false;
+    }
+}
+
+bool launch_bcg_step(int which, int prec, int nr, BState* s, const BcgArgs& a, hipStream_t st) {
+    return prec == 1 ? bcg_step_pr<1>(which, nr, s, a, st) : bcg_step_pr<2>(which, nr, s, a, st);
+}
+
+bool batch_nr_supported(int nr) { return nr == 1 || nr == 2 || nr == 3 || nr == 4 || nr == 6 || nr == 8 || nr == 12 || nr == 16; }
+
+void launch_pack(const zd* const* srcs_dev, int nr, zd* multi, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_pack, dim3(bsite_blocks(n)), dim3(256), 0, st, srcs_dev, nr, multi, n);
+}
+void launch_unpack(const zd* multi, int nr, zd* const* dsts_dev, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_unpack, dim3(bsite_blocks(n)), dim3(256), 0, st, multi, nr, dsts_dev, n);
+}
+void launch_clover_apply_nr(const Geom& g, const zd* blocks, const zd* in, zd* out, int nr, hipStream_t st) {
+    long n = (long)g.Vh * nr;
+    hipLaunchKernelGGL(k_clover_apply_nr, dim3((n + 255) / 256), dim3(256), 0, st, g, blocks, in, out, nr);
+}
+
+}  // namespace lym

@@ -213,7 +213,7 @@ __global__ __launch_bounds__(LYM_DSLASH_THREADS) LYM_OCC void k_dslash(DslashPar
 
 template <int PR, int LK, int DIAG, bool INV>
 static void launch_one(const DslashParams& a, int nblocks, hipStream_t st) {
-    hipLaunchKernelGGL((k_dslash<PR, LK, DIAG, INV>), dim3(nblocks), dim3(LYM_DSLASH_THREADS), 0, st, a);
+    hipLaunchKernelGGL((k_dslash<PR, LK, DIAG, INV>), dim3(nblocks), dim3(a.spb * a.nr), 0, st, a);
 }
 
 template <int PR, int LK>
@@ -241,7 +241,8 @@ static void launch_pr(const DslashParams& a, int link_kind, int diag, bool inv, 
 
 void launch_dslash(DslashParams a, int link_kind, int diag, bool inv, hipStream_t st) {
     if (a.nr < 1) a.nr = 1;
-    a.spb = LYM_DSLASH_THREADS / a.nr;
+    const int threads = (a.threads > 0 && a.threads <= LYM_DSLASH_THREADS) ? a.threads : LYM_DSLASH_THREADS;
+    a.spb = threads / a.nr;
     if (a.spb < 1) a.spb = 1;
     a.nblocks_per_parity = (a.g.Vh + a.spb - 1) / a.spb;
     int nblocks = a.nblocks_per_parity * (a.parity >= 0 ? 1 : 2);

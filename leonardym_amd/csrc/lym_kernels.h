@@ -35,6 +35,7 @@ struct DslashParams {
     int parity;            // 0 or 1: that parity only; -1: both
     int nblocks_per_parity;
     const int* skip;       // optional device flag: nonzero -> the launch does nothing (solver done)
+    int threads;           // workgroup size (0: LYM_DSLASH_THREADS); a workgroup holds threads / nr sites
 };
 
 // link_kind: 0 complex 18, 1 complex 12, 2 real 9, 3 real 6; diag: 0 none, 1 identity, 2 clover
@@ -124,6 +125,37 @@ void launch_rel_resid(MCGState* s, zd* r, cplx<float>* rs, const zd* b, const zd
                       double* partials, hipStream_t st);
 void launch_mcg_p(int prec, MCGState* s, void* p, float* p_sc, const cplx<float>* rs, long nsites, int stride,
                   hipStream_t st);
+
+// ---------------------------------------------------------------- multi-RHS mixed CG (batch.hip)
+constexpr int BMAX = 16;
+struct BState {
+    int all_done, any_rel, skip_rel, max_steps;
+    int done[BMAX], conv[BMAX], step[BMAX], rel[BMAX], nrel[BMAX];
+    double delta2;
+    double eps[BMAX], rr_s[BMAX], rr_true[BMAX], maxrr[BMAX], alpha[BMAX], beta[BMAX];
+};
+struct BcgArgs {
+    zd* x;
+    zd* r;
+    const zd* b;
+    const zd* Ax;
+    cplx<float>* xs;
+    cplx<float>* rs;
+    void* p;
+    float* p_sc;
+    void* Ap;
+    float* Ap_sc;
+    long nsites;
+    int stride;
+    int max_steps;
+    double* partials;  // BMAX * RED_BLOCKS doubles
+};
+// which: 0 init, 1 alpha, 2 update, 3 reliable accumulate, 4 reliable residual, 5 p update, 6 final accumulate
+bool launch_bcg_step(int which, int prec, int nr, BState* s, const BcgArgs& a, hipStream_t st);
+bool batch_nr_supported(int nr);
+void launch_pack(const zd* const* srcs_dev, int nr, zd* multi, long n, hipStream_t st);
+void launch_unpack(const zd* multi, int nr, zd* const* dsts_dev, long n, hipStream_t st);
+void launch_clover_apply_nr(const Geom& g, const zd* blocks, const zd* in, zd* out, int nr, hipStream_t st);
 
 // x = uniform [0,1) complex on real sites, zero on padding (BiCGStab start for a zero source)
 void launch_fill_uniform(zd* x, int nparity, int stride, int Vh, uint64_t seed, hipStream_t st);

@@ -17,6 +17,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
+#include <map>
 #include <string>
 
 #include "../../include/lym.h"
@@ -36,6 +38,12 @@ void apply_full_sloppy(lym_op* op, int prec, void* out, float* out_sc, const voi
                        const int* skip);
 void apply_schur_sloppy(lym_op* op, int prec, void* out_o, float* out_sc, const void* in_o, const float* in_sc,
                         void* tmp_e, float* tmp_sc, int gamma5, const int* skip);
+void apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* out_sc, const void* in,
+                   const float* in_sc, int gamma5, const int* skip);
+void apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, float* out_sc, const void* in_o,
+                    const float* in_sc, void* tmp_e, float* tmp_sc, int gamma5, const int* skip);
+void eo_source_nr(lym_op* op, int nr, const zd* b_e, const zd* b_o, zd* te, zd* src_o);
+void eo_reconstruct_nr(lym_op* op, int nr, const zd* b_e, const zd* x_o, zd* x_e);
 }  // namespace lym_internal
 
 using lym_internal::fail;
@@ -57,6 +65,21 @@ struct lym_solver {
     float* svsc[4];         // their per-site scales [2][stride]
     lym::cplx<float>* xs;   // FP32 x_s, r_s [2][12][stride]
     lym::cplx<float>* rs;
+    // multi-RHS batches
+    int rhs_block;          // RHS per batch (0: autotuned)
+    int tuned_nr, tuned_threads;
+    int bnr, bdom, bprec;   // nr, domain parities and sloppy precision the batch buffers hold
+    zd* bw[10];             // FP64 nr-fields
+    void* bsv[4];           // sloppy nr-fields p, Ap, Q^2 intermediate, Schur intermediate
+    float* bsvsc[4];
+    lym::cplx<float>* bxs;
+    lym::cplx<float>* brs;
+    double* bpartials;
+    void* bstate;
+    void* bhost;
+    zd** bptr_dev;
+    zd** bptr_host;
+    zd* bzero;              // zero source / dummy destination for padded lanes
 };
 
 namespace {
@@ -328,9 +351,322 @@ int run_mixed_cg(lym_solver* s, const LinOp& L, const SloppyOp& SL, const zd* b,
     return LYM_OK;
 }
 
+void free_batch(lym_solver* s) {
+    for (int i = 0; i < 10; ++i) {
+        if (s->bw[i]) (void)hipFree(s->bw[i]);
+        s->bw[i] = nullptr;
+    }
+    for (int i = 0; i < 4; ++i) {
+        if (s->bsv[i]) (void)hipFree(s->bsv[i]);
+        if (s->bsvsc[i]) (void)hipFree(s->bsvsc[i]);
+        s->bsv[i] = nullptr;
+        s->bsvsc[i] = nullptr;
+    }
+    if (s->bxs) (void)hipFree(s->bxs);
+    if (s->brs) (void)hipFree(s->brs);
+    s->bxs = s->brs = nullptr;
+    s->bnr = 0;
+    s->bdom = 0;
+}
+
+int dzalloc(lym_ctx* c, void** p, size_t bytes) {
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        *p = nullptr;
+        return fail(LYM_ERR_NOMEM, "batch buffer of %zu bytes", bytes);
+    }
+    if (hipMemsetAsync(*p, 0, bytes, c->stream) != hipSuccess) return fail(LYM_ERR_DEVICE, "hipMemsetAsync");
+    return LYM_OK;
+}
+
+// buffers for batches of nr RHS over `dom` parities (1: even-odd, 2: full)
+int ensure_batch(lym_solver* s, int nr, int dom, int prec) {
+    lym_ctx* c = s->ctx;
+    int rc;
+    if (!s->bpartials) {
+        if ((rc = dzalloc(c, (void**)&s->bpartials, sizeof(double) * lym::BMAX * lym::RED_BLOCKS))) return rc;
+        if ((rc = dzalloc(c, &s->bstate, sizeof(lym::BState)))) return rc;
+        if (hipHostMalloc(&s->bhost, sizeof(lym::BState), hipHostMallocDefault) != hipSuccess)
+            return fail(LYM_ERR_DEVICE, "hipHostMalloc");
+        if ((rc = dzalloc(c, (void**)&s->bptr_dev, sizeof(zd*) * lym::BMAX))) return rc;
+        if (hipHostMalloc((void**)&s->bptr_host, sizeof(zd*) * lym::BMAX, hipHostMallocDefault) != hipSuccess)
+            return fail(LYM_ERR_DEVICE, "hipHostMalloc");
+        if ((rc = dzalloc(c, (void**)&s->bzero, sizeof(zd) * full_n(c)))) return rc;
+    }
+    if (s->bnr >= nr && s->bdom >= dom && s->bprec == prec) return LYM_OK;
+    free_batch(s);
+    const size_t S = c->g.stride;
+    const size_t fullb = 24 * S * nr * sizeof(zd), domb = 12 * S * dom * nr * sizeof(zd);
+    for (int i = 0; i < 10; ++i)
+        if ((rc = dzalloc(c, (void**)&s->bw[i], i < 2 ? fullb : domb))) return rc;
+    const size_t es = prec == 1 ? 8 : 4;
+    for (int i = 0; i < 4; ++i) {
+        if ((rc = dzalloc(c, &s->bsv[i], 12 * S * dom * nr * es))) return rc;
+        if ((rc = dzalloc(c, (void**)&s->bsvsc[i], S * dom * nr * sizeof(float)))) return rc;
+    }
+    if ((rc = dzalloc(c, (void**)&s->bxs, 12 * S * dom * nr * 8))) return rc;
+    if ((rc = dzalloc(c, (void**)&s->brs, 12 * S * dom * nr * 8))) return rc;
+    s->bnr = nr;
+    s->bdom = dom;
+    s->bprec = prec;
+    return LYM_OK;
+}
+
+// Q^power on nr-fields at storage precision prec (FP64 uses the batch's FP64 scratch)
+struct BatchOp {
+    lym_op* op;
+    bool half;
+    int gamma5;
+    unsigned power;
+    int nr, threads;
+    void* mid;
+    float* mid_sc;
+    void* tmp;
+    float* tmp_sc;
+    void apply(int prec, void* out, float* out_sc, const void* in, const float* in_sc, const int* skip) const {
+        auto one = [&](void* o, float* osc, const void* i, const float* isc) {
+            if (half) lym_internal::apply_schur_nr(op, prec, nr, threads, o, osc, i, isc, tmp, tmp_sc, gamma5, skip);
+            else lym_internal::apply_full_nr(op, prec, nr, threads, o, osc, i, isc, gamma5, skip);
+        };
+        if (power == 1) {
+            one(out, out_sc, in, in_sc);
+        } else {
+            one(mid, mid_sc, in, in_sc);
+            one(out, out_sc, mid, mid_sc);
+        }
+    }
+};
+
+// Autotuner: RHS per batch and workgroup size, by timing the sloppy Q apply
+// per right-hand side; cached per lattice shape, link kind and precision.
+struct TuneKey {
+    int dims[4], lk, prec, half;
+    bool operator<(const TuneKey& o) const { return std::memcmp(this, &o, sizeof(*this)) < 0; }
+};
+std::map<TuneKey, std::pair<int, int>>& tune_cache() {
+    static std::map<TuneKey, std::pair<int, int>> m;
+    return m;
+}
+
+int autotune_batch(lym_solver* s, lym_op* op, bool half, int prec, unsigned nrhs, int* nr_out, int* thr_out) {
+    lym_ctx* c = s->ctx;
+    TuneKey key;
+    std::memset(&key, 0, sizeof(key));
+    for (int i = 0; i < 4; ++i) key.dims[i] = c->dims[i];
+    key.lk = lym_internal::link_kind_of(op->gauge);
+    key.prec = prec;
+    key.half = half;
+    auto it = tune_cache().find(key);
+    const int cands[5] = {4, 6, 8, 12, 16};
+    int maxnr = 4;
+    for (int k : cands)
+        if ((unsigned)k <= std::max(nrhs, 4u)) maxnr = k;
+    if (it != tune_cache().end() && it->second.first <= maxnr) {
+        *nr_out = it->second.first;
+        *thr_out = it->second.second;
+        return LYM_OK;
+    }
+    int rc = ensure_batch(s, maxnr, half ? 1 : 2, prec);
+    if (rc) return rc;
+    double best = 1e300;
+    int bnr = 4, bthr = 256;
+    for (int nr : cands) {
+        if (nr > maxnr) break;
+        for (int thr : {64, 128, 256}) {
+            if (thr < nr) continue;
+            BatchOp B{op, half, 1, 1, nr, thr, s->bsv[2], s->bsvsc[2], s->bsv[3], s->bsvsc[3]};
+            B.apply(prec, s->bsv[1], s->bsvsc[1], s->bsv[0], s->bsvsc[0], nullptr);  // warm
+            HIPCHECK(hipEventRecord(c->ev_start, c->stream));
+            for (int k = 0; k < 3; ++k) B.apply(prec, s->bsv[1], s->bsvsc[1], s->bsv[0], s->bsvsc[0], nullptr);
+            HIPCHECK(hipEventRecord(c->ev_stop, c->stream));
+            HIPCHECK(hipEventSynchronize(c->ev_stop));
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, c->ev_start, c->ev_stop);
+            double per = ms / (3.0 * nr);
+            if (per < best * 0.98) {
+                best = per;
+                bnr = nr;
+                bthr = thr;
+            }
+        }
+    }
+    tune_cache()[key] = {bnr, bthr};
+    *nr_out = bnr;
+    *thr_out = bthr;
+    return LYM_OK;
+}
+
+int next_supported(int n) {
+    for (int k : {1, 2, 3, 4, 6, 8, 12, 16})
+        if (k >= n) return k;
+    return 16;
+}
+
+// one batch of nb <= nr RHS (lanes nb..nr-1 padded with zero sources)
+int run_batch(lym_solver* s, lym_op* op, bool eo, bool half, int nr, int thr, int nb,
+              const lym_spinor* const* sources, lym_spinor* const* solutions, const double* precs,
+              lym_solve_info* infos) {
+    lym_ctx* c = s->ctx;
+    hipStream_t st = c->stream;
+    const int prec = s->sloppy_prec;
+    const long S = c->g.stride;
+    const int dom = (eo || half) ? 1 : 2;
+    const long nfull = 24 * S * nr, ndom = 12 * S * dom * nr;
+    zd *B = s->bw[0], *X = s->bw[1], *x = s->bw[2], *r = s->bw[3], *Ax = s->bw[4], *mid = s->bw[5],
+       *tmp = s->bw[6], *rhs = s->bw[7], *src = s->bw[8], *te = s->bw[9];
+    // pack sources
+    for (int q = 0; q < nr; ++q) s->bptr_host[q] = q < nb ? reinterpret_cast<zd*>(sources[q]->d) : s->bzero;
+    HIPCHECK(hipMemcpyAsync(s->bptr_dev, s->bptr_host, sizeof(zd*) * nr, hipMemcpyHostToDevice, st));
+    lym::launch_pack(s->bptr_dev, nr, B, full_n(c), st);
+    const zd* b;
+    if (eo) {
+        const zd* b_e = B;
+        const zd* b_o = B + 12 * S * nr;
+        lym_internal::eo_source_nr(op, nr, b_e, b_o, te, src);
+        lym::launch_scale_g5(src, 1, (int)(S * nr), st);
+        lym_internal::apply_schur_nr(op, 0, nr, thr, rhs, nullptr, src, nullptr, tmp, nullptr, 1, nullptr);
+        b = rhs;
+    } else {
+        b = half ? B + 12 * S * nr : B;
+    }
+    // the EO driver solves Qhat^2 with gamma5 = 1; otherwise the operator as configured
+    const int g5 = eo ? 1 : op->gamma5;
+    const unsigned pw = eo ? 2u : op->power;
+    BatchOp H64{op, dom == 1, g5, pw, nr, thr, mid, nullptr, tmp, nullptr};
+    BatchOp HS{op, dom == 1, g5, pw, nr, thr, s->bsv[2], s->bsvsc[2], s->bsv[3], s->bsvsc[3]};
+    // FP64 start: x = b, r = b - A x
+    lym::launch_copy(x, b, ndom, st);
+    H64.apply(0, Ax, nullptr, x, nullptr, nullptr);
+    lym::launch_sub(r, b, Ax, ndom, st);
+    auto* Sdev = reinterpret_cast<lym::BState*>(s->bstate);
+    auto* Sh = reinterpret_cast<lym::BState*>(s->bhost);
+    std::memset(Sh, 0, sizeof(lym::BState));
+    Sh->delta2 = s->delta * s->delta;
+    for (int q = 0; q < nr; ++q) Sh->eps[q] = (q < nb && precs) ? precs[q] : s->precision;
+    HIPCHECK(hipMemcpyAsync(Sdev, Sh, sizeof(lym::BState), hipMemcpyHostToDevice, st));
+    lym::BcgArgs a;
+    a.x = x;
+    a.r = r;
+    a.b = b;
+    a.Ax = Ax;
+    a.xs = s->bxs;
+    a.rs = s->brs;
+    a.p = s->bsv[0];
+    a.p_sc = s->bsvsc[0];
+    a.Ap = s->bsv[1];
+    a.Ap_sc = s->bsvsc[1];
+    a.nsites = (long)dom * S;
+    a.stride = (int)S;
+    a.max_steps = (int)s->max_steps;
+    a.partials = s->bpartials;
+    if (!lym::launch_bcg_step(0, prec, nr, Sdev, a, st)) return fail(LYM_ERR_UNSUPPORTED, "batch of %d RHS", nr);
+    HIPCHECK(hipMemcpyAsync(Sh, Sdev, sizeof(lym::BState), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    unsigned queued = 0;
+    // steps are queued in chunks; those queued after the last RHS stopped exit at once
+    while (!Sh->all_done && queued <= s->max_steps) {
+        unsigned k = queued == 0 ? 8 : 16;
+        if (k > s->max_steps + 1 - queued) k = s->max_steps + 1 - queued;
+        for (unsigned i = 0; i < k; ++i) {
+            HS.apply(prec, a.Ap, a.Ap_sc, a.p, a.p_sc, &Sdev->all_done);
+            lym::launch_bcg_step(1, prec, nr, Sdev, a, st);
+            lym::launch_bcg_step(2, prec, nr, Sdev, a, st);
+            lym::launch_bcg_step(3, prec, nr, Sdev, a, st);
+            H64.apply(0, Ax, nullptr, x, nullptr, &Sdev->skip_rel);
+            lym::launch_bcg_step(4, prec, nr, Sdev, a, st);
+            lym::launch_bcg_step(5, prec, nr, Sdev, a, st);
+        }
+        queued += k;
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(Sh, Sdev, sizeof(lym::BState), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+    lym::launch_bcg_step(6, prec, nr, Sdev, a, st);
+    // solutions
+    if (eo) {
+        lym::launch_copy(X + 12 * S * nr, x, 12 * S * nr, st);
+        lym_internal::eo_reconstruct_nr(op, nr, B, X + 12 * S * nr, X);
+    } else if (half) {
+        lym::launch_copy(X, B, 12 * S * nr, st);
+        lym::launch_copy(X + 12 * S * nr, x, 12 * S * nr, st);
+    } else {
+        lym::launch_copy(X, x, nfull, st);
+    }
+    for (int q = 0; q < nr; ++q) s->bptr_host[q] = q < nb ? reinterpret_cast<zd*>(solutions[q]->d) : s->bzero;
+    HIPCHECK(hipMemcpyAsync(s->bptr_dev, s->bptr_host, sizeof(zd*) * nr, hipMemcpyHostToDevice, st));
+    lym::launch_unpack(X, nr, s->bptr_dev, full_n(c), st);
+    HIPCHECK(hipMemcpyAsync(Sh, Sdev, sizeof(lym::BState), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    HIPCHECK(hipMemsetAsync(s->bzero, 0, sizeof(zd) * full_n(c), st));
+    for (int q = 0; q < nb; ++q) {
+        lym_solve_info& f = infos[q];
+        f.converged = Sh->conv[q];
+        f.steps = Sh->conv[q] ? (unsigned)Sh->step[q] : s->max_steps;
+        f.last_error = Sh->rr_true[q];
+        f.reliable_updates = (unsigned)Sh->nrel[q];
+        f.applies = 1 + (unsigned)Sh->step[q] + (unsigned)Sh->nrel[q];
+    }
+    return LYM_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int lym_solver_set_rhs_block(lym_solver* s, unsigned nrhs_per_batch) {
+    if (!s) return fail(LYM_ERR_INVALID, "null solver");
+    if (nrhs_per_batch != 0 && !lym::batch_nr_supported((int)nrhs_per_batch))
+        return fail(LYM_ERR_INVALID, "RHS per batch must be 0 (autotune) or one of 1,2,3,4,6,8,12,16");
+    s->rhs_block = (int)nrhs_per_batch;
+    return LYM_OK;
+}
+
+int lym_solver_get_rhs_block(lym_solver* s, unsigned* nrhs, unsigned* threads) {
+    if (!s) return fail(LYM_ERR_INVALID, "null solver");
+    if (nrhs) *nrhs = (unsigned)(s->rhs_block ? s->rhs_block : s->tuned_nr);
+    if (threads) *threads = (unsigned)s->tuned_threads;
+    return LYM_OK;
+}
+
+int lym_solver_solve_batch(lym_solver* s, lym_op* op, unsigned nrhs, const lym_spinor* const* sources,
+                           lym_spinor* const* solutions, const double* precisions, lym_solve_info* infos) {
+    if (!s || !op || (nrhs && (!sources || !solutions || !infos))) return fail(LYM_ERR_INVALID, "null argument");
+    lym_ctx* c = s->ctx;
+    if (s->kind != LYM_SOLVER_MIXED_CG && s->kind != LYM_SOLVER_PRECOND_MIXED_CG)
+        return fail(LYM_ERR_UNSUPPORTED, "batched solves need a mixed-precision CG solver");
+    for (unsigned q = 0; q < nrhs; ++q)
+        if (!sources[q] || !solutions[q] || sources[q]->ctx != c || solutions[q]->ctx != c)
+            return fail(LYM_ERR_INVALID, "spinor %u of another context", q);
+    if (op->ctx != c) return fail(LYM_ERR_INVALID, "operator of another context");
+    const bool eo = s->kind == LYM_SOLVER_PRECOND_MIXED_CG && s->even_odd && op->power == 1 &&
+                    (op->kind == LYM_OP_CLOVER || op->kind == LYM_OP_EO_CLOVER) && c->dims[0] % 2 == 0 &&
+                    c->dims[1] % 2 == 0 && c->dims[2] % 2 == 0 && c->dims[3] % 2 == 0;
+    const bool half = op->kind == LYM_OP_EO_CLOVER;
+    int rc = lym_internal::op_ready(op, eo);
+    if (!rc) rc = lym_internal::op_sloppy(op, s->sloppy_prec);
+    if (rc) return rc;
+    int nr = s->rhs_block, thr = 0;
+    if (nr == 0) {
+        rc = autotune_batch(s, op, eo || half, s->sloppy_prec, nrhs, &nr, &thr);
+        if (rc) return rc;
+        s->tuned_nr = nr;
+        s->tuned_threads = thr;
+    }
+    HIPCHECK(hipEventRecord(c->ev_start, c->stream));
+    for (unsigned q0 = 0; q0 < nrhs; q0 += nr) {
+        int nb = (int)std::min<unsigned>(nr, nrhs - q0);
+        int nrb = next_supported(nb);
+        rc = ensure_batch(s, nrb, (eo || half) ? 1 : 2, s->sloppy_prec);
+        if (!rc) rc = run_batch(s, op, eo, half, nrb, thr, nb, sources + q0, solutions + q0,
+                                precisions ? precisions + q0 : nullptr, infos + q0);
+        if (rc) return rc;
+    }
+    HIPCHECK(hipEventRecord(c->ev_stop, c->stream));
+    HIPCHECK(hipEventSynchronize(c->ev_stop));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev_start, c->ev_stop);
+    for (unsigned q = 0; q < nrhs; ++q) infos[q].seconds = ms * 1e-3;
+    return LYM_OK;
+}
 
 int lym_solver_create(lym_ctx* c, int kind, lym_solver** out) {
     if (!c || !out) return fail(LYM_ERR_INVALID, "null argument");

@@ -1,0 +1,115 @@
+"""Batched multi-RHS solves: every RHS of a batch reaches the FP64 stopping
+rule and the oracle's solution, and gives the same bits whatever batch it is
+solved in (batches of 1, 4 or the autotuned size; ragged last batches)."""
+import numpy as np
+import pytest
+
+from helpers import K_CLOVER, K_EO, coords
+
+import leonardym_amd as lym
+from test_gpu_parity import Case, g5, true_residual
+
+pytestmark = pytest.mark.gpu
+
+BATCH_CASES = [
+    dict(dims=(4, 4, 4, 4)),
+    dict(dims=(4, 6, 2, 8)),
+    dict(dims=(4, 4, 4, 8), storage=lym.LINKS_RECONSTRUCT),
+    dict(dims=(4, 4, 4, 4), rep="adj", kappa=0.17),
+]
+
+
+@pytest.fixture(scope="module", params=range(len(BATCH_CASES)), ids=lambda i: "bcase%d" % i)
+def case(request):
+    if lym.device_count() < 1:
+        pytest.fail("no HIP device visible")
+    print("batch case", BATCH_CASES[request.param], flush=True)
+    return Case(**BATCH_CASES[request.param])
+
+
+def solve_batch(case, op, kind, srcs, block, precisions=None, sloppy=lym.PREC_HALF, eps=None, max_steps=20000):
+    solver = lym.Solver(case.ctx, kind, precision=eps, max_steps=max_steps, sloppy=sloppy)
+    solver.set_rhs_block(block)
+    B = [case.spinor(b) for b in srcs]
+    X = [case.spinor() for _ in srcs]
+    infos = solver.solve_batch(op, B, X, precisions)
+    return [x.download() for x in X], infos, solver
+
+
+@pytest.mark.parametrize("sloppy", [lym.PREC_HALF, lym.PREC_SINGLE])
+def test_even_odd_batch_matches_alone_and_oracle(case, sloppy):
+    """Seven RHS through the even-odd mixed CG driver: batches of 4 (+3 ragged),
+    of 1 and autotuned give identical bits; each solution satisfies D x = b."""
+    srcs = [case.gauss() for _ in range(7)]
+    eps = [(1e-11 * np.linalg.norm(b)) ** 2 for b in srcs]
+    op = case.op(lym.OP_CLOVER)
+    xs4, inf4, _ = solve_batch(case, op, lym.SOLVER_PRECOND_MIXED_CG, srcs, 4, eps, sloppy)
+    xs1, inf1, _ = solve_batch(case, op, lym.SOLVER_PRECOND_MIXED_CG, srcs, 1, eps, sloppy)
+    xsa, infa, sa = solve_batch(case, op, lym.SOLVER_PRECOND_MIXED_CG, srcs, 0, eps, sloppy)
+    nr, thr = sa.rhs_block()
+    print("autotuned batch: %d RHS, %d threads" % (nr, thr))
+    assert nr in (4, 6, 8, 12, 16)
+    for q, b in enumerate(srcs):
+        assert inf4[q].converged and inf1[q].converged and infa[q].converged
+        assert inf4[q].steps == inf1[q].steps == infa[q].steps
+        assert np.array_equal(xs4[q], xs1[q]) and np.array_equal(xsa[q], xs1[q])
+        assert true_residual(case, xs4[q], b) <= 1e-10
+    xo, oinfo, ok = case.orc.solve("precond_bicgstab", srcs[0], eps[0], kind=K_CLOVER, max_steps=3000)
+    assert ok
+    assert np.linalg.norm((xs4[0] - xo).ravel()) / np.linalg.norm(xo.ravel()) < 1e-9
+
+
+def test_batch_agrees_with_single_solves(case):
+    """A batch and the single-RHS mixed CG solve the same systems (Q^2 x = Q g5 b)."""
+    bs = [case.gauss() for _ in range(3)]
+    rhs = [case.orc.apply(K_CLOVER, g5(b)) for b in bs]
+    eps = (1e-11 * max(np.linalg.norm(r) for r in rhs)) ** 2
+    op = case.op(lym.OP_CLOVER, power=2)
+    xs, infos, _ = solve_batch(case, op, lym.SOLVER_MIXED_CG, rhs, 4, None, eps=eps)
+    single = lym.Solver(case.ctx, lym.SOLVER_MIXED_CG, precision=eps, max_steps=20000, sloppy=lym.PREC_HALF)
+    for q in range(3):
+        X = case.spinor()
+        info = single.solve(op, case.spinor(rhs[q]), X)
+        assert info.converged and infos[q].converged
+        assert true_residual(case, xs[q], bs[q]) <= 1e-10
+        x1 = X.download()
+        assert np.linalg.norm((xs[q] - x1).ravel()) / np.linalg.norm(x1.ravel()) < 1e-9
+        assert abs(int(infos[q].steps) - int(info.steps)) <= max(3, info.steps // 20)
+
+
+def test_even_odd_operator_batch(case):
+    """Batched mixed CG on Qhat^2 (odd half); even half of each solution = source."""
+    srcs = [case.gauss() for _ in range(2)]
+    eps = (1e-11 * np.linalg.norm(srcs[0])) ** 2
+    xs, infos, _ = solve_batch(case, case.op(lym.OP_EO_CLOVER, power=2), lym.SOLVER_MIXED_CG, srcs, 2, eps=eps)
+    par = coords(case.dims).sum(1) % 2
+    xo, oinfo, ok = case.orc.solve("cg", srcs[0], eps, kind=K_EO, power=2, max_steps=5000)
+    assert ok and infos[0].converged
+    assert np.linalg.norm((xs[0] - xo).ravel()) / np.linalg.norm(xo.ravel()) < 1e-9
+    for q in range(2):
+        assert np.array_equal(xs[q][par == 0], srcs[q][par == 0])
+
+
+def test_batch_edge_cases(case):
+    """A zero source, a loose and a tight tolerance, and max steps, in one batch."""
+    b = case.gauss()
+    zero = np.zeros_like(b)
+    srcs = [b, zero, b]
+    base = (1e-11 * np.linalg.norm(b)) ** 2
+    xs, infos, _ = solve_batch(case, case.op(lym.OP_CLOVER), lym.SOLVER_PRECOND_MIXED_CG, srcs, 3,
+                               [base * 1e10, base, base])
+    assert infos[1].converged and infos[1].steps == 0 and not np.any(xs[1])
+    assert infos[0].converged and infos[2].converged
+    assert infos[0].steps < infos[2].steps
+    assert true_residual(case, xs[2], b) <= 1e-10
+    # the solve stops at max_steps per RHS and reports it
+    xs, infos, _ = solve_batch(case, case.op(lym.OP_CLOVER), lym.SOLVER_PRECOND_MIXED_CG, [b, b], 2,
+                               [1e-300, 1e-300], max_steps=5)
+    assert all(not i.converged and i.steps == 5 for i in infos)
+    assert all(np.all(np.isfinite(x)) for x in xs)
+    # kinds without a batched form are refused with a code, not a crash
+    solver = lym.Solver(case.ctx, lym.SOLVER_BICGSTAB)
+    with pytest.raises(lym.LymError):
+        solver.solve_batch(case.op(lym.OP_CLOVER), [case.spinor(b)], [case.spinor()])
+    with pytest.raises(lym.LymError):
+        lym.Solver(case.ctx, lym.SOLVER_MIXED_CG).set_rhs_block(5)
