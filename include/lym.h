@@ -131,7 +131,11 @@ int lym_op_create(lym_ctx* ctx, int kind, unsigned power, double kappa, double c
 int lym_op_create_by_name(lym_ctx* ctx, const char* name, unsigned power, double kappa, double csw, int gamma5,
                           lym_op** out);
 void lym_op_destroy(lym_op* op);
-int lym_op_set_gauge(lym_op* op, const lym_gauge* g);   /* setLattice (:125) */
+/* setLattice (:125).  The gauge field may belong to another context of the
+ * same geometry on the same device (one device-resident copy serving several
+ * streams); its clover field-strength blocks are built on first use, which must
+ * then not race between threads (prepare one operator on it first). */
+int lym_op_set_gauge(lym_op* op, const lym_gauge* g);
 int lym_op_set_kappa(lym_op* op, double kappa);         /* setKappa (:107); diagonal blocks rebuilt */
 int lym_op_set_csw(lym_op* op, double csw);             /* ImprovedDiracWilsonOperator::setCSW */
 int lym_op_set_gamma5(lym_op* op, int gamma5);          /* setGamma5 (:127) */
@@ -176,6 +180,48 @@ int lym_solver_solve_batch(lym_solver* s, lym_op* op, unsigned nrhs, const lym_s
 int lym_solver_set_rhs_block(lym_solver* s, unsigned nrhs_per_batch);
 /* the batch size and workgroup size in use (threads 0: default) */
 int lym_solver_get_rhs_block(lym_solver* s, unsigned* nrhs_per_batch, unsigned* threads);
+
+/* ------------------------------------------------------------------ serving
+ * A request loop over device-resident gauge fields (no counterpart in the
+ * reference, whose callers -- MesonCorrelator.cpp:25-99,
+ * ChiralCondensate.cpp:43-120 -- solve one source at a time in their own
+ * loop).  Each request solves D x = b for the clover operator of its lattice
+ * and kappa with the even-odd mixed-precision CG driver to
+ * |b - D x| <= tolerance |b|; worker threads (one HIP stream each per lattice)
+ * take requests in arrival order and batch the queued ones that share lattice
+ * and kappa.  Device state is charged against an HBM budget (0: 80% of the
+ * free HBM at creation); a worker releases its least recently used lattice
+ * state to admit a new one.  The source must stay valid and the solution
+ * writable until the request completes.  A result does not depend on the
+ * other requests in flight. */
+typedef struct lym_server lym_server;
+#define LYM_REQ_PENDING 0
+#define LYM_REQ_RUNNING 1
+#define LYM_REQ_DONE 2
+#define LYM_REQ_FAILED 3
+typedef struct {
+    int status;              /* LYM_REQ_* */
+    int error;               /* LYM_ERR_* when failed */
+    unsigned batch_size;     /* requests solved together with this one */
+    int worker;
+    double queue_seconds;    /* arrival -> start of service */
+    double service_seconds;  /* start of service -> result in host memory */
+    double latency_seconds;  /* arrival -> result in host memory */
+    lym_solve_info solve;
+} lym_request_info;
+int lym_server_create(unsigned workers, size_t hbm_budget_bytes, unsigned max_batch, lym_server** out);
+void lym_server_destroy(lym_server* s);   /* finishes queued requests first */
+/* sloppy precision LYM_PREC_SINGLE | LYM_PREC_HALF (default HALF), max steps (0: keep 20000) */
+int lym_server_set_solver(lym_server* s, int sloppy_prec, unsigned max_steps);
+int lym_server_add_lattice(lym_server* s, const int dims[4], int rep, int bc_t, const double* links,
+                           int links_have_bc_folded, int storage, double csw, int* lattice_id);
+int lym_server_submit(lym_server* s, int lattice_id, double kappa, double tolerance, const double* source,
+                      double* solution, uint64_t* request_id);
+/* timeout < 0 waits for completion; otherwise info->status tells whether it completed */
+int lym_server_wait(lym_server* s, uint64_t request_id, double timeout_seconds, lym_request_info* info);
+int lym_server_release(lym_server* s, uint64_t request_id);   /* forget a completed request */
+int lym_server_drain(lym_server* s);                           /* wait until nothing is queued or running */
+int lym_server_memory(lym_server* s, size_t* budget_bytes, size_t* in_use_bytes);
 
 /* ------------------------------------------------------------------ host-side input preparation
  * Seeded synthetic inputs (the reference seeds from time(NULL),

@@ -17,6 +17,7 @@ OP_WILSON, OP_BASIC_WILSON, OP_CLOVER, OP_EO_CLOVER = 0, 1, 2, 3
 SOLVER_CG, SOLVER_BICGSTAB, SOLVER_PRECOND_BICGSTAB = 0, 1, 2
 SOLVER_MIXED_CG, SOLVER_PRECOND_CG, SOLVER_PRECOND_MIXED_CG = 3, 4, 5
 PREC_DOUBLE, PREC_SINGLE, PREC_HALF = 0, 1, 2
+REQ_PENDING, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 LINKS_FULL, LINKS_RECONSTRUCT = 0, 1
 
 # C ABI: name -> (restype, argtypes)
@@ -33,6 +34,12 @@ _PI = ctypes.POINTER(ctypes.c_int)
 class SolveInfo(ctypes.Structure):
     _fields_ = [("converged", ctypes.c_int), ("steps", ctypes.c_uint), ("last_error", ctypes.c_double),
                 ("applies", ctypes.c_uint), ("seconds", ctypes.c_double), ("reliable_updates", ctypes.c_uint)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("error", ctypes.c_int), ("batch_size", ctypes.c_uint),
+                ("worker", ctypes.c_int), ("queue_seconds", ctypes.c_double), ("service_seconds", ctypes.c_double),
+                ("latency_seconds", ctypes.c_double), ("solve", SolveInfo)]
 
 
 SIGNATURES = {
@@ -79,6 +86,15 @@ SIGNATURES = {
     "lym_solver_solve_batch": (_I, [_P, _P, _U, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_D),
                                     ctypes.POINTER(SolveInfo)]),
     "lym_solver_set_rhs_block": (_I, [_P, _U]),
+    "lym_server_create": (_I, [_U, ctypes.c_size_t, _U, ctypes.POINTER(_P)]),
+    "lym_server_destroy": (None, [_P]),
+    "lym_server_set_solver": (_I, [_P, _I, _U]),
+    "lym_server_add_lattice": (_I, [_P, _PI, _I, _I, _PD, _I, _I, _D, _PI]),
+    "lym_server_submit": (_I, [_P, _I, _D, _D, _PD, _PD, ctypes.POINTER(_U64)]),
+    "lym_server_wait": (_I, [_P, _U64, _D, ctypes.POINTER(RequestInfo)]),
+    "lym_server_release": (_I, [_P, _U64]),
+    "lym_server_drain": (_I, [_P]),
+    "lym_server_memory": (_I, [_P, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     "lym_solver_get_rhs_block": (_I, [_P, ctypes.POINTER(_U), ctypes.POINTER(_U)]),
     "lym_gen_su3_links": (_I, [_P, _D, _U64, _P]),
     "lym_gen_su2_links": (_I, [_P, _D, _U64, _P]),
@@ -382,6 +398,75 @@ class Solver:
         if self.h:
             lib().lym_solver_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request loop over device-resident gauge fields (see lym_server_* in lym.h).
+
+    submit() keeps references to the source and solution arrays until the
+    request is waited for, as the library reads and writes them asynchronously."""
+
+    def __init__(self, workers=4, hbm_budget=0, max_batch=16, sloppy=None, max_steps=0):
+        h = ctypes.c_void_p()
+        _check(lib().lym_server_create(workers, hbm_budget, max_batch, ctypes.byref(h)))
+        self.h = h
+        self._live = {}
+        if sloppy is not None or max_steps:
+            _check(lib().lym_server_set_solver(self.h, PREC_HALF if sloppy is None else sloppy, max_steps))
+
+    def add_lattice(self, dims, links, rep=REP_SU3_FUND, bc=-1, folded=False, storage=LINKS_FULL, csw=1.0):
+        """links: [V,4,3,3], element (i, j), as for Gauge."""
+        dt = np.float64 if rep == REP_SU2_ADJ else np.complex128
+        links = np.ascontiguousarray(np.swapaxes(np.asarray(links, dt), -1, -2))
+        d = (ctypes.c_int * 4)(*dims)
+        lid = ctypes.c_int()
+        _check(lib().lym_server_add_lattice(self.h, d, rep, bc, links.ctypes.data_as(_PD), int(folded), storage,
+                                            csw, ctypes.byref(lid)))
+        return lid.value
+
+    def submit(self, lattice, kappa, tol, source, out=None):
+        source = np.ascontiguousarray(source, dtype=np.complex128)
+        if out is None:
+            out = np.empty_like(source)
+        if out.dtype != np.complex128 or not out.flags.c_contiguous or out.size != source.size:
+            raise ValueError("solution must be a C-contiguous complex128 array of the source's size")
+        rid = _U64()
+        _check(lib().lym_server_submit(self.h, lattice, kappa, tol, source.ctypes.data_as(_PD),
+                                       out.ctypes.data_as(_PD), ctypes.byref(rid)))
+        self._live[rid.value] = (source, out)
+        return rid.value
+
+    def wait(self, rid, timeout=-1.0):
+        """(RequestInfo, solution array); the request is released once complete."""
+        info = RequestInfo()
+        rc = lib().lym_server_wait(self.h, rid, timeout, ctypes.byref(info))
+        if info.status in (REQ_DONE, REQ_FAILED):
+            src, out = self._live.pop(rid)
+            lib().lym_server_release(self.h, rid)
+            _check(rc)
+            return info, out
+        _check(rc)
+        return info, None
+
+    def drain(self):
+        _check(lib().lym_server_drain(self.h))
+
+    def memory(self):
+        a, b = ctypes.c_size_t(), ctypes.c_size_t()
+        _check(lib().lym_server_memory(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if self.h:
+            lib().lym_server_destroy(self.h)
+            self.h = None
+            self._live.clear()
 
     def __del__(self):
         try:

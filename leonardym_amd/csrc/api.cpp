@@ -702,7 +702,11 @@ void lym_op_destroy(lym_op* op) {
 }
 
 int lym_op_set_gauge(lym_op* op, const lym_gauge* g) {
-    if (!op || !g || g->ctx != op->ctx) return fail(LYM_ERR_INVALID, "gauge field of another context");
+    if (!op || !g) return fail(LYM_ERR_INVALID, "null argument");
+    const lym_ctx *a = op->ctx, *b = g->ctx;
+    if (a != b && (a->device != b->device || a->rep != b->rep || a->g.bc_t != b->g.bc_t ||
+                   std::memcmp(a->dims, b->dims, sizeof(a->dims)) != 0))
+        return fail(LYM_ERR_INVALID, "gauge field of a context with another geometry or device");
     op->gauge = g;
     op->diag_dirty = 1;
     op->sloppy_prec = 0;

@@ -1,0 +1,461 @@
+// server.cpp -- solve-request serving loop over device-resident gauge fields.
+//
+// Callers register lattices (links uploaded once, kept in HBM) and submit
+// requests (lattice, kappa, tolerance, source, destination).  Worker threads,
+// each with its own HIP stream (default priority) per lattice, take requests
+// in arrival order; a worker batches the queued requests that share lattice
+// and kappa (same operator) into one multi-RHS solve.  Device state per
+// (worker, lattice) -- context, operator with its clover blocks, solver work
+// vectors and batch buffers, spinors -- is created on first use and charged
+// against an HBM budget; a worker that would exceed it first releases the
+// least recently used idle state (of any worker), and otherwise waits until
+// another worker's batch ends.
+//
+// Each request is solved by the even-odd mixed-precision CG driver (D x = b,
+// the reference's PreconditionedBiCGStab entry point,
+// source/inverters/PreconditionedBiCGStab.cpp:43-91) to |b - D x| <= tol |b|
+// through the stopping rule |r|^2 < (0.1 tol |b|)^2 on the even-odd system.
+// Results do not depend on what else is queued: a request gives the same
+// bits alone and in any batch (see batch.hip).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/lym.h"
+#include "lym_common.h"
+
+namespace lym_internal {
+int fail(int code, const char* fmt, ...);
+}
+using lym_internal::fail;
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+struct Request {
+    uint64_t id;
+    int lattice;
+    double kappa, tol;
+    const double* src;
+    double* out;
+    clk::time_point t_arrival, t_start, t_done;
+    int status = LYM_REQ_PENDING;
+    int error = 0;
+    std::string msg;
+    lym_solve_info info{};
+    unsigned batch = 0;
+    int worker = -1;
+};
+
+struct Lattice {
+    int dims[4];
+    int rep, bc, storage;
+    double csw;
+    lym_ctx* base = nullptr;
+    lym_gauge* gauge = nullptr;
+};
+
+// device state of one worker for one lattice
+struct WState {
+    lym_ctx* ctx = nullptr;
+    lym_op* op = nullptr;
+    lym_solver* solver = nullptr;
+    std::vector<lym_spinor*> src, sol;
+    size_t charge = 0;
+    uint64_t last_use = 0;
+    bool in_use = false;
+};
+
+void free_state(WState& s) {
+    for (auto* p : s.src) lym_spinor_destroy(p);
+    for (auto* p : s.sol) lym_spinor_destroy(p);
+    s.src.clear();
+    s.sol.clear();
+    if (s.solver) lym_solver_destroy(s.solver);
+    if (s.op) lym_op_destroy(s.op);
+    if (s.ctx) lym_ctx_destroy(s.ctx);
+    s.solver = nullptr;
+    s.op = nullptr;
+    s.ctx = nullptr;
+}
+
+double seconds(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+}  // namespace
+
+struct lym_server {
+    int device = 0;
+    unsigned nworkers = 1, max_batch = 16;
+    int sloppy = LYM_PREC_HALF;
+    unsigned max_steps = 20000;
+    size_t budget = 0, used = 0;
+    std::vector<std::unique_ptr<Lattice>> lattices;
+    std::vector<std::map<int, WState>> states;  // per worker
+    std::mutex m;
+    std::condition_variable cv_work, cv_done, cv_mem;
+    std::deque<Request*> queue;
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
+    std::vector<std::thread> threads;
+    unsigned busy = 0;
+    uint64_t next_id = 1, tick = 0;
+    bool stop = false;
+};
+
+namespace {
+
+// device bytes of one worker's state for a lattice, for batches up to nr RHS:
+// operator (blocks, inverse, scratch, low-precision copies), solver (8 FP64
+// work vectors, mixed and batch buffers), nr source and solution spinors
+size_t state_bytes(const Lattice& L, unsigned nr) {
+    const double V = (double)L.dims[0] * L.dims[1] * L.dims[2] * L.dims[3];
+    const double full = V * 12 * 16;  // FP64 full spinor
+    const double op = V * (36 * 16 + 18 * 16) + 2 * full + V * 4 * 18 * 4 + V * 54 * 4;
+    const double solver = 8 * full + 6 * full / 2;
+    const double batch = nr * (2 * full + 8 * full / 2 + 4 * V * 12 * 4 + 2 * V * 6 * 8);
+    const double spinors = 2.0 * nr * full + full;  // + staging
+    return (size_t)((op + solver + batch + spinors) * 1.05) + (64u << 20);
+}
+
+int make_state(lym_server* S, int w, int li, WState& st) {
+    const Lattice& L = *S->lattices[li];
+    int rc = lym_ctx_create(L.dims, L.rep, L.bc, &st.ctx);
+    if (!rc) rc = lym_op_create(st.ctx, LYM_OP_CLOVER, 1, 0.125, L.csw, 1, &st.op);
+    if (!rc) rc = lym_op_set_gauge(st.op, L.gauge);
+    if (!rc) rc = lym_solver_create(st.ctx, LYM_SOLVER_PRECOND_MIXED_CG, &st.solver);
+    if (!rc) rc = lym_solver_set_sloppy_precision(st.solver, S->sloppy);
+    if (!rc) rc = lym_solver_set_max_steps(st.solver, S->max_steps);
+    if (!rc) rc = lym_solver_set_rhs_block(st.solver, S->max_batch);
+    for (unsigned q = 0; !rc && q < S->max_batch; ++q) {
+        lym_spinor *a = nullptr, *b = nullptr;
+        rc = lym_spinor_create(st.ctx, &a);
+        if (!rc) st.src.push_back(a);
+        if (!rc) rc = lym_spinor_create(st.ctx, &b);
+        if (!rc) st.sol.push_back(b);
+    }
+    (void)w;
+    if (rc) free_state(st);
+    return rc;
+}
+
+// the next batch: the oldest request and the queued ones sharing its operator
+std::vector<Request*> take_batch(lym_server* S) {
+    std::vector<Request*> b;
+    Request* r0 = S->queue.front();
+    for (auto it = S->queue.begin(); it != S->queue.end() && b.size() < S->max_batch;) {
+        Request* r = *it;
+        if (r->lattice == r0->lattice && r->kappa == r0->kappa) {
+            b.push_back(r);
+            it = S->queue.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    return b;
+}
+
+void finish(lym_server* S, std::vector<Request*>& batch, int rc, const char* msg) {
+    auto now = clk::now();
+    std::lock_guard<std::mutex> lk(S->m);
+    for (Request* r : batch) {
+        r->t_done = now;
+        r->status = rc ? LYM_REQ_FAILED : LYM_REQ_DONE;
+        r->error = rc;
+        if (rc) r->msg = msg ? msg : "";
+    }
+}
+
+// device state for (w, li), created within the budget; called with S->m held.
+// States of other workers may be released here, but only idle ones: a worker
+// holds a pointer only to the state it is using.
+WState* admit(lym_server* S, int w, int li, std::unique_lock<std::mutex>& lk, int* rc) {
+    auto& mine = S->states[w];
+    auto it = mine.find(li);
+    if (it != mine.end()) {
+        it->second.last_use = ++S->tick;
+        it->second.in_use = true;
+        return &it->second;
+    }
+    const size_t need = state_bytes(*S->lattices[li], S->max_batch);
+    for (;;) {
+        while (S->used + need > S->budget) {
+            int vw = -1, vl = -1;
+            uint64_t oldest = UINT64_MAX;
+            for (int k = 0; k < (int)S->states.size(); ++k)
+                for (auto& kv : S->states[k])
+                    if (!kv.second.in_use && kv.second.last_use < oldest) {
+                        oldest = kv.second.last_use;
+                        vw = k;
+                        vl = kv.first;
+                    }
+            if (vw < 0) break;
+            WState dead = S->states[vw][vl];
+            S->states[vw].erase(vl);
+            S->used -= dead.charge;
+            lk.unlock();
+            free_state(dead);
+            lk.lock();
+        }
+        if (S->used + need <= S->budget) break;
+        if (S->busy <= 1) {  // nothing in use elsewhere that could become idle
+            *rc = fail(LYM_ERR_NOMEM, "lattice state of %zu bytes exceeds the HBM budget of %zu", need, S->budget);
+            return nullptr;
+        }
+        S->cv_mem.wait(lk);
+    }
+    S->used += need;
+    lk.unlock();
+    WState st;
+    *rc = make_state(S, w, li, st);
+    lk.lock();
+    if (*rc) {
+        S->used -= need;
+        S->cv_mem.notify_all();
+        return nullptr;
+    }
+    st.charge = need;
+    st.last_use = ++S->tick;
+    st.in_use = true;
+    mine[li] = st;
+    return &mine[li];
+}
+
+void worker(lym_server* S, int w) {
+    (void)hipSetDevice(S->device);
+    std::unique_lock<std::mutex> lk(S->m);
+    for (;;) {
+        S->cv_work.wait(lk, [&] { return S->stop || !S->queue.empty(); });
+        if (S->stop && S->queue.empty()) break;
+        std::vector<Request*> batch = take_batch(S);
+        const int li = batch[0]->lattice;
+        const double kappa = batch[0]->kappa;
+        auto t0 = clk::now();
+        for (Request* r : batch) {
+            r->status = LYM_REQ_RUNNING;
+            r->t_start = t0;
+            r->worker = w;
+            r->batch = (unsigned)batch.size();
+        }
+        ++S->busy;
+        int rc = LYM_OK;
+        WState* st = admit(S, w, li, lk, &rc);
+        lk.unlock();
+        std::string msg;
+        if (st) {
+            const unsigned n = (unsigned)batch.size();
+            std::vector<double> eps(n);
+            std::vector<lym_solve_info> infos(n);
+            rc = lym_op_set_kappa(st->op, kappa);
+            for (unsigned q = 0; !rc && q < n; ++q) {
+                double b2 = 0;
+                rc = lym_spinor_upload(st->src[q], batch[q]->src);
+                if (!rc) rc = lym_spinor_squared_norm(st->src[q], &b2);
+                const double t = 0.1 * batch[q]->tol;
+                eps[q] = t * t * b2;
+            }
+            if (!rc) rc = lym_solver_solve_batch(st->solver, st->op, n, st->src.data(), st->sol.data(), eps.data(),
+                                                 infos.data());
+            for (unsigned q = 0; !rc && q < n; ++q) rc = lym_spinor_download(st->sol[q], batch[q]->out);
+            if (!rc)
+                for (unsigned q = 0; q < n; ++q) batch[q]->info = infos[q];
+            if (rc) msg = lym_last_error();
+        } else {
+            msg = lym_last_error();
+        }
+        finish(S, batch, rc, msg.c_str());
+        lk.lock();
+        if (st) st->in_use = false;
+        --S->busy;
+        S->cv_done.notify_all();
+        S->cv_mem.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int lym_server_create(unsigned workers, size_t hbm_budget_bytes, unsigned max_batch, lym_server** out) {
+    if (!out) return fail(LYM_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (workers == 0 || workers > 16) return fail(LYM_ERR_INVALID, "workers must be 1..16");
+    if (max_batch == 0 || max_batch > 16) return fail(LYM_ERR_INVALID, "max_batch must be 1..16");
+    int dev = 0, ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(LYM_ERR_DEVICE, "no HIP device available");
+    (void)hipGetDevice(&dev);
+    size_t freeb = 0, total = 0;
+    if (hipMemGetInfo(&freeb, &total) != hipSuccess) return fail(LYM_ERR_DEVICE, "hipMemGetInfo");
+    auto* S = new lym_server();
+    S->device = dev;
+    S->nworkers = workers;
+    S->max_batch = max_batch;
+    // default budget: 80% of the free HBM
+    S->budget = hbm_budget_bytes ? std::min(hbm_budget_bytes, freeb) : (size_t)(0.8 * (double)freeb);
+    S->states.resize(workers);
+    for (unsigned w = 0; w < workers; ++w) S->threads.emplace_back(worker, S, (int)w);
+    *out = S;
+    return LYM_OK;
+}
+
+void lym_server_destroy(lym_server* S) {
+    if (!S) return;
+    {
+        std::lock_guard<std::mutex> lk(S->m);
+        S->stop = true;
+    }
+    S->cv_work.notify_all();
+    for (auto& t : S->threads) t.join();
+    for (auto& mine : S->states)
+        for (auto& kv : mine) free_state(kv.second);
+    for (auto& L : S->lattices) {
+        if (L->gauge) lym_gauge_destroy(L->gauge);
+        if (L->base) lym_ctx_destroy(L->base);
+    }
+    delete S;
+}
+
+int lym_server_set_solver(lym_server* S, int sloppy_prec, unsigned max_steps) {
+    if (!S) return fail(LYM_ERR_INVALID, "null server");
+    if (sloppy_prec != LYM_PREC_SINGLE && sloppy_prec != LYM_PREC_HALF)
+        return fail(LYM_ERR_INVALID, "sloppy precision must be LYM_PREC_SINGLE or LYM_PREC_HALF");
+    std::lock_guard<std::mutex> lk(S->m);
+    if (!S->reqs.empty()) return fail(LYM_ERR_INVALID, "set the solver before the first request");
+    S->sloppy = sloppy_prec;
+    if (max_steps) S->max_steps = max_steps;
+    return LYM_OK;
+}
+
+int lym_server_add_lattice(lym_server* S, const int dims[4], int rep, int bc_t, const double* links,
+                           int links_have_bc_folded, int storage, double csw, int* lattice_id) {
+    if (!S || !dims || !links || !lattice_id) return fail(LYM_ERR_INVALID, "null argument");
+    (void)hipSetDevice(S->device);
+    auto L = std::make_unique<Lattice>();
+    std::memcpy(L->dims, dims, sizeof(L->dims));
+    L->rep = rep;
+    L->bc = bc_t;
+    L->storage = storage;
+    L->csw = csw;
+    int rc = lym_ctx_create(dims, rep, bc_t, &L->base);
+    if (!rc) rc = lym_gauge_create(L->base, links, links_have_bc_folded, storage, &L->gauge);
+    if (!rc && csw != 0.0) {
+        // build the gauge field's clover blocks once, before workers share it
+        lym_op* op = nullptr;
+        lym_spinor *a = nullptr, *b = nullptr;
+        rc = lym_op_create(L->base, LYM_OP_CLOVER, 1, 0.125, csw, 1, &op);
+        if (!rc) rc = lym_op_set_gauge(op, L->gauge);
+        if (!rc) rc = lym_spinor_create(L->base, &a);
+        if (!rc) rc = lym_spinor_create(L->base, &b);
+        if (!rc) rc = lym_op_multiply(op, b, a);
+        if (!rc) rc = lym_ctx_synchronize(L->base);
+        if (a) lym_spinor_destroy(a);
+        if (b) lym_spinor_destroy(b);
+        if (op) lym_op_destroy(op);
+    }
+    if (rc) {
+        if (L->gauge) lym_gauge_destroy(L->gauge);
+        if (L->base) lym_ctx_destroy(L->base);
+        return rc;
+    }
+    std::lock_guard<std::mutex> lk(S->m);
+    S->lattices.push_back(std::move(L));
+    *lattice_id = (int)S->lattices.size() - 1;
+    return LYM_OK;
+}
+
+int lym_server_submit(lym_server* S, int lattice_id, double kappa, double tolerance, const double* source,
+                      double* solution, uint64_t* request_id) {
+    if (!S || !source || !solution || !request_id) return fail(LYM_ERR_INVALID, "null argument");
+    if (!(tolerance > 0)) return fail(LYM_ERR_INVALID, "tolerance must be positive");
+    if (!(kappa > 0)) return fail(LYM_ERR_INVALID, "kappa must be positive");
+    auto r = std::make_unique<Request>();
+    r->t_arrival = clk::now();
+    r->lattice = lattice_id;
+    r->kappa = kappa;
+    r->tol = tolerance;
+    r->src = source;
+    r->out = solution;
+    {
+        std::lock_guard<std::mutex> lk(S->m);
+        if (lattice_id < 0 || lattice_id >= (int)S->lattices.size())
+            return fail(LYM_ERR_INVALID, "unknown lattice %d", lattice_id);
+        r->id = S->next_id++;
+        *request_id = r->id;
+        S->queue.push_back(r.get());
+        S->reqs[r->id] = std::move(r);
+    }
+    S->cv_work.notify_one();
+    return LYM_OK;
+}
+
+static void fill_info(const Request& r, lym_request_info* info) {
+    std::memset(info, 0, sizeof(*info));
+    info->status = r.status;
+    info->error = r.error;
+    info->batch_size = r.batch;
+    info->worker = r.worker;
+    info->solve = r.info;
+    if (r.status == LYM_REQ_DONE || r.status == LYM_REQ_FAILED) {
+        info->queue_seconds = seconds(r.t_arrival, r.t_start);
+        info->service_seconds = seconds(r.t_start, r.t_done);
+        info->latency_seconds = seconds(r.t_arrival, r.t_done);
+    }
+}
+
+int lym_server_wait(lym_server* S, uint64_t request_id, double timeout_seconds, lym_request_info* info) {
+    if (!S || !info) return fail(LYM_ERR_INVALID, "null argument");
+    std::unique_lock<std::mutex> lk(S->m);
+    auto it = S->reqs.find(request_id);
+    if (it == S->reqs.end()) return fail(LYM_ERR_INVALID, "unknown request %llu", (unsigned long long)request_id);
+    Request* r = it->second.get();
+    auto done = [&] { return r->status == LYM_REQ_DONE || r->status == LYM_REQ_FAILED; };
+    if (timeout_seconds < 0) {
+        S->cv_done.wait(lk, done);
+    } else if (!S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_seconds), done)) {
+        fill_info(*r, info);  // still queued or running: info->status says which
+        return LYM_OK;
+    }
+    fill_info(*r, info);
+    if (r->status == LYM_REQ_FAILED) return fail(r->error, "request %llu: %s", (unsigned long long)request_id,
+                                                 r->msg.c_str());
+    return LYM_OK;
+}
+
+int lym_server_release(lym_server* S, uint64_t request_id) {
+    if (!S) return fail(LYM_ERR_INVALID, "null server");
+    std::lock_guard<std::mutex> lk(S->m);
+    auto it = S->reqs.find(request_id);
+    if (it == S->reqs.end()) return fail(LYM_ERR_INVALID, "unknown request");
+    if (it->second->status != LYM_REQ_DONE && it->second->status != LYM_REQ_FAILED)
+        return fail(LYM_ERR_INVALID, "request %llu still in flight", (unsigned long long)request_id);
+    S->reqs.erase(it);
+    return LYM_OK;
+}
+
+int lym_server_drain(lym_server* S) {
+    if (!S) return fail(LYM_ERR_INVALID, "null server");
+    std::unique_lock<std::mutex> lk(S->m);
+    S->cv_done.wait(lk, [&] { return S->queue.empty() && S->busy == 0; });
+    return LYM_OK;
+}
+
+int lym_server_memory(lym_server* S, size_t* budget, size_t* in_use) {
+    if (!S) return fail(LYM_ERR_INVALID, "null server");
+    std::lock_guard<std::mutex> lk(S->m);
+    if (budget) *budget = S->budget;
+    if (in_use) *in_use = S->used;
+    return LYM_OK;
+}
+
+}  // extern "C"

@@ -19,6 +19,7 @@
 #include <cstring>
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <string>
 
 #include "../../include/lym.h"
@@ -446,6 +447,10 @@ std::map<TuneKey, std::pair<int, int>>& tune_cache() {
     static std::map<TuneKey, std::pair<int, int>> m;
     return m;
 }
+std::mutex& tune_mutex() {
+    static std::mutex m;
+    return m;
+}
 
 int autotune_batch(lym_solver* s, lym_op* op, bool half, int prec, unsigned nrhs, int* nr_out, int* thr_out) {
     lym_ctx* c = s->ctx;
@@ -455,15 +460,18 @@ int autotune_batch(lym_solver* s, lym_op* op, bool half, int prec, unsigned nrhs
     key.lk = lym_internal::link_kind_of(op->gauge);
     key.prec = prec;
     key.half = half;
-    auto it = tune_cache().find(key);
     const int cands[5] = {4, 6, 8, 12, 16};
     int maxnr = 4;
     for (int k : cands)
         if ((unsigned)k <= std::max(nrhs, 4u)) maxnr = k;
-    if (it != tune_cache().end() && it->second.first <= maxnr) {
-        *nr_out = it->second.first;
-        *thr_out = it->second.second;
-        return LYM_OK;
+    {
+        std::lock_guard<std::mutex> lk(tune_mutex());
+        auto it = tune_cache().find(key);
+        if (it != tune_cache().end() && it->second.first <= maxnr) {
+            *nr_out = it->second.first;
+            *thr_out = it->second.second;
+            return LYM_OK;
+        }
     }
     int rc = ensure_batch(s, maxnr, half ? 1 : 2, prec);
     if (rc) return rc;
@@ -489,7 +497,10 @@ int autotune_batch(lym_solver* s, lym_op* op, bool half, int prec, unsigned nrhs
             }
         }
     }
-    tune_cache()[key] = {bnr, bthr};
+    {
+        std::lock_guard<std::mutex> lk(tune_mutex());
+        tune_cache()[key] = {bnr, bthr};
+    }
     *nr_out = bnr;
     *thr_out = bthr;
     return LYM_OK;
