@@ -11,11 +11,26 @@ working set (~604 MB) exceeds the 256 MB Infinity Cache.
 
 value = algorithmic bytes / time, with 1536 B per site
   (4 links x 18 reals + spinor in 24 + out 24 + clover 72 reals, FP64).
-Also reported: the even-odd Schur apply, FP64 solves per second to true
-residual 1e-10 (CG on Q^2 and the even-odd BiCGStab driver), and the CPU
-oracle (the FP64 restatement of the reference loops) timed on the host.
+Also reported, each in its own object of the same JSON line:
+  eo_schur  the even-odd Schur apply (4416 B per odd site);
+  solves    FP64 solves/s to true residual 1e-10 on 16^3x32 (CG on Q^2, the
+            even-odd BiCGStab and CG drivers) and the mixed-precision even-odd
+            CG (FP16 and FP32 sloppy storage, FP64 reliable updates), with the
+            error of each mixed solution against the FP64 one;
+  adjoint   the same mixed-vs-FP64 comparison for SU(2) adjoint fermions,
+            24^3x48, kappa = 0.17;
+  batch     36 right-hand sides on 32^3x64 (12 point sources x kappa in
+            {0.130, 0.132, 0.134}): batched mixed CG against FP64 even-odd CG
+            one RHS at a time;
+  serving   Poisson request arrivals over {8^4, 12^4, 16^3x32, 24^3x48}
+            lattices, kappa ~ U[0.120, 0.134], tolerance in {1e-8, 1e-10,
+            1e-12}: p50/p99 latency (arrival -> solution in host memory) and
+            solves/s per arrival rate;
+  cpu_baseline  the oracle (FP64 restatement of the reference loops) timed on
+            the host cores on the same apply.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-solve]
+                [--sections dslash,solve,adjoint,batch,serving] [--serve-full]
 """
 import argparse
 import json
@@ -36,6 +51,22 @@ GAUGE_SEED, SOURCE_SEED = 1001, 2001
 ROTATE = 3
 BYTES_PER_SITE_FULL = (4 * 18 + 24 + 24 + 72) * 8      # 1536
 BYTES_PER_ODD_SITE_SCHUR = (264 + 16 * 18) * 8         # 4416
+
+
+PROFILE = os.path.join(ROOT, "profiles", "k_dslash_clover_fp64_16x16x16x32.md")
+
+
+def profiled_traffic():
+    """HBM bytes per full apply from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (copy-calibrated), or None."""
+    try:
+        with open(PROFILE) as fh:
+            for line in fh:
+                if "traffic_bytes_per_launch:" in line:
+                    return int(line.split(":")[1].strip())
+    except OSError:
+        pass
+    return None
 
 
 def log(msg):
@@ -96,8 +127,19 @@ def gpu_bench(args):
     res["eo_schur"] = {"ms": ms, "gbs": BYTES_PER_ODD_SITE_SCHUR * (V // 2) / (ms * 1e-3) / 1e9}
     log("bench: EO Schur apply %.4f ms  %.1f GB/s" % (ms, res["eo_schur"]["gbs"]))
 
+    secs = set(args.sections.split(","))
     if not args.no_solve:
-        res["solves"] = solve_bench(lym, ctx, sets, V)
+        if "solve" in secs:
+            res["solves"] = solve_bench(lym, ctx, sets, V)
+        if "adjoint" in secs:
+            res["adjoint"] = adjoint_bench(lym)
+        if "batch" in secs:
+            res["batch"] = batch_bench(lym)
+        if "serving" in secs:
+            if args.serve_full:
+                res["serving"] = serve_bench(lym, (1, 2, 5, 10, 20, 50), 500)
+            else:
+                res["serving"] = serve_bench(lym, (5, 20, 50), 60)
     if args.calibrate:
         # k_copy over a full 16^3x32 spinor: 24 * stride * 16 B read and written
         g, op, src, out, eo = sets[0]
@@ -149,7 +191,164 @@ def solve_bench(lym, ctx, sets, V):
     out["fp64_eo_bicgstab"] = {"solves_per_s": 1.0 / dt, "seconds": dt, "iterations": int(info.steps),
                                "true_residual": float(tr), "converged": bool(info.converged)}
     log("bench: FP64 EO BiCGStab: %.3f s/solve, %d iterations, true residual %.2e" % (dt, info.steps, tr))
+    D = lym.Operator(ctx, lym.OP_CLOVER, 1, KAPPA, CSW, False, gauge=g)
+    out.update(mixed_vs_fp64(lym, ctx, op, D, src, "16^3x32", nrep=3))
     return out
+
+
+def mixed_vs_fp64(lym, ctx, op, D, src, label, nrep=2):
+    """FP64 even-odd CG driver against the mixed-precision one (FP16 and FP32
+    sloppy storage) on one source: solves/s, iterations, true residual and the
+    mixed solution's error against the FP64 solution."""
+    out = {}
+    bn = src.squared_norm()
+    eps = (1e-11 ** 2) * bn
+    x64, x = lym.Spinor(ctx), lym.Spinor(ctx)
+    Dx = lym.Spinor(ctx)
+
+    def run(kind, sloppy, xo):
+        s = lym.Solver(ctx, kind, precision=eps, max_steps=20000, sloppy=sloppy)
+        s.solve(op, src, xo)  # warm (allocates, converts)
+        t0 = time.perf_counter()
+        for _ in range(nrep):
+            info = s.solve(op, src, xo)
+        dt = (time.perf_counter() - t0) / nrep
+        D.multiply(Dx, xo)
+        tr = float(np.sqrt(Dx.difference_norm(src) / bn))
+        return {"solves_per_s": 1.0 / dt, "seconds": dt, "iterations": int(info.steps),
+                "reliable_updates": int(info.reliable_updates), "true_residual": tr,
+                "converged": bool(info.converged)}
+
+    out["fp64_eo_cg"] = run(lym.SOLVER_PRECOND_CG, None, x64)
+    xn = x64.squared_norm()
+    for name, sl in (("mixed_fp16_eo_cg", lym.PREC_HALF), ("mixed_fp32_eo_cg", lym.PREC_SINGLE)):
+        r = run(lym.SOLVER_PRECOND_MIXED_CG, sl, x)
+        r["error_vs_fp64"] = float(np.sqrt(x.difference_norm(x64) / xn))
+        r["speedup_vs_fp64"] = r["solves_per_s"] / out["fp64_eo_cg"]["solves_per_s"]
+        out[name] = r
+    for k, v in out.items():
+        log("bench: %s %s: %.4f s/solve, %d it, true residual %.2e%s" % (
+            label, k, v["seconds"], v["iterations"], v["true_residual"],
+            (", error vs FP64 %.2e" % v["error_vs_fp64"]) if "error_vs_fp64" in v else ""))
+    return out
+
+
+def adjoint_bench(lym):
+    dims = (24, 24, 24, 48)
+    V = int(np.prod(dims))
+    R = lym.su2_to_adjoint(lym.gen_su2_links(dims, EPS, 1002))
+    ctx = lym.Context(dims, lym.REP_SU2_ADJ, -1)
+    g = lym.Gauge(ctx, fold_bc(dims, R), folded=True, storage=lym.LINKS_FULL)
+    op = lym.Operator(ctx, lym.OP_CLOVER, 1, 0.17, CSW, True, gauge=g)
+    D = lym.Operator(ctx, lym.OP_CLOVER, 1, 0.17, CSW, False, gauge=g)
+    src = lym.Spinor(ctx, lym.gaussian_spinor(V, 2002))
+    res = mixed_vs_fp64(lym, ctx, op, D, src, "SU(2) adjoint 24^3x48", nrep=1)
+    res["config"] = "SU(2) adjoint, 24^3x48, kappa 0.17, csw 1.0, eps 0.5, antiperiodic t, Gaussian source"
+    return res
+
+
+def batch_bench(lym):
+    """36 RHS on 32^3x64: batched mixed CG (autotuned RHS per batch) against
+    FP64 even-odd CG one RHS at a time (3 of the 36, one per kappa, timed)."""
+    dims = (32, 32, 32, 64)
+    V = int(np.prod(dims))
+    kappas = (0.130, 0.132, 0.134)
+    ctx = lym.Context(dims, lym.REP_SU3_FUND, -1)
+    g = lym.Gauge(ctx, fold_bc(dims, lym.gen_su3_links(dims, EPS, 1003)), folded=True, storage=lym.LINKS_FULL)
+    site = tuple(int(v) for v in np.random.default_rng(2003).integers(0, dims))
+    srcs = []
+    for sc in range(12):
+        srcs.append(lym.Spinor(ctx, lym.point_source(dims, site, sc // 3, sc % 3)))
+    sols = [lym.Spinor(ctx) for _ in range(12)]
+    Dx = lym.Spinor(ctx)
+    eps = 1e-11 ** 2  # |b| = 1 for a point source
+    # FP64, one RHS at a time
+    fp64_t, fp64_it = 0.0, []
+    s64 = lym.Solver(ctx, lym.SOLVER_PRECOND_CG, precision=eps, max_steps=20000)
+    for i, k in enumerate(kappas):
+        op = lym.Operator(ctx, lym.OP_CLOVER, 1, k, CSW, True, gauge=g)
+        if i == 0:
+            s64.solve(op, srcs[0], sols[0])  # warm
+        t0 = time.perf_counter()
+        info = s64.solve(op, srcs[4 * i], sols[0])
+        fp64_t += time.perf_counter() - t0
+        fp64_it.append(int(info.steps))
+    fp64_rate = len(kappas) / fp64_t
+    log("bench: 32^3x64 FP64 EO CG one RHS at a time: %.3f s/solve, iterations %s" % (1 / fp64_rate, fp64_it))
+    # batched mixed precision, 12 RHS per kappa
+    sb = lym.Solver(ctx, lym.SOLVER_PRECOND_MIXED_CG, precision=eps, max_steps=20000, sloppy=lym.PREC_HALF)
+    op = lym.Operator(ctx, lym.OP_CLOVER, 1, kappas[0], CSW, True, gauge=g)
+    sb.solve_batch(op, srcs[:2], sols[:2])  # warm: autotune, allocate
+    t0 = time.perf_counter()
+    its, worst, nconv = [], 0.0, 0
+    for k in kappas:
+        op = lym.Operator(ctx, lym.OP_CLOVER, 1, k, CSW, True, gauge=g)
+        D = lym.Operator(ctx, lym.OP_CLOVER, 1, k, CSW, False, gauge=g)
+        infos = sb.solve_batch(op, srcs, sols)
+        ctx.synchronize()
+        t_k = time.perf_counter()
+        for q in range(12):
+            D.multiply(Dx, sols[q])
+            worst = max(worst, float(np.sqrt(Dx.difference_norm(srcs[q]))))
+        t0 += time.perf_counter() - t_k  # residual checks are not part of the timed work
+        its += [int(i.steps) for i in infos]
+        nconv += sum(int(i.converged) for i in infos)
+    bt = time.perf_counter() - t0
+    nr, thr = sb.rhs_block()
+    rate = 36 / bt
+    log("bench: 32^3x64 batched mixed CG: 36 RHS in %.3f s (%.2f solves/s, %.2fx FP64), RHS/batch %d, "
+        "threads %d, max true residual %.2e" % (bt, rate, rate / fp64_rate, nr, thr, worst))
+    return {"config": "SU(3) clover 32^3x64, csw 1.0, eps 0.5, 12 point sources at site %s x kappa %s" % (site, kappas),
+            "fp64_one_at_a_time": {"solves_per_s": fp64_rate, "iterations": fp64_it, "sampled_rhs": len(kappas)},
+            "batched_mixed": {"solves_per_s": rate, "seconds_36": bt, "rhs_per_batch": nr, "threads": thr,
+                              "iterations_min_max": [min(its), max(its)], "converged": nconv,
+                              "max_true_residual": worst},
+            "speedup": rate / fp64_rate}
+
+
+SERVE_LATTICES = [(8, 8, 8, 8), (12, 12, 12, 12), (16, 16, 16, 32), (24, 24, 24, 48)]
+
+
+def serve_bench(lym, lambdas, nreq, workers=4):
+    rng = np.random.default_rng(6001)
+    srv = lym.Server(workers=workers, max_batch=4)
+    ids, srcs = [], []
+    for i, d in enumerate(SERVE_LATTICES):
+        ids.append(srv.add_lattice(d, lym.gen_su3_links(d, EPS, 4000 + i), csw=CSW))
+        V = int(np.prod(d))
+        srcs.append([lym.gaussian_spinor(V, 5000 + 10 * i + k) for k in range(3)])
+    # warm: every worker builds its state for every lattice
+    warm = [srv.submit(ids[i], 0.125, 1e-8, srcs[i][0]) for i in range(len(ids)) for _ in range(2 * workers)]
+    for r in warm:
+        srv.wait(r)
+    out = []
+    for lam in lambdas:
+        arr = np.cumsum(rng.exponential(1.0 / lam, nreq))
+        spec = [(int(rng.integers(len(ids))), float(rng.uniform(0.120, 0.134)),
+                 float(rng.choice([1e-8, 1e-10, 1e-12])), int(rng.integers(3))) for _ in range(nreq)]
+        rids = []
+        t0 = time.perf_counter()
+        for (li, k, tol, si), ta in zip(spec, arr):
+            dt = t0 + ta - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            rids.append(srv.submit(ids[li], k, tol, srcs[li][si]))
+        lat, conv = [], 0
+        for r in rids:
+            info, _ = srv.wait(r)
+            lat.append(info.latency_seconds)
+            conv += int(info.solve.converged)
+        span = time.perf_counter() - t0
+        lat = np.array(lat)
+        row = {"lambda_per_s": lam, "requests": nreq, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+               "p99_ms": float(np.percentile(lat, 99) * 1e3), "mean_ms": float(lat.mean() * 1e3),
+               "solves_per_s": nreq / span, "converged": conv}
+        log("bench: serving lambda %g/s: p50 %.1f ms, p99 %.1f ms, %.2f solves/s" % (
+            lam, row["p50_ms"], row["p99_ms"], row["solves_per_s"]))
+        out.append(row)
+    srv.close()
+    return {"config": "SU(3) clover csw 1.0 eps 0.5 on %s, kappa ~ U[0.120,0.134], tol in {1e-8,1e-10,1e-12}, "
+                      "Poisson arrivals, %d workers" % (SERVE_LATTICES, workers), "rates": out}
 
 
 def cpu_baseline(budget_s=10.0):
@@ -184,13 +383,17 @@ def main():
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-solve", action="store_true", help="only the operator applies")
+    ap.add_argument("--sections", default="solve,adjoint,batch,serving",
+                    help="solver benchmarks to run (comma-separated)")
+    ap.add_argument("--serve-full", action="store_true",
+                    help="serving at lambda 1,2,5,10,20,50 req/s with 500 requests each (long)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run 20 plain spinor copies (known bytes) to scale the HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this setup has a single MI355X; "
-              "a %d-GPU run is not supported" % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU; a %d-GPU run is not implemented (the multi-GPU "
+              "decomposition of this path is designed but not built)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     res = gpu_bench(args)
     line = {
@@ -210,11 +413,12 @@ def main():
                                "eps=0.5, antiperiodic t, %d rotating sets" % ROTATE,
                    "lattice": "16x16x16x32", "link_storage": "18 reals", "bytes_per_site": BYTES_PER_SITE_FULL},
         "roofline": {"bound": "hbm", "achieved": round(res["gbs"], 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(res["gbs"] / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(res["gbs"] / HBM_PEAK_GBS, 4), "traffic": profiled_traffic()},
         "eo_schur": {k: round(v, 4) for k, v in res["eo_schur"].items()},
     }
-    if "solves" in res:
-        line["solves"] = res["solves"]
+    for k in ("solves", "adjoint", "batch", "serving"):
+        if k in res:
+            line[k] = res[k]
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

@@ -29,6 +29,13 @@
 #ifndef LYM_GRID_PER_CU
 #define LYM_GRID_PER_CU 0
 #endif
+// Full-lattice applies: odd-parity sites take each direction's backward hop
+// before its forward hop, so the two reads of a link U_mu(y) -- by y (forward)
+// and by y + mu (backward), sites of opposite parity -- happen at the same
+// step of the kernel instead of one step apart.
+#ifndef LYM_ALT_ORDER
+#define LYM_ALT_ORDER 0
+#endif
 #ifdef LYM_WAVES_PER_EU
 #define LYM_OCC __attribute__((amdgpu_waves_per_eu(LYM_WAVES_PER_EU, LYM_WAVES_PER_EU)))
 #else
@@ -95,8 +102,9 @@ __device__ __forceinline__ void pin(cplx<T> acc[12]) {
     asm volatile("" ::: "memory");
 }
 
-// one hop pair (forward and backward) in direction MU for site (p, site), rhs r
-template <int MU, int PR, int LK>
+// one hop pair (forward and backward, or backward first when BWD_FIRST) in
+// direction MU for site (p, site), rhs r
+template <int MU, int PR, int LK, bool BWD_FIRST = false>
 __device__ __forceinline__ void hop_pair(const DslashParams& a, int p, int site, int r, const int c[4],
                                          cplx<typename Prec<PR>::T> acc[12]) {
     typedef typename Prec<PR>::T T;
@@ -111,14 +119,22 @@ __device__ __forceinline__ void hop_pair(const DslashParams& a, int p, int site,
     const SpinorRef<PR> src{reinterpret_cast<const typename Prec<PR>::S*>(a.in[q]), a.in_sc[q]};
     cplx<T> psi[12];
     LinkMat<LK, T> L;
-    load_spinor<PR>(src, g.stride, a.nr, iu, r, psi);
-    load_link<PR, LK>(a.links, g.stride, p, MU, site, L);
-    hop_dir<MU, true, LK, T>(acc, psi, L, anti && c[3] == g.L[3] - 1);
-    pin(acc);
+    if constexpr (!BWD_FIRST) {
+        load_spinor<PR>(src, g.stride, a.nr, iu, r, psi);
+        load_link<PR, LK>(a.links, g.stride, p, MU, site, L);
+        hop_dir<MU, true, LK, T>(acc, psi, L, anti && c[3] == g.L[3] - 1);
+        pin(acc);
+    }
     load_spinor<PR>(src, g.stride, a.nr, id, r, psi);
     load_link<PR, LK>(a.links, g.stride, q, MU, id, L);
     hop_dir<MU, false, LK, T>(acc, psi, L, anti && c[3] == 0);
     pin(acc);
+    if constexpr (BWD_FIRST) {
+        load_spinor<PR>(src, g.stride, a.nr, iu, r, psi);
+        load_link<PR, LK>(a.links, g.stride, p, MU, site, L);
+        hop_dir<MU, true, LK, T>(acc, psi, L, anti && c[3] == g.L[3] - 1);
+        pin(acc);
+    }
 }
 
 template <int PR, int LK, int DIAG, bool INV>
@@ -131,10 +147,17 @@ __device__ __forceinline__ void dslash_site(const DslashParams& a, int p, int si
     cplx<T> acc[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[k] = mk((T)0, (T)0);
-    hop_pair<0, PR, LK>(a, p, site, r, c, acc);
-    hop_pair<1, PR, LK>(a, p, site, r, c, acc);
-    hop_pair<2, PR, LK>(a, p, site, r, c, acc);
-    hop_pair<3, PR, LK>(a, p, site, r, c, acc);
+    if (LYM_ALT_ORDER && a.parity < 0 && p == 1) {
+        hop_pair<0, PR, LK, true>(a, p, site, r, c, acc);
+        hop_pair<1, PR, LK, true>(a, p, site, r, c, acc);
+        hop_pair<2, PR, LK, true>(a, p, site, r, c, acc);
+        hop_pair<3, PR, LK, true>(a, p, site, r, c, acc);
+    } else {
+        hop_pair<0, PR, LK>(a, p, site, r, c, acc);
+        hop_pair<1, PR, LK>(a, p, site, r, c, acc);
+        hop_pair<2, PR, LK>(a, p, site, r, c, acc);
+        hop_pair<3, PR, LK>(a, p, site, r, c, acc);
+    }
     const T hk = (T)(-a.kappa * a.hop_sign);
     cplx<T> v[12];
     if constexpr (DIAG == 0) {

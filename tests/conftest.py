@@ -11,7 +11,7 @@ for p in (HERE, ROOT):
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (run on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X")
     config.addinivalue_line("markers", "slow: larger CPU cases")
 
 
