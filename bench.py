@@ -276,7 +276,7 @@ def batch_bench(lym):
     fp64_rate = len(kappas) / fp64_t
     log("bench: 32^3x64 FP64 EO CG one RHS at a time: %.3f s/solve, iterations %s" % (1 / fp64_rate, fp64_it))
     # batched mixed precision, 12 RHS per kappa
-    sb = lym.Solver(ctx, lym.SOLVER_PRECOND_MIXED_CG, precision=eps, max_steps=20000, sloppy=lym.PREC_HALF)
+    sb = lym.Solver(ctx, lym.SOLVER_PRECOND_MIXED_CG, precision=eps, max_steps=4000, sloppy=lym.PREC_HALF)
     op = lym.Operator(ctx, lym.OP_CLOVER, 1, kappas[0], CSW, True, gauge=g)
     sb.solve_batch(op, srcs[:2], sols[:2])  # warm: autotune, allocate
     t0 = time.perf_counter()
@@ -286,6 +286,8 @@ def batch_bench(lym):
         D = lym.Operator(ctx, lym.OP_CLOVER, 1, k, CSW, False, gauge=g)
         infos = sb.solve_batch(op, srcs, sols)
         ctx.synchronize()
+        log("bench: 32^3x64 batched kappa %.3f: iterations %s, converged %d of 12" % (
+            k, [int(i.steps) for i in infos], sum(int(i.converged) for i in infos)))
         t_k = time.perf_counter()
         for q in range(12):
             D.multiply(Dx, sols[q])

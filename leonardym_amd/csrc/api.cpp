@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -115,15 +116,26 @@ int op_prepare(lym_op* op, bool force_blocks) {
     lym::launch_clover_A(c->g, reinterpret_cast<const zd*>(g->B), op->kappa * op->csw, reinterpret_cast<zd*>(op->A),
                          c->stream);
     CHECK_LAUNCH();
+    // unpivoted LDL^H inverse first; the pivoted Gauss-Jordan redoes the parity
+    // if any block had a small pivot, and reports singular blocks
     int* flag = reinterpret_cast<int*>(c->red_result) + 32;
-    HIPCHECK(hipMemsetAsync(flag, 0, sizeof(int), c->stream));
-    lym::launch_clover_inv(c->g, reinterpret_cast<const zd*>(op->A), reinterpret_cast<zd*>(op->Ainv), flag,
-                           c->stream);
+    HIPCHECK(hipMemsetAsync(flag, 0, 2 * sizeof(int), c->stream));
+    lym::launch_clover_inv_ldl(c->g, reinterpret_cast<const zd*>(op->A), reinterpret_cast<zd*>(op->Ainv), flag + 1,
+                               c->stream);
     CHECK_LAUNCH();
-    int hflag = 0;
-    HIPCHECK(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    int hflag[2] = {0, 0};
+    HIPCHECK(hipMemcpyAsync(hflag, flag, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
-    if (hflag) return fail(LYM_ERR_SINGULAR, "%d clover blocks are singular", hflag);
+    // LYM_CLOVER_INV_PIVOTED=1 always takes the pivoted path (lets tests cover it)
+    const char* force = std::getenv("LYM_CLOVER_INV_PIVOTED");
+    if (hflag[1] || (force && force[0] == '1')) {
+        lym::launch_clover_inv(c->g, reinterpret_cast<const zd*>(op->A), reinterpret_cast<zd*>(op->Ainv), flag,
+                               c->stream);
+        CHECK_LAUNCH();
+        HIPCHECK(hipMemcpyAsync(hflag, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+    }
+    if (hflag[0]) return fail(LYM_ERR_SINGULAR, "%d clover blocks are singular", hflag[0]);
     op->diag_dirty = 0;
     op->sloppy_prec = 0;
     return LYM_OK;

@@ -213,6 +213,70 @@ __global__ void k_clover_A(Geom g, const zd* __restrict__ B, double kc, zd* __re
     A[i] = a;
 }
 
+// Inverse of a hermitian 6x6 block through A = L D L^H (L unit lower
+// triangular, D real) without pivoting: every index is static, so the whole
+// factorisation lives in registers.  Returns false when a pivot is below
+// 1e-12 of the largest diagonal entry (the caller then redoes the parity with
+// the pivoted kernel below).
+__device__ __forceinline__ bool herm6_inverse_ldl(const zd M[36], zd R[36]) {
+    zd L[36];
+    double d[6], di[6];
+    double scale = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) scale = fmax(scale, fabs(M[7 * i].re));
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        double dj = M[7 * j].re;
+#pragma unroll
+        for (int k = 0; k < j; ++k) dj -= (L[6 * j + k].re * L[6 * j + k].re + L[6 * j + k].im * L[6 * j + k].im) * d[k];
+        d[j] = dj;
+        ok = ok && fabs(dj) > 1e-12 * scale;
+        di[j] = 1.0 / dj;
+#pragma unroll
+        for (int i = j + 1; i < 6; ++i) {
+            zd s = M[6 * i + j];
+#pragma unroll
+            for (int k = 0; k < j; ++k) s = s - L[6 * i + k] * (d[k] * conj(L[6 * j + k]));
+            L[6 * i + j] = di[j] * s;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        zd y[6], x[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            zd v = mk(i == c ? 1.0 : 0.0, 0.0);
+#pragma unroll
+            for (int k = 0; k < i; ++k) v = v - L[6 * i + k] * y[k];
+            y[i] = v;
+        }
+#pragma unroll
+        for (int i = 5; i >= 0; --i) {
+            zd v = di[i] * y[i];
+#pragma unroll
+            for (int k = i + 1; k < 6; ++k) v = v - conj(L[6 * k + i]) * x[k];
+            x[i] = v;
+        }
+#pragma unroll
+        for (int i = 0; i < 6; ++i) R[6 * i + c] = x[i];
+    }
+    return ok;
+}
+
+__global__ __launch_bounds__(128) void k_clover_inv_ldl(Geom g, const zd* __restrict__ A, zd* __restrict__ Ainv,
+                                                       int* fallback) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.Vh) return;
+#pragma unroll 1
+    for (int blk = 0; blk < 2; ++blk) {
+        zd M[36], R[36];
+        load_herm6(A + (size_t)18 * blk * g.stride, g.stride, idx, M);
+        if (!herm6_inverse_ldl(M, R)) atomicAdd(fallback, 1);
+        store_herm6(Ainv + (size_t)18 * blk * g.stride, g.stride, idx, R);
+    }
+}
+
 // blockwise inverse on parity-0 sites: Gauss-Jordan with partial pivoting
 __global__ __launch_bounds__(64) void k_clover_inv(Geom g, const zd* __restrict__ A, zd* __restrict__ Ainv,
                                                    int* flag) {
@@ -286,6 +350,10 @@ void launch_clover_A(const Geom& g, const zd* B, double kc, zd* A, hipStream_t s
 
 void launch_clover_inv(const Geom& g, const zd* A, zd* Ainv, int* flag, hipStream_t st) {
     hipLaunchKernelGGL(k_clover_inv, dim3((g.Vh + 63) / 64), dim3(64), 0, st, g, A, Ainv, flag);
+}
+
+void launch_clover_inv_ldl(const Geom& g, const zd* A, zd* Ainv, int* fallback, hipStream_t st) {
+    hipLaunchKernelGGL(k_clover_inv_ldl, dim3((g.Vh + 127) / 128), dim3(128), 0, st, g, A, Ainv, fallback);
 }
 
 void launch_clover_apply(const Geom& g, const zd* blocks, const zd* in, zd* out, int gamma5, hipStream_t st) {

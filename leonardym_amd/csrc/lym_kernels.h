@@ -52,7 +52,8 @@ void launch_links_to_eo(const Geom& g, const void* links_aos, int real_links, in
 void launch_clover_B(const Geom& g, const void* links, int link_kind, zd* B, hipStream_t st);
 void launch_clover_A(const Geom& g, const zd* B, double kc, zd* A, hipStream_t st);
 // Ainv on parity 0 from A; returns via *flag (device int) nonzero when a block is singular
-void launch_clover_inv(const Geom& g, const zd* A, zd* Ainv, int* flag, hipStream_t st);
+void launch_clover_inv(const Geom& g, const zd* A, zd* Ainv, int* flag, hipStream_t st);  // pivoted; flag: singular count
+void launch_clover_inv_ldl(const Geom& g, const zd* A, zd* Ainv, int* fallback, hipStream_t st);  // unpivoted
 void launch_clover_apply(const Geom& g, const zd* blocks, const zd* in, zd* out, int gamma5, hipStream_t st);
 
 // ---------------------------------------------------------------- BLAS-1 (blas.hip)

@@ -141,6 +141,22 @@ def test_even_odd_operator(case, gamma5):
         op.multiply_add(case.spinor(), case.spinor(psi), case.spinor(psi), 1.0)
 
 
+@pytest.mark.parametrize("csw", [1.0, 6.0])
+def test_clover_inverse_paths_agree(case, csw, monkeypatch):
+    """R_ee^-1 from the unpivoted LDL^H kernel and from the pivoted
+    Gauss-Jordan fallback give the same EO operator as the oracle."""
+    psi = case.gauss()
+    fld = OracleField(case.dims, case.orc.U, kappa=case.kappa, csw=csw, need_clover=True)
+    ref = fld.apply(K_EO, psi)
+    got_ldl = case.apply(case.op(lym.OP_EO_CLOVER, csw=csw), psi)
+    monkeypatch.setenv("LYM_CLOVER_INV_PIVOTED", "1")
+    got_piv = case.apply(case.op(lym.OP_EO_CLOVER, csw=csw), psi)
+    monkeypatch.delenv("LYM_CLOVER_INV_PIVOTED")
+    tol = SITE_TOL if csw == 1.0 else 1e-11
+    assert rel_err_per_site(got_ldl, ref) < tol
+    assert rel_err_per_site(got_piv, ref) < tol
+
+
 def test_kappa_change_rebuilds_clover(case):
     psi = case.gauss()
     op = case.op(lym.OP_EO_CLOVER)
