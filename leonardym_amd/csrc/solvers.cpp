@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <algorithm>
 #include <map>
@@ -81,6 +82,10 @@ struct lym_solver {
     zd** bptr_dev;
     zd** bptr_host;
     zd* bzero;              // zero source / dummy destination for padded lanes
+    // captured iteration chunks (see queue_iterations)
+    uint64_t graph_key[4];
+    hipGraphExec_t graph_exec[4];
+    uint64_t graph_used[4], graph_tick;
 };
 
 namespace {
@@ -147,6 +152,91 @@ unsigned next_chunk(double r0, double r_now, unsigned done_iters, double target,
     return c;
 }
 
+// ---------------------------------------------------------------- iteration graphs
+// An iteration of every solver here is a fixed sequence of kernel launches
+// whose arguments do not change during a solve, and the host looks at the
+// state only between chunks of iterations.  A chunk of GRAPH_ITERS iterations
+// is therefore captured once into a HIP graph (kernel nodes only) and replayed;
+// a launch-bound solve (small lattice, low precision) then costs one graph
+// launch per chunk instead of ~16 kernel launches per iteration.  The capture
+// is keyed by everything its launches read from the host (pointers, kappa,
+// csw, flags, sizes); solvers keep the last four.  LYM_GRAPHS=0 queues the
+// kernels directly.
+constexpr unsigned GRAPH_ITERS = 8;
+
+struct Key {
+    uint64_t h = 1469598103934665603ull;
+    void bytes(const void* p, size_t n) {
+        const unsigned char* c = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
+    }
+    template <class T>
+    Key& operator<<(const T& v) {
+        bytes(&v, sizeof(v));
+        return *this;
+    }
+};
+
+// what the operator's launches capture
+void key_op(Key& k, const lym_op* op) {
+    k << op << op->ctx << op->kind << op->power << op->kappa << op->csw << op->gamma5 << op->gauge << op->gauge->U
+      << op->gauge->compression << op->A << op->Ainv << op->tmp << op->tmp2 << op->sloppy_prec << op->Us << op->As
+      << op->As_sc << op->Ainvs << op->Ainvs_sc << op->ctx->g.stride << op->ctx->g.Vh;
+}
+
+bool graphs_enabled() {
+    const char* e = std::getenv("LYM_GRAPHS");
+    return !(e && e[0] == '0');
+}
+
+void drop_graphs(lym_solver* s) {
+    for (int i = 0; i < 4; ++i) {
+        if (s->graph_exec[i]) (void)hipGraphExecDestroy(s->graph_exec[i]);
+        s->graph_exec[i] = nullptr;
+        s->graph_key[i] = 0;
+    }
+}
+
+// queue k iterations of body() (at least k: whole chunks when replaying a graph;
+// iterations past the solver's stop are no-ops by its device state)
+template <class F>
+int queue_iterations(lym_solver* s, uint64_t key, unsigned k, F&& body, unsigned* queued) {
+    hipStream_t st = s->ctx->stream;
+    if (!graphs_enabled()) {
+        for (unsigned i = 0; i < k; ++i) body();
+        *queued += k;
+        return LYM_OK;
+    }
+    int slot = -1;
+    for (int i = 0; i < 4; ++i)
+        if (s->graph_exec[i] && s->graph_key[i] == key) slot = i;
+    if (slot < 0) {
+        slot = 0;
+        for (int i = 1; i < 4; ++i)
+            if (s->graph_used[i] < s->graph_used[slot]) slot = i;
+        if (s->graph_exec[slot]) (void)hipGraphExecDestroy(s->graph_exec[slot]);
+        s->graph_exec[slot] = nullptr;
+        s->graph_key[slot] = 0;
+        HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (unsigned i = 0; i < GRAPH_ITERS; ++i) body();
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(st, &graph);
+        if (e != hipSuccess || !graph) return fail(LYM_ERR_DEVICE, "graph capture: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&s->graph_exec[slot], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            s->graph_exec[slot] = nullptr;
+            return fail(LYM_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(e));
+        }
+        s->graph_key[slot] = key;
+    }
+    s->graph_used[slot] = ++s->graph_tick;
+    const unsigned n = (k + GRAPH_ITERS - 1) / GRAPH_ITERS;
+    for (unsigned i = 0; i < n; ++i) HIPCHECK(hipGraphLaunch(s->graph_exec[slot], st));
+    *queued += n * GRAPH_ITERS;
+    return LYM_OK;
+}
+
 double read_norm2(lym_ctx* c, const zd* v, long n) {
     lym::launch_norm2(v, n, c->red_partials, c->red_result, c->stream);
     double out = 0;
@@ -172,15 +262,20 @@ int run_cg(lym_solver* s, const LinOp& L, const zd* b, zd* xout, const zd* x0, l
     HIPCHECK(hipStreamSynchronize(st));
     const double r0 = H->rr;
     unsigned queued = 0;
+    Key key;
+    key << 1 << s << n << b;
+    key_op(key, L.op);
+    key << L.half << L.gamma5 << L.power << L.scratch;
+    auto body = [&] {
+        L.apply(Ap, p, &S->done);
+        lym::launch_cg_alpha(S, p, Ap, n, c->red_partials, st);
+        lym::launch_cg_update(S, x, r, p, Ap, n, c->red_partials, st);
+        lym::launch_cg_p(S, p, r, n, st);
+    };
     while (!H->done && queued <= s->max_steps) {
         unsigned k = next_chunk(r0, H->rr, (unsigned)H->step, s->precision, s->max_steps + 1 - queued);
-        for (unsigned i = 0; i < k; ++i) {
-            L.apply(Ap, p, &S->done);
-            lym::launch_cg_alpha(S, p, Ap, n, c->red_partials, st);
-            lym::launch_cg_update(S, x, r, p, Ap, n, c->red_partials, st);
-            lym::launch_cg_p(S, p, r, n, st);
-        }
-        queued += k;
+        int rc = queue_iterations(s, key.h, k, body, &queued);
+        if (rc) return rc;
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(H, S, sizeof(lym::CGState), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
@@ -219,19 +314,24 @@ int run_bicgstab(lym_solver* s, const LinOp& L, const zd* b, zd* xout, const zd*
     const double r0 = read_norm2(c, r, n);
     double rnow = r0;
     unsigned queued = 0;
+    Key key;
+    key << 2 << s << n << b;
+    key_op(key, L.op);
+    key << L.half << L.gamma5 << L.power << L.scratch;
+    auto body = [&] {
+        lym::launch_bicg_rho(S, rh, r, n, c->red_partials, st);
+        lym::launch_bicg_p(S, p, r, nu, n, st);
+        L.apply(nu, p, &S->done);
+        lym::launch_bicg_alpha(S, rh, nu, n, c->red_partials, st);
+        lym::launch_bicg_s(S, sv, r, nu, n, st);
+        L.apply(t, sv, &S->done);
+        lym::launch_bicg_omega(S, t, sv, n, c->red_partials, st);
+        lym::launch_bicg_update(S, x, r, p, sv, t, b, n, c->red_partials, st);
+    };
     while (!H->done && queued <= s->max_steps) {
         unsigned k = next_chunk(r0, rnow, (unsigned)H->step, s->precision, s->max_steps + 1 - queued);
-        for (unsigned i = 0; i < k; ++i) {
-            lym::launch_bicg_rho(S, rh, r, n, c->red_partials, st);
-            lym::launch_bicg_p(S, p, r, nu, n, st);
-            L.apply(nu, p, &S->done);
-            lym::launch_bicg_alpha(S, rh, nu, n, c->red_partials, st);
-            lym::launch_bicg_s(S, sv, r, nu, n, st);
-            L.apply(t, sv, &S->done);
-            lym::launch_bicg_omega(S, t, sv, n, c->red_partials, st);
-            lym::launch_bicg_update(S, x, r, p, sv, t, b, n, c->red_partials, st);
-        }
-        queued += k;
+        int rc = queue_iterations(s, key.h, k, body, &queued);
+        if (rc) return rc;
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(H, S, sizeof(lym::BiCGState), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
@@ -325,18 +425,24 @@ int run_mixed_cg(lym_solver* s, const LinOp& L, const SloppyOp& SL, const zd* b,
     HIPCHECK(hipStreamSynchronize(st));
     const double r0 = H->rr_s;
     unsigned queued = 0;
+    Key key;
+    key << 3 << s << nsites << b << prec << s->xs << s->rs << p << Ap << p_sc << Ap_sc;
+    key_op(key, L.op);
+    key << L.half << L.gamma5 << L.power << L.scratch << SL.half << SL.gamma5 << SL.power << SL.mid << SL.mid_sc
+        << SL.tmp_e << SL.tmp_sc;
+    auto body = [&] {
+        SL.apply(Ap, Ap_sc, p, p_sc, &S->done);
+        lym::launch_mcg_alpha(prec, S, p, p_sc, Ap, Ap_sc, nsites, c->g.stride, c->red_partials, st);
+        lym::launch_mcg_update(prec, S, s->xs, s->rs, p, p_sc, Ap, Ap_sc, nsites, c->g.stride, c->red_partials, st);
+        lym::launch_rel_accum(S, x, s->xs, nsites, c->g.stride, st);
+        L.apply(Ax, x, &S->skip_rel);
+        lym::launch_rel_resid(S, r, s->rs, b, Ax, nsites, c->g.stride, c->red_partials, st);
+        lym::launch_mcg_p(prec, S, p, p_sc, s->rs, nsites, c->g.stride, st);
+    };
     while (!H->done && queued <= s->max_steps) {
         unsigned k = next_chunk(r0, H->rr_s, (unsigned)H->step, s->precision, s->max_steps + 1 - queued);
-        for (unsigned i = 0; i < k; ++i) {
-            SL.apply(Ap, Ap_sc, p, p_sc, &S->done);
-            lym::launch_mcg_alpha(prec, S, p, p_sc, Ap, Ap_sc, nsites, c->g.stride, c->red_partials, st);
-            lym::launch_mcg_update(prec, S, s->xs, s->rs, p, p_sc, Ap, Ap_sc, nsites, c->g.stride, c->red_partials, st);
-            lym::launch_rel_accum(S, x, s->xs, nsites, c->g.stride, st);
-            L.apply(Ax, x, &S->skip_rel);
-            lym::launch_rel_resid(S, r, s->rs, b, Ax, nsites, c->g.stride, c->red_partials, st);
-            lym::launch_mcg_p(prec, S, p, p_sc, s->rs, nsites, c->g.stride, st);
-        }
-        queued += k;
+        int rc = queue_iterations(s, key.h, k, body, &queued);
+        if (rc) return rc;
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(H, S, sizeof(lym::MCGState), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
@@ -574,19 +680,24 @@ int run_batch(lym_solver* s, lym_op* op, bool eo, bool half, int nr, int thr, in
     HIPCHECK(hipStreamSynchronize(st));
     unsigned queued = 0;
     // steps are queued in chunks; those queued after the last RHS stopped exit at once
+    Key key;
+    key << 4 << s << nr << thr << prec << eo << half << g5 << pw << x << r << b << Ax << mid << tmp << a.xs << a.rs
+        << a.p << a.Ap << a.p_sc << a.Ap_sc << a.nsites << HS.mid << HS.tmp << HS.mid_sc << HS.tmp_sc;
+    key_op(key, op);
+    auto body = [&] {
+        HS.apply(prec, a.Ap, a.Ap_sc, a.p, a.p_sc, &Sdev->all_done);
+        lym::launch_bcg_step(1, prec, nr, Sdev, a, st);
+        lym::launch_bcg_step(2, prec, nr, Sdev, a, st);
+        lym::launch_bcg_step(3, prec, nr, Sdev, a, st);
+        H64.apply(0, Ax, nullptr, x, nullptr, &Sdev->skip_rel);
+        lym::launch_bcg_step(4, prec, nr, Sdev, a, st);
+        lym::launch_bcg_step(5, prec, nr, Sdev, a, st);
+    };
     while (!Sh->all_done && queued <= s->max_steps) {
         unsigned k = queued == 0 ? 8 : 16;
         if (k > s->max_steps + 1 - queued) k = s->max_steps + 1 - queued;
-        for (unsigned i = 0; i < k; ++i) {
-            HS.apply(prec, a.Ap, a.Ap_sc, a.p, a.p_sc, &Sdev->all_done);
-            lym::launch_bcg_step(1, prec, nr, Sdev, a, st);
-            lym::launch_bcg_step(2, prec, nr, Sdev, a, st);
-            lym::launch_bcg_step(3, prec, nr, Sdev, a, st);
-            H64.apply(0, Ax, nullptr, x, nullptr, &Sdev->skip_rel);
-            lym::launch_bcg_step(4, prec, nr, Sdev, a, st);
-            lym::launch_bcg_step(5, prec, nr, Sdev, a, st);
-        }
-        queued += k;
+        int rc = queue_iterations(s, key.h, k, body, &queued);
+        if (rc) return rc;
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(Sh, Sdev, sizeof(lym::BState), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
@@ -712,6 +823,14 @@ void lym_solver_destroy(lym_solver* s) {
     if (s->xs) (void)hipFree(s->xs);
     if (s->rs) (void)hipFree(s->rs);
     if (s->host_state) (void)hipHostFree(s->host_state);
+    free_batch(s);
+    drop_graphs(s);
+    if (s->bpartials) (void)hipFree(s->bpartials);
+    if (s->bstate) (void)hipFree(s->bstate);
+    if (s->bhost) (void)hipHostFree(s->bhost);
+    if (s->bptr_dev) (void)hipFree(s->bptr_dev);
+    if (s->bptr_host) (void)hipHostFree(s->bptr_host);
+    if (s->bzero) (void)hipFree(s->bzero);
     delete s;
 }
 

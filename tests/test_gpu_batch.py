@@ -113,3 +113,16 @@ def test_batch_edge_cases(case):
         solver.solve_batch(case.op(lym.OP_CLOVER), [case.spinor(b)], [case.spinor()])
     with pytest.raises(lym.LymError):
         lym.Solver(case.ctx, lym.SOLVER_MIXED_CG).set_rhs_block(5)
+
+
+def test_batch_graph_replay_equals_eager(case, monkeypatch):
+    srcs = [case.gauss() for _ in range(3)]
+    eps = [(1e-11 * np.linalg.norm(b)) ** 2 for b in srcs]
+    op = case.op(lym.OP_CLOVER)
+    monkeypatch.setenv("LYM_GRAPHS", "0")
+    xe, ie, _ = solve_batch(case, op, lym.SOLVER_PRECOND_MIXED_CG, srcs, 4, eps)
+    monkeypatch.delenv("LYM_GRAPHS")
+    xg, ig, _ = solve_batch(case, op, lym.SOLVER_PRECOND_MIXED_CG, srcs, 4, eps)
+    for q in range(3):
+        assert ie[q].steps == ig[q].steps and ie[q].converged and ig[q].converged
+        assert np.array_equal(xe[q], xg[q])
