@@ -36,6 +36,13 @@
 #ifndef LYM_ALT_ORDER
 #define LYM_ALT_ORDER 0
 #endif
+// Hops whose loads may be in flight together: a register barrier (pin) after
+// every hop (1), every direction pair (2) or every two directions (4).
+// More hops in flight raise memory-level parallelism per wave at the cost of
+// registers, which pays when the lattice gives few waves per SIMD anyway.
+#ifndef LYM_PIN_HOPS
+#define LYM_PIN_HOPS 1
+#endif
 #ifdef LYM_WAVES_PER_EU
 #define LYM_OCC __attribute__((amdgpu_waves_per_eu(LYM_WAVES_PER_EU, LYM_WAVES_PER_EU)))
 #else
@@ -119,22 +126,28 @@ __device__ __forceinline__ void hop_pair(const DslashParams& a, int p, int site,
     const SpinorRef<PR> src{reinterpret_cast<const typename Prec<PR>::S*>(a.in[q]), a.in_sc[q]};
     cplx<T> psi[12];
     LinkMat<LK, T> L;
+    constexpr bool pin_mid = LYM_PIN_HOPS == 1;
+    constexpr bool pin_end = LYM_PIN_HOPS <= 2 || (MU & 1);
     if constexpr (!BWD_FIRST) {
         load_spinor<PR>(src, g.stride, a.nr, iu, r, psi);
         load_link<PR, LK>(a.links, g.stride, p, MU, site, L);
         hop_dir<MU, true, LK, T>(acc, psi, L, anti && c[3] == g.L[3] - 1);
-        pin(acc);
+        if constexpr (pin_mid) pin(acc);
     }
-    load_spinor<PR>(src, g.stride, a.nr, id, r, psi);
-    load_link<PR, LK>(a.links, g.stride, q, MU, id, L);
-    hop_dir<MU, false, LK, T>(acc, psi, L, anti && c[3] == 0);
-    pin(acc);
+    {
+        cplx<T> psi2[12];
+        LinkMat<LK, T> L2;
+        load_spinor<PR>(src, g.stride, a.nr, id, r, psi2);
+        load_link<PR, LK>(a.links, g.stride, q, MU, id, L2);
+        hop_dir<MU, false, LK, T>(acc, psi2, L2, anti && c[3] == 0);
+    }
     if constexpr (BWD_FIRST) {
+        if constexpr (pin_mid) pin(acc);
         load_spinor<PR>(src, g.stride, a.nr, iu, r, psi);
         load_link<PR, LK>(a.links, g.stride, p, MU, site, L);
         hop_dir<MU, true, LK, T>(acc, psi, L, anti && c[3] == g.L[3] - 1);
-        pin(acc);
     }
+    if constexpr (pin_end) pin(acc);
 }
 
 template <int PR, int LK, int DIAG, bool INV>
