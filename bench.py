@@ -128,6 +128,28 @@ def gpu_bench(args):
     log("bench: EO Schur apply %.4f ms  %.1f GB/s" % (ms, res["eo_schur"]["gbs"]))
 
     secs = set(args.sections.split(","))
+    # the same apply with 12-real links (third row rebuilt): its own algorithmic
+    # bytes (1344 B/site) and the 18-real-equivalent rate
+    sets12 = []
+    for r in range(ROTATE):
+        U = fold_bc(DIMS, lym.gen_su3_links(DIMS, EPS, GAUGE_SEED + r))
+        g12 = lym.Gauge(ctx, U, folded=True, storage=lym.LINKS_RECONSTRUCT)
+        sets12.append((g12, lym.Operator(ctx, lym.OP_CLOVER, 1, KAPPA, CSW, True, gauge=g12), sets[r][2], sets[r][3]))
+    for i in range(args.warmup):
+        g12, op12, src, out = sets12[i % ROTATE]
+        op12.multiply(out, src)
+    ctx.synchronize()
+    ctx.timer_start()
+    for i in range(args.steps):
+        g12, op12, src, out = sets12[i % ROTATE]
+        op12.multiply(out, src)
+    ms = ctx.timer_stop() / args.steps
+    res["reconstruct12"] = {"ms": ms, "gbs": 1344 * V / (ms * 1e-3) / 1e9,
+                            "gbs_18_real_effective": BYTES_PER_SITE_FULL * V / (ms * 1e-3) / 1e9}
+    log("bench: 12-real links clover apply %.4f ms  %.1f GB/s (18-real effective %.1f)" % (
+        ms, res["reconstruct12"]["gbs"], res["reconstruct12"]["gbs_18_real_effective"]))
+    del sets12
+
     if not args.no_solve:
         if "solve" in secs:
             res["solves"] = solve_bench(lym, ctx, sets, V)
@@ -417,6 +439,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(res["gbs"], 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(res["gbs"] / HBM_PEAK_GBS, 4), "traffic": profiled_traffic()},
         "eo_schur": {k: round(v, 4) for k, v in res["eo_schur"].items()},
+        "reconstruct12": {k: round(v, 4) for k, v in res["reconstruct12"].items()},
     }
     for k in ("solves", "adjoint", "batch", "serving"):
         if k in res:

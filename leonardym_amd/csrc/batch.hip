@@ -91,7 +91,7 @@ __device__ __forceinline__ void bs_store(typename Prec<PR>::S* d, float* sc, int
         float m = 0.f;
 #pragma unroll
         for (int k = 0; k < 12; ++k) m = fmaxf(m, fmaxf(fabsf(s[k].re), fabsf(s[k].im)));
-        const float inv = m > 0.f ? 1.0f / m : 0.f;
+        const float inv = fp16_site_inv(m);
 #pragma unroll
         for (int k = 0; k < 12; ++k) d[bidx(stride, NR, j, k, r)] = from_c<PR>(inv * s[k]);
         sc[j * NR + r] = m;

@@ -141,6 +141,19 @@ __device__ __forceinline__ void load_spinor(SpinorRef<PR> f, int stride, int nr,
     }
 }
 
+// Reciprocal of a per-site FP16 scale m (the site's largest |component|).  A
+// site whose m is below the smallest normal float is stored as zero: 1/m of a
+// subnormal overflows to inf, and inf * 0 = NaN would poison the vector.  Such
+// sites appear where a localised vector (a point source after a few hops)
+// decays below 1e-38; they are negligible in FP32 arithmetic anyway.
+__device__ __forceinline__ float fp16_site_inv(float& m) {
+    if (!(m >= 1.17549435e-38f)) {
+        m = 0.f;
+        return 0.f;
+    }
+    return 1.0f / m;
+}
+
 template <int PR>
 __device__ __forceinline__ void store_spinor(SpinorOut<PR> f, int stride, int nr, int site, int r,
                                              const cplx<typename Prec<PR>::T> v[12]) {
@@ -150,7 +163,7 @@ __device__ __forceinline__ void store_spinor(SpinorOut<PR> f, int stride, int nr
         float m = 0.f;
 #pragma unroll
         for (int k = 0; k < 12; ++k) m = fmaxf(m, fmaxf(fabsf(v[k].re), fabsf(v[k].im)));
-        const float inv = m > 0.f ? 1.0f / m : 0.f;
+        const float inv = fp16_site_inv(m);
 #pragma unroll
         for (int k = 0; k < 12; ++k) st_nt_raw(f.base + k * ks + o, from_c<PR>(inv * v[k]));
         f.scale[o] = m;

@@ -99,7 +99,7 @@ __device__ __forceinline__ void sv_store(const SVec<PR>& v, int stride, long j, 
         float m = 0.f;
 #pragma unroll
         for (int k = 0; k < 12; ++k) m = fmaxf(m, fmaxf(fabsf(s[k].re), fabsf(s[k].im)));
-        const float inv = m > 0.f ? 1.0f / m : 0.f;
+        const float inv = fp16_site_inv(m);
 #pragma unroll
         for (int k = 0; k < 12; ++k) b[(long)k * stride] = from_c<PR>(inv * s[k]);
         v.sc[j] = m;
@@ -159,9 +159,9 @@ __global__ void k_convert_blocks(const zd* src, typename Prec<PR>::S* dst, float
                 zd z = s[(long)k * stride];
                 m = fmaxf(m, fmaxf(fabsf((float)z.re), fabsf((float)z.im)));
             }
-            sc[j] = m;
         }
-        const float inv = m > 0.f ? 1.0f / m : 0.f;
+        const float inv = fp16_site_inv(m);
+        if constexpr (PR == 2) sc[j] = m;
         for (int k = 0; k < 36; ++k) {
             zd z = s[(long)k * stride];
             d[(long)k * stride] = from_c<PR>(mk((typename Prec<PR>::T)(inv * z.re), (typename Prec<PR>::T)(inv * z.im)));

@@ -126,3 +126,31 @@ def test_batch_graph_replay_equals_eager(case, monkeypatch):
     for q in range(3):
         assert ie[q].steps == ig[q].steps and ie[q].converged and ig[q].converged
         assert np.array_equal(xe[q], xg[q])
+
+
+def test_point_sources_16x16x16x32():
+    """Point sources (a localised source decays below the smallest normal
+    float within a few dozen hops): mixed CG with FP16 storage, alone and
+    batched, reaches true residual 1e-10 and the FP64 solution."""
+    if lym.device_count() < 1:
+        pytest.fail("no HIP device visible")
+    case = Case((16, 16, 16, 32), kappa=0.13, seed=31)
+    site = (3, 5, 7, 9)
+    srcs = [lym.point_source(case.dims, site, sc // 3, sc % 3) for sc in (0, 4, 11)]
+    op = case.op(lym.OP_CLOVER)
+    eps = 1e-22
+    s64 = lym.Solver(case.ctx, lym.SOLVER_PRECOND_CG, precision=eps, max_steps=4000)
+    for sloppy in (lym.PREC_HALF, lym.PREC_SINGLE):
+        xs, infos, _ = solve_batch(case, op, lym.SOLVER_PRECOND_MIXED_CG, srcs, 2, None, sloppy, eps=eps, max_steps=4000)
+        single = lym.Solver(case.ctx, lym.SOLVER_PRECOND_MIXED_CG, precision=eps, max_steps=4000, sloppy=sloppy)
+        for q, b in enumerate(srcs):
+            X1, X64 = case.spinor(), case.spinor()
+            i1 = single.solve(op, case.spinor(b), X1)
+            i64 = s64.solve(op, case.spinor(b), X64)
+            print("point source %d sloppy %d: batch %d it, alone %d it, FP64 %d it" % (
+                q, sloppy, infos[q].steps, i1.steps, i64.steps))
+            assert infos[q].converged and i1.converged and i64.converged
+            x64 = X64.download()
+            assert true_residual(case, xs[q], b) <= 1e-10
+            assert np.linalg.norm((xs[q] - x64).ravel()) / np.linalg.norm(x64.ravel()) < 1e-9
+            assert np.linalg.norm((X1.download() - x64).ravel()) / np.linalg.norm(x64.ravel()) < 1e-9
