@@ -238,16 +238,14 @@ __global__ __launch_bounds__(RED_BLOCKS) void k_bcg_update_final(BState* s, cons
             if (s->done[q]) continue;
             const double rn = out[q];
             if (rn > s->maxrr[q]) s->maxrr[q] = rn;
-            if (rn < s->delta2 * s->maxrr[q] || rn < s->eps[q]) {
-                s->rel[q] = 1;
-                any = 1;
-                continue;
-            }
+            // due reliable updates run at the next reliable-update slot
+            if (rn < s->delta2 * s->maxrr[q] || rn < s->eps[q]) s->rel[q] = 1;
             s->beta[q] = rn / s->rr_s[q];
             s->rr_s[q] = rn;
             s->step[q] += 1;
             if (s->step[q] >= s->max_steps) s->done[q] = 1;
         }
+        for (int q = 0; q < NR; ++q) any = any || (s->rel[q] && !s->done[q]);
         s->any_rel = any;
         s->skip_rel = any ? 0 : 1;
         int all = 1;
@@ -318,16 +316,13 @@ __global__ __launch_bounds__(RED_BLOCKS) void k_brel_final(BState* s, const doub
             s->rr_true[q] = rt;
             s->nrel[q] += 1;
             s->rel[q] = 0;
-            s->step[q] += 1;
             if (rt < s->eps[q]) {
                 s->done[q] = 1;
                 s->conv[q] = 1;
                 continue;
             }
-            s->beta[q] = rt / s->rr_s[q];
             s->rr_s[q] = rt;
             s->maxrr[q] = rt;
-            if (s->step[q] >= s->max_steps) s->done[q] = 1;
         }
         s->any_rel = 0;
         s->skip_rel = 1;

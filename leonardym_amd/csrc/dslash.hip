@@ -43,6 +43,11 @@
 #ifndef LYM_PIN_HOPS
 #define LYM_PIN_HOPS 1
 #endif
+// the same for FP32/FP16 storage, whose hops need far fewer registers
+// (8: no barrier between hops at all)
+#ifndef LYM_PIN_HOPS_LOWP
+#define LYM_PIN_HOPS_LOWP 1
+#endif
 #ifdef LYM_WAVES_PER_EU
 #define LYM_OCC __attribute__((amdgpu_waves_per_eu(LYM_WAVES_PER_EU, LYM_WAVES_PER_EU)))
 #else
@@ -126,8 +131,9 @@ __device__ __forceinline__ void hop_pair(const DslashParams& a, int p, int site,
     const SpinorRef<PR> src{reinterpret_cast<const typename Prec<PR>::S*>(a.in[q]), a.in_sc[q]};
     cplx<T> psi[12];
     LinkMat<LK, T> L;
-    constexpr bool pin_mid = LYM_PIN_HOPS == 1;
-    constexpr bool pin_end = LYM_PIN_HOPS <= 2 || (MU & 1);
+    constexpr int PH = PR == 0 ? LYM_PIN_HOPS : LYM_PIN_HOPS_LOWP;
+    constexpr bool pin_mid = PH == 1;
+    constexpr bool pin_end = PH <= 2 || (PH == 4 && (MU & 1));
     if constexpr (!BWD_FIRST) {
         load_spinor<PR>(src, g.stride, a.nr, iu, r, psi);
         load_link<PR, LK>(a.links, g.stride, p, MU, site, L);

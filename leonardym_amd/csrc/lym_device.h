@@ -111,6 +111,95 @@ __device__ __forceinline__ typename Prec<PR>::S from_c(cplx<typename Prec<PR>::T
     }
 }
 
+// Field access through buffer instructions: a wave-uniform 128-bit resource
+// (base address in SGPRs) per component stream plus one 32-bit per-lane byte
+// offset shared by all streams of a site index.  Plain pointer arithmetic
+// leaves a 64-bit VGPR address per stream (v_lshl_add_u64 for each of the ~230
+// loads of a stencil site), which costs registers and occupancy.  Each
+// component stream is < 4 GiB.  LYM_BUFFER_LOADS=0 keeps plain loads.
+#ifndef LYM_BUFFER_LOADS
+#define LYM_BUFFER_LOADS 1
+#endif
+
+template <class E>
+__device__ __forceinline__ const E* lane_at(const E* ubase, unsigned lane_elems) {
+    return reinterpret_cast<const E*>(reinterpret_cast<const char*>(ubase) + lane_elems * (unsigned)sizeof(E));
+}
+template <class E>
+__device__ __forceinline__ E* lane_at(E* ubase, unsigned lane_elems) {
+    return reinterpret_cast<E*>(reinterpret_cast<char*>(ubase) + lane_elems * (unsigned)sizeof(E));
+}
+
+#if LYM_BUFFER_LOADS
+// raw buffer resource for gfx9-family (gfx950): stride 0, 4 GiB range
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_of(const void* base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, 0x7fffffff, 0x00020000);
+}
+// cache policy bits of the buffer instructions: 2 = non-temporal
+template <class E, int AUX = 0>
+__device__ __forceinline__ E buf_ld(const E* ubase, unsigned lane_elems) {
+    const __amdgpu_buffer_rsrc_t r = rsrc_of(ubase);
+    const unsigned off = lane_elems * (unsigned)sizeof(E);
+    E e;
+    if constexpr (sizeof(E) == 16) {
+        auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, AUX);
+        __builtin_memcpy(&e, &v, 16);
+    } else if constexpr (sizeof(E) == 8) {
+        auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, AUX);
+        __builtin_memcpy(&e, &v, 8);
+    } else if constexpr (sizeof(E) == 4) {
+        auto v = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, AUX);
+        __builtin_memcpy(&e, &v, 4);
+    } else {
+        static_assert(sizeof(E) == 2, "element size");
+        auto v = __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, AUX);
+        __builtin_memcpy(&e, &v, 2);
+    }
+    return e;
+}
+template <class E, int AUX = 0>
+__device__ __forceinline__ void buf_st(E* ubase, unsigned lane_elems, E e) {
+    const __amdgpu_buffer_rsrc_t r = rsrc_of(ubase);
+    const unsigned off = lane_elems * (unsigned)sizeof(E);
+    if constexpr (sizeof(E) == 16) {
+        typedef unsigned v4u __attribute__((ext_vector_type(4)));
+        v4u v;
+        __builtin_memcpy(&v, &e, 16);
+        __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, AUX);
+    } else if constexpr (sizeof(E) == 8) {
+        typedef unsigned v2u __attribute__((ext_vector_type(2)));
+        v2u v;
+        __builtin_memcpy(&v, &e, 8);
+        __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, AUX);
+    } else {
+        static_assert(sizeof(E) == 4, "element size");
+        unsigned v;
+        __builtin_memcpy(&v, &e, 4);
+        __builtin_amdgcn_raw_buffer_store_b32(v, r, off, 0, AUX);
+    }
+}
+#endif
+
+// element (ubase)[lane] through the chosen path; NT = non-temporal
+template <class E, bool NT = false>
+__device__ __forceinline__ E fld(const E* ubase, unsigned lane) {
+#if LYM_BUFFER_LOADS
+    return buf_ld<E, NT ? 2 : 0>(ubase, lane);
+#else
+    if constexpr (NT && sizeof(E) >= 4) return ld_nt_raw(lane_at(ubase, lane));
+    else return *lane_at(ubase, lane);
+#endif
+}
+template <class E, bool NT = true>
+__device__ __forceinline__ void fst(E* ubase, unsigned lane, E e) {
+#if LYM_BUFFER_LOADS
+    buf_st<E, NT ? 2 : 0>(ubase, lane, e);
+#else
+    if constexpr (NT && sizeof(E) >= 4) st_nt_raw(lane_at(ubase, lane), e);
+    else *lane_at(ubase, lane) = e;
+#endif
+}
+
 // ---------------------------------------------------------------- spinors
 // One parity of a (multi-RHS) spinor field.
 template <int PR>
@@ -127,15 +216,15 @@ struct SpinorOut {
 template <int PR, bool NT = false>
 __device__ __forceinline__ void load_spinor(SpinorRef<PR> f, int stride, int nr, int site, int r,
                                             cplx<typename Prec<PR>::T> s[12]) {
-    const size_t o = (size_t)site * nr + r;
+    const unsigned o = (unsigned)site * (unsigned)nr + (unsigned)r;
     const size_t ks = (size_t)stride * nr;
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
-        typename Prec<PR>::S e = NT ? ld_nt_raw(f.base + k * ks + o) : f.base[k * ks + o];
+        typename Prec<PR>::S e = fld<typename Prec<PR>::S, NT>(f.base + k * ks, o);
         s[k] = to_c<PR>(e);
     }
     if constexpr (PR == 2) {
-        const float sc = f.scale[o];
+        const float sc = fld<float>(f.scale, o);
 #pragma unroll
         for (int k = 0; k < 12; ++k) s[k] = sc * s[k];
     }
@@ -157,7 +246,7 @@ __device__ __forceinline__ float fp16_site_inv(float& m) {
 template <int PR>
 __device__ __forceinline__ void store_spinor(SpinorOut<PR> f, int stride, int nr, int site, int r,
                                              const cplx<typename Prec<PR>::T> v[12]) {
-    const size_t o = (size_t)site * nr + r;
+    const unsigned o = (unsigned)site * (unsigned)nr + (unsigned)r;
     const size_t ks = (size_t)stride * nr;
     if constexpr (PR == 2) {
         float m = 0.f;
@@ -165,11 +254,11 @@ __device__ __forceinline__ void store_spinor(SpinorOut<PR> f, int stride, int nr
         for (int k = 0; k < 12; ++k) m = fmaxf(m, fmaxf(fabsf(v[k].re), fabsf(v[k].im)));
         const float inv = fp16_site_inv(m);
 #pragma unroll
-        for (int k = 0; k < 12; ++k) st_nt_raw(f.base + k * ks + o, from_c<PR>(inv * v[k]));
-        f.scale[o] = m;
+        for (int k = 0; k < 12; ++k) fst(f.base + k * ks, o, from_c<PR>(inv * v[k]));
+        fst<float, false>(f.scale, o, m);
     } else {
 #pragma unroll
-        for (int k = 0; k < 12; ++k) st_nt_raw(f.base + k * ks + o, v[k]);
+        for (int k = 0; k < 12; ++k) fst(f.base + k * ks, o, v[k]);
     }
 }
 
@@ -194,14 +283,10 @@ __device__ __forceinline__ void load_link(const void* base, int stride, int p, i
     if constexpr (LK == 0 || LK == 1) {
         constexpr int NC = (LK == 0) ? 9 : 6;
         const typename Prec<PR>::S* b =
-            reinterpret_cast<const typename Prec<PR>::S*>(base) + (size_t)((p * 4 + mu) * NC) * stride + idx;
+            reinterpret_cast<const typename Prec<PR>::S*>(base) + (size_t)((p * 4 + mu) * NC) * stride;
 #pragma unroll
         for (int k = 0; k < NC; ++k) {
-#if LYM_NT_LINKS
-            L.m[k] = to_c<PR>(ld_nt_raw(b + (size_t)k * stride));
-#else
-            L.m[k] = to_c<PR>(b[(size_t)k * stride]);
-#endif
+            L.m[k] = to_c<PR>(fld<typename Prec<PR>::S, LYM_NT_LINKS != 0>(b + (size_t)k * stride, (unsigned)idx));
         }
         if constexpr (LK == 1) {
             L.m[6] = conj(L.m[1] * L.m[5] - L.m[2] * L.m[4]);
@@ -211,9 +296,9 @@ __device__ __forceinline__ void load_link(const void* base, int stride, int p, i
     } else {
         constexpr int NC = (LK == 2) ? 9 : 6;
         const typename Prec<PR>::R* b =
-            reinterpret_cast<const typename Prec<PR>::R*>(base) + (size_t)((p * 4 + mu) * NC) * stride + idx;
+            reinterpret_cast<const typename Prec<PR>::R*>(base) + (size_t)((p * 4 + mu) * NC) * stride;
 #pragma unroll
-        for (int k = 0; k < NC; ++k) L.m[k] = (T)b[(size_t)k * stride];
+        for (int k = 0; k < NC; ++k) L.m[k] = (T)fld<typename Prec<PR>::R, LYM_NT_LINKS != 0>(b + (size_t)k * stride, (unsigned)idx);
         if constexpr (LK == 3) {
             L.m[6] = L.m[1] * L.m[5] - L.m[2] * L.m[4];
             L.m[7] = L.m[2] * L.m[3] - L.m[0] * L.m[5];
@@ -271,7 +356,7 @@ __device__ __forceinline__ void herm6(const typename Prec<PR>::S* A, int stride,
     T d[6];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        cplx<T> dd = to_c<PR>(ld_nt_raw(A + (size_t)k * stride + idx));
+        cplx<T> dd = to_c<PR>(fld<typename Prec<PR>::S, true>(A + (size_t)k * stride, (unsigned)idx));
         d[2 * k] = sc * dd.re;
         d[2 * k + 1] = sc * dd.im;
     }
@@ -282,7 +367,7 @@ __device__ __forceinline__ void herm6(const typename Prec<PR>::S* A, int stride,
     for (int i = 1; i < 6; ++i) {
 #pragma unroll
         for (int j = 0; j < i; ++j) {
-            cplx<T> a = sc * to_c<PR>(ld_nt_raw(A + (size_t)k * stride + idx));
+            cplx<T> a = sc * to_c<PR>(fld<typename Prec<PR>::S, true>(A + (size_t)k * stride, (unsigned)idx));
             cfma(o[i], a, v[j]);
             cfmac(o[j], a, v[i]);
             ++k;

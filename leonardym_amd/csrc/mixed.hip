@@ -289,10 +289,11 @@ __global__ __launch_bounds__(RED_BLOCKS) void k_mcg_update_final(MCGState* s, co
     if (threadIdx.x == 0) {
         const double rn = out[0];
         if (rn > s->maxrr) s->maxrr = rn;
+        // a reliable update is due; it runs at the next reliable-update slot
+        // (the iteration itself completes in sloppy precision)
         if (rn < s->delta2 * s->maxrr || rn < s->eps) {
             s->rel = 1;
             s->skip_rel = 0;
-            return;  // the reliable update finishes the iteration
         }
         s->beta = rn / s->rr_s;
         s->rr_s = rn;
@@ -364,16 +365,14 @@ __global__ __launch_bounds__(RED_BLOCKS) void k_rel_final(MCGState* s, const dou
         s->nrel += 1;
         s->rel = 0;
         s->skip_rel = 1;
-        s->step += 1;
         if (rt < s->eps) {
             s->done = 1;
             s->converged = 1;
             return;
         }
-        s->beta = rt / s->rr_s;
+        // r_s now holds the true residual; p (already updated) is kept
         s->rr_s = rt;
         s->maxrr = rt;
-        if (s->step >= s->max_steps) s->done = 1;
     }
 }
 

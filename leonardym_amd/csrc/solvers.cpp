@@ -163,6 +163,15 @@ unsigned next_chunk(double r0, double r_now, unsigned done_iters, double target,
 // csw, flags, sizes); solvers keep the last four.  LYM_GRAPHS=0 queues the
 // kernels directly.
 constexpr unsigned GRAPH_ITERS = 8;
+// Mixed-precision solvers launch their reliable-update kernels (accumulate,
+// FP64 apply, true residual) only every REL_EVERY-th iteration; an update
+// that falls due in between waits for that slot while the sloppy iteration
+// goes on.  This removes 5-7 mostly-idle launches from three of four
+// iterations.  GRAPH_ITERS is a multiple of REL_EVERY, so captured chunks and
+// directly queued iterations place the slots identically.
+constexpr unsigned REL_EVERY = 4;
+static_assert(GRAPH_ITERS % REL_EVERY == 0, "slots must line up with captured chunks");
+inline bool rel_slot(unsigned it) { return it % REL_EVERY == REL_EVERY - 1; }
 
 struct Key {
     uint64_t h = 1469598103934665603ull;
@@ -197,13 +206,15 @@ void drop_graphs(lym_solver* s) {
     }
 }
 
-// queue k iterations of body() (at least k: whole chunks when replaying a graph;
-// iterations past the solver's stop are no-ops by its device state)
+// queue k iterations of body(iteration index) (at least k: whole chunks when
+// replaying a graph; iterations past the solver's stop are no-ops by its
+// device state).  When replaying, the queued count is a multiple of
+// GRAPH_ITERS, so the index modulo GRAPH_ITERS is the same in both paths.
 template <class F>
 int queue_iterations(lym_solver* s, uint64_t key, unsigned k, F&& body, unsigned* queued) {
     hipStream_t st = s->ctx->stream;
     if (!graphs_enabled()) {
-        for (unsigned i = 0; i < k; ++i) body();
+        for (unsigned i = 0; i < k; ++i) body(*queued + i);
         *queued += k;
         return LYM_OK;
     }
@@ -218,7 +229,7 @@ int queue_iterations(lym_solver* s, uint64_t key, unsigned k, F&& body, unsigned
         s->graph_exec[slot] = nullptr;
         s->graph_key[slot] = 0;
         HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        for (unsigned i = 0; i < GRAPH_ITERS; ++i) body();
+        for (unsigned i = 0; i < GRAPH_ITERS; ++i) body(i);
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(st, &graph);
         if (e != hipSuccess || !graph) return fail(LYM_ERR_DEVICE, "graph capture: %s", hipGetErrorString(e));
@@ -266,7 +277,7 @@ int run_cg(lym_solver* s, const LinOp& L, const zd* b, zd* xout, const zd* x0, l
     key << 1 << s << n << b;
     key_op(key, L.op);
     key << L.half << L.gamma5 << L.power << L.scratch;
-    auto body = [&] {
+    auto body = [&](unsigned) {
         L.apply(Ap, p, &S->done);
         lym::launch_cg_alpha(S, p, Ap, n, c->red_partials, st);
         lym::launch_cg_update(S, x, r, p, Ap, n, c->red_partials, st);
@@ -318,7 +329,7 @@ int run_bicgstab(lym_solver* s, const LinOp& L, const zd* b, zd* xout, const zd*
     key << 2 << s << n << b;
     key_op(key, L.op);
     key << L.half << L.gamma5 << L.power << L.scratch;
-    auto body = [&] {
+    auto body = [&](unsigned) {
         lym::launch_bicg_rho(S, rh, r, n, c->red_partials, st);
         lym::launch_bicg_p(S, p, r, nu, n, st);
         L.apply(nu, p, &S->done);
@@ -430,14 +441,16 @@ int run_mixed_cg(lym_solver* s, const LinOp& L, const SloppyOp& SL, const zd* b,
     key_op(key, L.op);
     key << L.half << L.gamma5 << L.power << L.scratch << SL.half << SL.gamma5 << SL.power << SL.mid << SL.mid_sc
         << SL.tmp_e << SL.tmp_sc;
-    auto body = [&] {
+    auto body = [&](unsigned it) {
         SL.apply(Ap, Ap_sc, p, p_sc, &S->done);
         lym::launch_mcg_alpha(prec, S, p, p_sc, Ap, Ap_sc, nsites, c->g.stride, c->red_partials, st);
         lym::launch_mcg_update(prec, S, s->xs, s->rs, p, p_sc, Ap, Ap_sc, nsites, c->g.stride, c->red_partials, st);
-        lym::launch_rel_accum(S, x, s->xs, nsites, c->g.stride, st);
-        L.apply(Ax, x, &S->skip_rel);
-        lym::launch_rel_resid(S, r, s->rs, b, Ax, nsites, c->g.stride, c->red_partials, st);
         lym::launch_mcg_p(prec, S, p, p_sc, s->rs, nsites, c->g.stride, st);
+        if (rel_slot(it)) {
+            lym::launch_rel_accum(S, x, s->xs, nsites, c->g.stride, st);
+            L.apply(Ax, x, &S->skip_rel);
+            lym::launch_rel_resid(S, r, s->rs, b, Ax, nsites, c->g.stride, c->red_partials, st);
+        }
     };
     while (!H->done && queued <= s->max_steps) {
         unsigned k = next_chunk(r0, H->rr_s, (unsigned)H->step, s->precision, s->max_steps + 1 - queued);
@@ -684,14 +697,16 @@ int run_batch(lym_solver* s, lym_op* op, bool eo, bool half, int nr, int thr, in
     key << 4 << s << nr << thr << prec << eo << half << g5 << pw << x << r << b << Ax << mid << tmp << a.xs << a.rs
         << a.p << a.Ap << a.p_sc << a.Ap_sc << a.nsites << HS.mid << HS.tmp << HS.mid_sc << HS.tmp_sc;
     key_op(key, op);
-    auto body = [&] {
+    auto body = [&](unsigned it) {
         HS.apply(prec, a.Ap, a.Ap_sc, a.p, a.p_sc, &Sdev->all_done);
         lym::launch_bcg_step(1, prec, nr, Sdev, a, st);
         lym::launch_bcg_step(2, prec, nr, Sdev, a, st);
-        lym::launch_bcg_step(3, prec, nr, Sdev, a, st);
-        H64.apply(0, Ax, nullptr, x, nullptr, &Sdev->skip_rel);
-        lym::launch_bcg_step(4, prec, nr, Sdev, a, st);
         lym::launch_bcg_step(5, prec, nr, Sdev, a, st);
+        if (rel_slot(it)) {
+            lym::launch_bcg_step(3, prec, nr, Sdev, a, st);
+            H64.apply(0, Ax, nullptr, x, nullptr, &Sdev->skip_rel);
+            lym::launch_bcg_step(4, prec, nr, Sdev, a, st);
+        }
     };
     while (!Sh->all_done && queued <= s->max_steps) {
         unsigned k = queued == 0 ? 8 : 16;
