@@ -127,6 +127,9 @@ def gpu_bench(args):
     res["eo_schur"] = {"ms": ms, "gbs": BYTES_PER_ODD_SITE_SCHUR * (V // 2) / (ms * 1e-3) / 1e9}
     log("bench: EO Schur apply %.4f ms  %.1f GB/s" % (ms, res["eo_schur"]["gbs"]))
 
+    # (left out of the counter runs: their copy-kernel calibration reads the
+    # k_copy dispatches of the 16^3x32 spinor only)
+    res["stream"] = stream_bench(lym) if not args.calibrate else None
     secs = set(args.sections.split(","))
     # the same apply with 12-real links (third row rebuilt): its own algorithmic
     # bytes (1344 B/site) and the 18-real-equivalent rate
@@ -218,6 +221,30 @@ def solve_bench(lym, ctx, sets, V):
     return out
 
 
+def stream_bench(lym):
+    """Measured HBM stream rate on this GPU: a plain spinor copy (read + write)
+    over four rotating 402 MB fields (32^3x64), far beyond the 256 MB
+    Infinity Cache."""
+    dims = (32, 32, 32, 64)
+    V = int(np.prod(dims))
+    ctx = lym.Context(dims)
+    bufs = [lym.Spinor(ctx) for _ in range(4)]
+    for b in bufs:
+        b.set_zero()
+    for i in range(8):
+        bufs[(i + 1) % 4].copy_from(bufs[i % 4])
+    ctx.synchronize()
+    ctx.timer_start()
+    n = 40
+    for i in range(n):
+        bufs[(i + 1) % 4].copy_from(bufs[i % 4])
+    ms = ctx.timer_stop() / n
+    gbs = 2 * 192 * V / (ms * 1e-3) / 1e9
+    log("bench: stream copy (read + write) %.1f GB/s" % gbs)
+    del bufs
+    return {"copy_gbs": gbs, "bytes_per_copy": 2 * 192 * V}
+
+
 def mixed_vs_fp64(lym, ctx, op, D, src, label, nrep=2):
     """FP64 even-odd CG driver against the mixed-precision one (FP16 and FP32
     sloppy storage) on one source: solves/s, iterations, true residual and the
@@ -300,7 +327,7 @@ def batch_bench(lym):
     # batched mixed precision, 12 RHS per kappa
     sb = lym.Solver(ctx, lym.SOLVER_PRECOND_MIXED_CG, precision=eps, max_steps=4000, sloppy=lym.PREC_HALF)
     op = lym.Operator(ctx, lym.OP_CLOVER, 1, kappas[0], CSW, True, gauge=g)
-    sb.solve_batch(op, srcs[:2], sols[:2])  # warm: autotune, allocate
+    sb.solve_batch(op, srcs, sols)  # warm: autotune over up to 12 RHS, allocate
     t0 = time.perf_counter()
     its, worst, nconv = [], 0.0, 0
     for k in kappas:
@@ -437,7 +464,9 @@ def main():
                                "eps=0.5, antiperiodic t, %d rotating sets" % ROTATE,
                    "lattice": "16x16x16x32", "link_storage": "18 reals", "bytes_per_site": BYTES_PER_SITE_FULL},
         "roofline": {"bound": "hbm", "achieved": round(res["gbs"], 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(res["gbs"] / HBM_PEAK_GBS, 4), "traffic": profiled_traffic()},
+                     "unit": "GB/s", "frac": round(res["gbs"] / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(),
+                     "measured_stream_gbs": round(res["stream"]["copy_gbs"], 1) if res["stream"] else None,
+                     "frac_of_measured_stream": round(res["gbs"] / res["stream"]["copy_gbs"], 4) if res["stream"] else None},
         "eo_schur": {k: round(v, 4) for k, v in res["eo_schur"].items()},
         "reconstruct12": {k: round(v, 4) for k, v in res["reconstruct12"].items()},
     }

@@ -562,8 +562,12 @@ struct TuneKey {
     int dims[4], lk, prec, half;
     bool operator<(const TuneKey& o) const { return std::memcmp(this, &o, sizeof(*this)) < 0; }
 };
-std::map<TuneKey, std::pair<int, int>>& tune_cache() {
-    static std::map<TuneKey, std::pair<int, int>> m;
+// cached choice: (nr, threads, largest nr the choice was tuned over)
+struct TuneVal {
+    int nr, threads, maxnr;
+};
+std::map<TuneKey, TuneVal>& tune_cache() {
+    static std::map<TuneKey, TuneVal> m;
     return m;
 }
 std::mutex& tune_mutex() {
@@ -585,10 +589,12 @@ int autotune_batch(lym_solver* s, lym_op* op, bool half, int prec, unsigned nrhs
         if ((unsigned)k <= std::max(nrhs, 4u)) maxnr = k;
     {
         std::lock_guard<std::mutex> lk(tune_mutex());
+        // reuse a choice tuned over at least the candidates this call allows;
+        // a choice tuned over fewer RHS is retuned
         auto it = tune_cache().find(key);
-        if (it != tune_cache().end() && it->second.first <= maxnr) {
-            *nr_out = it->second.first;
-            *thr_out = it->second.second;
+        if (it != tune_cache().end() && it->second.maxnr >= maxnr) {
+            *nr_out = std::min(it->second.nr, maxnr);
+            *thr_out = it->second.threads;
             return LYM_OK;
         }
     }
@@ -618,7 +624,7 @@ int autotune_batch(lym_solver* s, lym_op* op, bool half, int prec, unsigned nrhs
     }
     {
         std::lock_guard<std::mutex> lk(tune_mutex());
-        tune_cache()[key] = {bnr, bthr};
+        tune_cache()[key] = TuneVal{bnr, bthr, maxnr};
     }
     *nr_out = bnr;
     *thr_out = bthr;
