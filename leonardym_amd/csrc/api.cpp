@@ -315,7 +315,7 @@ static void set_op_data(lym::DslashParams& a, const lym_op* op, int prec) {
 
 // full lattice, nr right-hand sides, storage precision prec: out = Op in
 void apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* out_sc, const void* in,
-                   const float* in_sc, int gamma5, const int* skip) {
+                   const float* in_sc, int gamma5, const int* skip, const int* lane_flags, int lane_skip_value) {
     lym_ctx* c = op->ctx;
     const size_t po = 12 * (size_t)c->g.stride * nr * selem(prec);
     const size_t so = (size_t)c->g.stride * nr;
@@ -336,12 +336,15 @@ void apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* 
     a.gamma5 = gamma5;
     a.parity = -1;
     a.skip = skip;
+    a.lane_flags = lane_flags;
+    a.lane_skip_value = lane_skip_value;
     lym::launch_dslash(a, link_kind(op->gauge), op_has_clover(op) ? 2 : 1, false, c->stream);
 }
 
 // Schur complement on odd-half nr-fields, storage precision prec, explicit even intermediate
 void apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, float* out_sc, const void* in_o,
-                    const float* in_sc, void* tmp_e, float* tmp_sc, int gamma5, const int* skip) {
+                    const float* in_sc, void* tmp_e, float* tmp_sc, int gamma5, const int* skip,
+                    const int* lane_flags, int lane_skip_value) {
     lym_ctx* c = op->ctx;
     int lk = link_kind(op->gauge);
     lym::DslashParams a = base_params(op);
@@ -349,6 +352,8 @@ void apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, floa
     a.nr = nr;
     a.threads = threads;
     a.skip = skip;
+    a.lane_flags = lane_flags;
+    a.lane_skip_value = lane_skip_value;
     lym::DslashParams b = a;
     a.parity = 0;
     a.in[1] = in_o;
@@ -370,12 +375,12 @@ void apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, floa
 
 void apply_full_sloppy(lym_op* op, int prec, void* out, float* out_sc, const void* in, const float* in_sc, int gamma5,
                        const int* skip) {
-    apply_full_nr(op, prec, 1, 0, out, out_sc, in, in_sc, gamma5, skip);
+    apply_full_nr(op, prec, 1, 0, out, out_sc, in, in_sc, gamma5, skip, nullptr, 0);
 }
 
 void apply_schur_sloppy(lym_op* op, int prec, void* out_o, float* out_sc, const void* in_o, const float* in_sc,
                         void* tmp_e, float* tmp_sc, int gamma5, const int* skip) {
-    apply_schur_nr(op, prec, 1, 0, out_o, out_sc, in_o, in_sc, tmp_e, tmp_sc, gamma5, skip);
+    apply_schur_nr(op, prec, 1, 0, out_o, out_sc, in_o, in_sc, tmp_e, tmp_sc, gamma5, skip, nullptr, 0);
 }
 
 // the EO driver's source and reconstruction steps on nr-fields (FP64)

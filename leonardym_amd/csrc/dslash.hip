@@ -239,6 +239,7 @@ __global__ __launch_bounds__(LYM_DSLASH_THREADS) LYM_OCC void k_dslash(DslashPar
     const int hi = lo + (xcd < rr ? q + 1 : q);
     const int ls = (int)threadIdx.x / a.nr, r = (int)threadIdx.x - ls * a.nr;
     if (ls >= a.spb) return;
+    if (a.lane_flags && a.lane_flags[r] == a.lane_skip_value) return;
     for (int lb = lo + k; lb < hi; lb += nphys) {
         int p, bi;
         if (a.parity >= 0) {

@@ -35,6 +35,8 @@ struct DslashParams {
     int parity;            // 0 or 1: that parity only; -1: both
     int nblocks_per_parity;
     const int* skip;       // optional device flag: nonzero -> the launch does nothing (solver done)
+    const int* lane_flags; // optional per-RHS device flags: lane r does nothing when
+    int lane_skip_value;   //   lane_flags[r] == lane_skip_value (an RHS that is done or needs no update)
     int threads;           // workgroup size (0: LYM_DSLASH_THREADS); a workgroup holds threads / nr sites
 };
 

@@ -41,9 +41,11 @@ void apply_full_sloppy(lym_op* op, int prec, void* out, float* out_sc, const voi
 void apply_schur_sloppy(lym_op* op, int prec, void* out_o, float* out_sc, const void* in_o, const float* in_sc,
                         void* tmp_e, float* tmp_sc, int gamma5, const int* skip);
 void apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* out_sc, const void* in,
-                   const float* in_sc, int gamma5, const int* skip);
+                   const float* in_sc, int gamma5, const int* skip, const int* lane_flags = nullptr,
+                   int lane_skip_value = 0);
 void apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, float* out_sc, const void* in_o,
-                    const float* in_sc, void* tmp_e, float* tmp_sc, int gamma5, const int* skip);
+                    const float* in_sc, void* tmp_e, float* tmp_sc, int gamma5, const int* skip,
+                    const int* lane_flags = nullptr, int lane_skip_value = 0);
 void eo_source_nr(lym_op* op, int nr, const zd* b_e, const zd* b_o, zd* te, zd* src_o);
 void eo_reconstruct_nr(lym_op* op, int nr, const zd* b_e, const zd* x_o, zd* x_e);
 }  // namespace lym_internal
@@ -542,10 +544,16 @@ struct BatchOp {
     float* mid_sc;
     void* tmp;
     float* tmp_sc;
-    void apply(int prec, void* out, float* out_sc, const void* in, const float* in_sc, const int* skip) const {
+    // lane_flags/lane_skip_value: RHS lanes whose flag equals the value do nothing
+    void apply(int prec, void* out, float* out_sc, const void* in, const float* in_sc, const int* skip,
+               const int* lane_flags = nullptr, int lane_skip_value = 0) const {
         auto one = [&](void* o, float* osc, const void* i, const float* isc) {
-            if (half) lym_internal::apply_schur_nr(op, prec, nr, threads, o, osc, i, isc, tmp, tmp_sc, gamma5, skip);
-            else lym_internal::apply_full_nr(op, prec, nr, threads, o, osc, i, isc, gamma5, skip);
+            if (half)
+                lym_internal::apply_schur_nr(op, prec, nr, threads, o, osc, i, isc, tmp, tmp_sc, gamma5, skip,
+                                             lane_flags, lane_skip_value);
+            else
+                lym_internal::apply_full_nr(op, prec, nr, threads, o, osc, i, isc, gamma5, skip, lane_flags,
+                                            lane_skip_value);
         };
         if (power == 1) {
             one(out, out_sc, in, in_sc);
@@ -704,13 +712,14 @@ int run_batch(lym_solver* s, lym_op* op, bool eo, bool half, int nr, int thr, in
         << a.p << a.Ap << a.p_sc << a.Ap_sc << a.nsites << HS.mid << HS.tmp << HS.mid_sc << HS.tmp_sc;
     key_op(key, op);
     auto body = [&](unsigned it) {
-        HS.apply(prec, a.Ap, a.Ap_sc, a.p, a.p_sc, &Sdev->all_done);
+        // finished RHS skip the sloppy apply; only RHS with a due update take the FP64 one
+        HS.apply(prec, a.Ap, a.Ap_sc, a.p, a.p_sc, &Sdev->all_done, Sdev->done, 1);
         lym::launch_bcg_step(1, prec, nr, Sdev, a, st);
         lym::launch_bcg_step(2, prec, nr, Sdev, a, st);
         lym::launch_bcg_step(5, prec, nr, Sdev, a, st);
         if (rel_slot(it)) {
             lym::launch_bcg_step(3, prec, nr, Sdev, a, st);
-            H64.apply(0, Ax, nullptr, x, nullptr, &Sdev->skip_rel);
+            H64.apply(0, Ax, nullptr, x, nullptr, &Sdev->skip_rel, Sdev->rel, 0);
             lym::launch_bcg_step(4, prec, nr, Sdev, a, st);
         }
     };
