@@ -80,6 +80,11 @@ typedef struct {
 /* ------------------------------------------------------------------ errors / device */
 const char* lym_last_error(void);
 int lym_device_count(int* n);
+/* Page-locked host memory (no counterpart in the reference): spinor and link
+ * uploads/downloads from such buffers run as direct DMA at full link speed
+ * instead of being staged through the driver's pinned pool. */
+int lym_host_alloc(size_t bytes, void** out);
+void lym_host_free(void* p);
 
 /* ------------------------------------------------------------------ context
  * Lattice geometry + stream: replaces LocalLayout::initialize()

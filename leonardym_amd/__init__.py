@@ -45,6 +45,8 @@ class RequestInfo(ctypes.Structure):
 SIGNATURES = {
     "lym_last_error": (ctypes.c_char_p, []),
     "lym_device_count": (_I, [_PI]),
+    "lym_host_alloc": (_I, [ctypes.c_size_t, ctypes.POINTER(_P)]),
+    "lym_host_free": (None, [_P]),
     "lym_ctx_create": (_I, [_P, _I, _I, ctypes.POINTER(_P)]),
     "lym_ctx_destroy": (None, [_P]),
     "lym_ctx_synchronize": (_I, [_P]),
@@ -176,6 +178,33 @@ def gaussian_spinor(nsites, seed):
     out = np.empty((nsites, 4, 3), np.complex128)
     _check(lib().lym_gen_gaussian_spinor(nsites, seed, _arr(out)))
     return out
+
+
+class _Pinned:
+    """Owner of a page-locked host block; freed when the last array viewing it goes."""
+
+    def __init__(self, nbytes):
+        p = ctypes.c_void_p()
+        _check(lib().lym_host_alloc(nbytes, ctypes.byref(p)))
+        self.p, self.nbytes = p, nbytes
+
+    def __del__(self):
+        try:
+            if self.p:
+                lib().lym_host_free(self.p)
+                self.p = None
+        except Exception:
+            pass
+
+
+def pinned_empty(shape, dtype=np.complex128):
+    """numpy array in page-locked host memory (fast uploads and downloads)."""
+    dt = np.dtype(dtype)
+    n = int(np.prod(shape)) * dt.itemsize
+    owner = _Pinned(max(n, 1))
+    buf = (ctypes.c_char * max(n, 1)).from_address(owner.p.value)
+    buf._owner = owner  # keeps the block alive as long as the array
+    return np.frombuffer(buf, dtype=dt, count=int(np.prod(shape))).reshape(shape)
 
 
 def point_source(dims, site, spin, colour):

@@ -148,6 +148,7 @@ void free_sloppy(lym_op* op) {
     op->Us = op->As = op->Ainvs = nullptr;
     op->As_sc = op->Ainvs_sc = nullptr;
     op->sloppy_prec = 0;
+    op->sloppy_alloc = 0;
 }
 
 // low-precision copies of links and blocks for a mixed-precision solve
@@ -155,21 +156,25 @@ int op_sloppy_data(lym_op* op, int prec) {
     lym_ctx* c = op->ctx;
     if (prec != 1 && prec != 2) return fail(LYM_ERR_INVALID, "sloppy precision must be 1 (FP32) or 2 (FP16)");
     if (op->sloppy_prec == prec) return LYM_OK;
-    free_sloppy(op);
+    // buffers stay allocated across kappa/csw/gauge changes (a free would
+    // synchronise the device); only a change of precision reallocates
+    if (op->sloppy_alloc != prec) free_sloppy(op);
     const lym_gauge* g = op->gauge;
     const long S = c->g.stride;
     const int ncomp = g->real_links ? g->compression : g->compression / 2;
     const long nl = 2L * 4 * ncomp * S;
     const size_t le = g->real_links ? (prec == 1 ? 4 : 2) : (prec == 1 ? 8 : 4);
     const size_t se = prec == 1 ? 8 : 4;  // spinor/block element
-    int rc = dmalloc(c, &op->Us, nl * le);
+    int rc = LYM_OK;
+    if (!op->Us) rc = dmalloc(c, &op->Us, 2L * 4 * 9 * S * le);  // room for 18-real links
     if (rc) return rc;
+    op->sloppy_alloc = prec;
     lym::launch_convert_links(prec, g->U, op->Us, nl, g->real_links, c->stream);
     if (op->A) {
-        rc = dmalloc(c, &op->As, 2L * 36 * S * se);
-        if (!rc) rc = dmalloc(c, (void**)&op->As_sc, 2L * S * sizeof(float));
-        if (!rc) rc = dmalloc(c, &op->Ainvs, 36L * S * se);
-        if (!rc) rc = dmalloc(c, (void**)&op->Ainvs_sc, S * sizeof(float));
+        if (!op->As) rc = dmalloc(c, &op->As, 2L * 36 * S * se);
+        if (!rc && !op->As_sc) rc = dmalloc(c, (void**)&op->As_sc, 2L * S * sizeof(float));
+        if (!rc && !op->Ainvs) rc = dmalloc(c, &op->Ainvs, 36L * S * se);
+        if (!rc && !op->Ainvs_sc) rc = dmalloc(c, (void**)&op->Ainvs_sc, S * sizeof(float));
         if (rc) return rc;
         lym::launch_convert_blocks(prec, reinterpret_cast<const zd*>(op->A), op->As, op->As_sc, 2, (int)S, c->stream);
         lym::launch_convert_blocks(prec, reinterpret_cast<const zd*>(op->Ainv), op->Ainvs, op->Ainvs_sc, 1, (int)S,
@@ -425,6 +430,21 @@ int lym_device_count(int* n) {
     }
     *n = k;
     return LYM_OK;
+}
+
+int lym_host_alloc(size_t bytes, void** out) {
+    if (!out) return fail(LYM_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (bytes == 0) return LYM_OK;
+    if (hipHostMalloc(out, bytes, hipHostMallocDefault) != hipSuccess) {
+        *out = nullptr;
+        return fail(LYM_ERR_NOMEM, "hipHostMalloc of %zu bytes", bytes);
+    }
+    return LYM_OK;
+}
+
+void lym_host_free(void* p) {
+    if (p) (void)hipHostFree(p);
 }
 
 int lym_ctx_create(const int dims[4], int rep, int bc_t, lym_ctx** out) {

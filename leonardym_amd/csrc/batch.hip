@@ -12,56 +12,63 @@
 
 namespace lym {
 
-constexpr int BR_THREADS = 256;
+// Every per-RHS sum runs over fixed chunks of BC consecutive sites: a chunk's
+// sum is an LDS tree over its BC sites, and the final sum an LDS tree over
+// per-thread strided sums of the chunk partials, with a fixed FT-thread
+// block.  Neither tree depends on NR, so an RHS sums its terms in the same
+// order in a batch of any size.
+constexpr int BC = BATCH_CHUNK;  // sites per chunk
+constexpr int FT = 1024;   // threads of the final-sum block
 
-template <int NV>
-__device__ __forceinline__ void b_block_store(double v[NV], double* partials) {
-    __shared__ double sh[NV][BR_THREADS / 64];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) v[k] += __shfl_xor(v[k], off, 64);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) sh[k][w] = v[k];
+// chunk sum of v (one value per (site, lane) thread; block = BC sites x NR lanes)
+template <int NR>
+__device__ __forceinline__ void chunk_store(double v, double* partials, long nchunks) {
+    __shared__ double red[NR * BC];
+    const int t = threadIdx.x, sl = t / NR, q = t - sl * NR;
+    red[q * BC + sl] = v;
     __syncthreads();
-    if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            double s = 0;
-#pragma unroll
-            for (int j = 0; j < BR_THREADS / 64; ++j) s += sh[k][j];
-            partials[k * RED_BLOCKS + blockIdx.x] = s;
-        }
+    for (int off = BC / 2; off > 0; off >>= 1) {
+        if (sl < off) red[q * BC + sl] += red[q * BC + sl + off];
+        __syncthreads();
+    }
+    if (sl == 0) partials[(long)q * nchunks + blockIdx.x] = red[q * BC];
 }
 
-template <int NV>
-__device__ __forceinline__ void b_final_sum(const double* partials, double out[NV]) {
-    __shared__ double sh[NV][RED_BLOCKS / 64];
-    double v[NV];
+// final per-RHS sums (valid in thread 0), one FT-thread block
+template <int NR>
+__device__ __forceinline__ void final_sums(const double* partials, long nchunks, double out[NR]) {
+    __shared__ double sh[FT];
+    for (int q = 0; q < NR; ++q) {
+        double v = 0;
+        for (long c = threadIdx.x; c < nchunks; c += FT) v += partials[(long)q * nchunks + c];
+        sh[threadIdx.x] = v;
+        __syncthreads();
 #pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] = partials[k * RED_BLOCKS + threadIdx.x];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) v[k] += __shfl_xor(v[k], off, 64);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) sh[k][w] = v[k];
-    __syncthreads();
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            double s = 0;
-#pragma unroll
-            for (int j = 0; j < RED_BLOCKS / 64; ++j) s += sh[k][j];
-            out[k] = s;
+        for (int off = FT / 2; off > 0; off >>= 1) {
+            if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+            __syncthreads();
         }
+        if (threadIdx.x == 0) out[q] = sh[0];
+        __syncthreads();
+    }
 }
 
 #define BSITE_LOOP(j, n) for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < (n); j += (long)gridDim.x * blockDim.x)
+
+// (site, lane) of this thread in a chunked launch; false past the field
+template <int NR>
+__device__ __forceinline__ bool chunk_pair(long nsites, long& j, int& q) {
+    const int t = threadIdx.x, sl = t / NR;
+    q = t - sl * NR;
+    j = (long)blockIdx.x * BC + sl;
+    return j < nsites;
+}
+
+#define PAIR_LOOP(j, q, NR, nsites)                                                                        \
+    for (long t_ = (long)blockIdx.x * blockDim.x + threadIdx.x, j, q; t_ < (nsites) * (NR) &&              \
+                                                                  ((j = t_ / (NR)), (q = t_ - j * (NR)), 1); \
+         t_ += (long)gridDim.x * blockDim.x)
 
 // element (parity p, component k, site i, rhs r) of an nr-field: ((p*12 + k)*stride + i)*nr + r
 __device__ __forceinline__ long bidx(int stride, int nr, long j, int k, int r) {
@@ -113,37 +120,37 @@ __device__ __forceinline__ void bg_store(E* d, int stride, long j, int r, const 
 }
 
 // ---------------------------------------------------------------- kernels
+// Sums: one thread per (site, RHS) pair, RHS fastest (coalesced like the
+// fields), chunk_store / final_sums.  Element-wise kernels: PAIR_LOOP.
 template <int PR, int NR>
-__global__ __launch_bounds__(BR_THREADS) void k_bcg_init(const zd* r, cplx<float>* rs, cplx<float>* xs,
-                                                         typename Prec<PR>::S* p, float* p_sc, long nsites, int stride,
-                                                         double* partials) {
-    double v[NR];
+__global__ __launch_bounds__(BC * NR) void k_bcg_init(const zd* r, cplx<float>* rs, cplx<float>* xs,
+                                                      typename Prec<PR>::S* p, float* p_sc, long nsites, int stride,
+                                                      double* partials, long nchunks) {
+    long j;
+    int q;
+    double v = 0;
+    if (chunk_pair<NR>(nsites, j, q)) {
+        zd a[12];
+        cplx<float> f[12], z[12];
+        bg_load<zd, NR>(r, stride, j, q, a);
 #pragma unroll
-    for (int q = 0; q < NR; ++q) v[q] = 0;
-    BSITE_LOOP(j, nsites) {
-#pragma unroll
-        for (int q = 0; q < NR; ++q) {
-            zd a[12];
-            cplx<float> f[12], z[12];
-            bg_load<zd, NR>(r, stride, j, q, a);
-#pragma unroll
-            for (int k = 0; k < 12; ++k) {
-                v[q] += a[k].re * a[k].re + a[k].im * a[k].im;
-                f[k] = mk((float)a[k].re, (float)a[k].im);
-                z[k] = mk(0.f, 0.f);
-            }
-            bg_store<cplx<float>, NR>(rs, stride, j, q, f);
-            bg_store<cplx<float>, NR>(xs, stride, j, q, z);
-            bs_store<PR, NR>(p, p_sc, stride, j, q, f);
+        for (int k = 0; k < 12; ++k) {
+            v += a[k].re * a[k].re + a[k].im * a[k].im;
+            f[k] = mk((float)a[k].re, (float)a[k].im);
+            z[k] = mk(0.f, 0.f);
         }
+        bg_store<cplx<float>, NR>(rs, stride, j, q, f);
+        bg_store<cplx<float>, NR>(xs, stride, j, q, z);
+        bs_store<PR, NR>(p, p_sc, stride, j, q, f);
     }
-    b_block_store<NR>(v, partials);
+    chunk_store<NR>(v, partials, nchunks);
 }
 
 template <int NR>
-__global__ __launch_bounds__(RED_BLOCKS) void k_bcg_init_final(BState* s, const double* partials, int max_steps) {
+__global__ __launch_bounds__(FT) void k_bcg_init_final(BState* s, const double* partials, long nchunks,
+                                                       int max_steps) {
     double out[NR];
-    b_final_sum<NR>(partials, out);
+    final_sums<NR>(partials, nchunks, out);
     if (threadIdx.x == 0) {
         int all = 1;
         for (int q = 0; q < NR; ++q) {
@@ -165,73 +172,67 @@ __global__ __launch_bounds__(RED_BLOCKS) void k_bcg_init_final(BState* s, const 
 }
 
 template <int PR, int NR>
-__global__ __launch_bounds__(BR_THREADS) void k_bcg_dot(const BState* s, const typename Prec<PR>::S* p,
-                                                        const float* p_sc, const typename Prec<PR>::S* Ap,
-                                                        const float* Ap_sc, long nsites, int stride, double* partials) {
+__global__ __launch_bounds__(BC * NR) void k_bcg_dot(const BState* s, const typename Prec<PR>::S* p,
+                                                     const float* p_sc, const typename Prec<PR>::S* Ap,
+                                                     const float* Ap_sc, long nsites, int stride, double* partials,
+                                                     long nchunks) {
     if (s->all_done) return;
-    double v[NR];
+    long j;
+    int q;
+    double v = 0;
+    if (chunk_pair<NR>(nsites, j, q) && !s->done[q]) {
+        cplx<float> a[12], b[12];
+        bs_load<PR, NR>(p, p_sc, stride, j, q, a);
+        bs_load<PR, NR>(Ap, Ap_sc, stride, j, q, b);
 #pragma unroll
-    for (int q = 0; q < NR; ++q) v[q] = 0;
-    BSITE_LOOP(j, nsites) {
-#pragma unroll
-        for (int q = 0; q < NR; ++q) {
-            cplx<float> a[12], b[12];
-            bs_load<PR, NR>(p, p_sc, stride, j, q, a);
-            bs_load<PR, NR>(Ap, Ap_sc, stride, j, q, b);
-#pragma unroll
-            for (int k = 0; k < 12; ++k) v[q] += (double)a[k].re * b[k].re + (double)a[k].im * b[k].im;
-        }
+        for (int k = 0; k < 12; ++k) v += (double)a[k].re * b[k].re + (double)a[k].im * b[k].im;
     }
-    b_block_store<NR>(v, partials);
+    chunk_store<NR>(v, partials, nchunks);
 }
 
 template <int NR>
-__global__ __launch_bounds__(RED_BLOCKS) void k_bcg_alpha_final(BState* s, const double* partials) {
+__global__ __launch_bounds__(FT) void k_bcg_alpha_final(BState* s, const double* partials, long nchunks) {
     if (s->all_done) return;
     double out[NR];
-    b_final_sum<NR>(partials, out);
+    final_sums<NR>(partials, nchunks, out);
     if (threadIdx.x == 0)
         for (int q = 0; q < NR; ++q)
             if (!s->done[q]) s->alpha[q] = s->rr_s[q] / out[q];
 }
 
 template <int PR, int NR>
-__global__ __launch_bounds__(BR_THREADS) void k_bcg_update(const BState* s, cplx<float>* xs, cplx<float>* rs,
-                                                           const typename Prec<PR>::S* p, const float* p_sc,
-                                                           const typename Prec<PR>::S* Ap, const float* Ap_sc,
-                                                           long nsites, int stride, double* partials) {
+__global__ __launch_bounds__(BC * NR) void k_bcg_update(const BState* s, cplx<float>* xs, cplx<float>* rs,
+                                                        const typename Prec<PR>::S* p, const float* p_sc,
+                                                        const typename Prec<PR>::S* Ap, const float* Ap_sc,
+                                                        long nsites, int stride, double* partials, long nchunks) {
     if (s->all_done) return;
-    double v[NR];
+    long j;
+    int q;
+    double v = 0;
+    if (chunk_pair<NR>(nsites, j, q) && !s->done[q]) {
+        const float al = (float)s->alpha[q];
+        cplx<float> x[12], r[12], a[12], b[12];
+        bg_load<cplx<float>, NR>(xs, stride, j, q, x);
+        bg_load<cplx<float>, NR>(rs, stride, j, q, r);
+        bs_load<PR, NR>(p, p_sc, stride, j, q, a);
+        bs_load<PR, NR>(Ap, Ap_sc, stride, j, q, b);
 #pragma unroll
-    for (int q = 0; q < NR; ++q) v[q] = 0;
-    BSITE_LOOP(j, nsites) {
-#pragma unroll
-        for (int q = 0; q < NR; ++q) {
-            if (s->done[q]) continue;
-            const float al = (float)s->alpha[q];
-            cplx<float> x[12], r[12], a[12], b[12];
-            bg_load<cplx<float>, NR>(xs, stride, j, q, x);
-            bg_load<cplx<float>, NR>(rs, stride, j, q, r);
-            bs_load<PR, NR>(p, p_sc, stride, j, q, a);
-            bs_load<PR, NR>(Ap, Ap_sc, stride, j, q, b);
-#pragma unroll
-            for (int k = 0; k < 12; ++k) {
-                x[k] = x[k] + al * a[k];
-                r[k] = r[k] - al * b[k];
-                v[q] += (double)r[k].re * r[k].re + (double)r[k].im * r[k].im;
-            }
-            bg_store<cplx<float>, NR>(xs, stride, j, q, x);
-            bg_store<cplx<float>, NR>(rs, stride, j, q, r);
+        for (int k = 0; k < 12; ++k) {
+            x[k] = x[k] + al * a[k];
+            r[k] = r[k] - al * b[k];
+            v += (double)r[k].re * r[k].re + (double)r[k].im * r[k].im;
         }
+        bg_store<cplx<float>, NR>(xs, stride, j, q, x);
+        bg_store<cplx<float>, NR>(rs, stride, j, q, r);
     }
-    b_block_store<NR>(v, partials);
+    chunk_store<NR>(v, partials, nchunks);
 }
 
 template <int NR>
-__global__ __launch_bounds__(RED_BLOCKS) void k_bcg_update_final(BState* s, const double* partials) {
+__global__ __launch_bounds__(FT) void k_bcg_update_final(BState* s, const double* partials, long nchunks) {
     if (s->all_done) return;
     double out[NR];
-    b_final_sum<NR>(partials, out);
+    final_sums<NR>(partials, nchunks, out);
     if (threadIdx.x == 0) {
         int any = 0;
         for (int q = 0; q < NR; ++q) {
@@ -254,61 +255,57 @@ __global__ __launch_bounds__(RED_BLOCKS) void k_bcg_update_final(BState* s, cons
     }
 }
 
+// reliable update, part 1, for the RHS with a due update: x += x_s, x_s = 0
 template <int NR>
 __global__ void k_brel_accum(const BState* s, zd* x, cplx<float>* xs, long nsites, int stride) {
     if (s->skip_rel) return;
-    BSITE_LOOP(j, nsites) {
+    PAIR_LOOP(j, q, NR, nsites) {
+        if (!s->rel[q]) continue;
+        zd a[12];
+        cplx<float> f[12];
+        bg_load<zd, NR>(x, stride, j, (int)q, a);
+        bg_load<cplx<float>, NR>(xs, stride, j, (int)q, f);
 #pragma unroll
-        for (int q = 0; q < NR; ++q) {
-            if (!s->rel[q]) continue;
-            zd a[12];
-            cplx<float> f[12];
-            bg_load<zd, NR>(x, stride, j, q, a);
-            bg_load<cplx<float>, NR>(xs, stride, j, q, f);
-#pragma unroll
-            for (int k = 0; k < 12; ++k) {
-                a[k] = a[k] + mk((double)f[k].re, (double)f[k].im);
-                f[k] = mk(0.f, 0.f);
-            }
-            bg_store<zd, NR>(x, stride, j, q, a);
-            bg_store<cplx<float>, NR>(xs, stride, j, q, f);
+        for (int k = 0; k < 12; ++k) {
+            a[k] = a[k] + mk((double)f[k].re, (double)f[k].im);
+            f[k] = mk(0.f, 0.f);
         }
+        bg_store<zd, NR>(x, stride, j, (int)q, a);
+        bg_store<cplx<float>, NR>(xs, stride, j, (int)q, f);
     }
 }
 
+// reliable update, part 2: r = b - Ax (FP64), r_s = r, |r|^2
 template <int NR>
-__global__ __launch_bounds__(BR_THREADS) void k_brel_resid(const BState* s, zd* r, cplx<float>* rs, const zd* b,
-                                                           const zd* Ax, long nsites, int stride, double* partials) {
+__global__ __launch_bounds__(BC * NR) void k_brel_resid(const BState* s, zd* r, cplx<float>* rs, const zd* b,
+                                                        const zd* Ax, long nsites, int stride, double* partials,
+                                                        long nchunks) {
     if (s->skip_rel) return;
-    double v[NR];
+    long j;
+    int q;
+    double v = 0;
+    if (chunk_pair<NR>(nsites, j, q) && s->rel[q]) {
+        zd bb[12], ax[12];
+        cplx<float> f[12];
+        bg_load<zd, NR>(b, stride, j, q, bb);
+        bg_load<zd, NR>(Ax, stride, j, q, ax);
 #pragma unroll
-    for (int q = 0; q < NR; ++q) v[q] = 0;
-    BSITE_LOOP(j, nsites) {
-#pragma unroll
-        for (int q = 0; q < NR; ++q) {
-            if (!s->rel[q]) continue;
-            zd bb[12], ax[12];
-            cplx<float> f[12];
-            bg_load<zd, NR>(b, stride, j, q, bb);
-            bg_load<zd, NR>(Ax, stride, j, q, ax);
-#pragma unroll
-            for (int k = 0; k < 12; ++k) {
-                bb[k] = bb[k] - ax[k];
-                v[q] += bb[k].re * bb[k].re + bb[k].im * bb[k].im;
-                f[k] = mk((float)bb[k].re, (float)bb[k].im);
-            }
-            bg_store<zd, NR>(r, stride, j, q, bb);
-            bg_store<cplx<float>, NR>(rs, stride, j, q, f);
+        for (int k = 0; k < 12; ++k) {
+            bb[k] = bb[k] - ax[k];
+            v += bb[k].re * bb[k].re + bb[k].im * bb[k].im;
+            f[k] = mk((float)bb[k].re, (float)bb[k].im);
         }
+        bg_store<zd, NR>(r, stride, j, q, bb);
+        bg_store<cplx<float>, NR>(rs, stride, j, q, f);
     }
-    b_block_store<NR>(v, partials);
+    chunk_store<NR>(v, partials, nchunks);
 }
 
 template <int NR>
-__global__ __launch_bounds__(RED_BLOCKS) void k_brel_final(BState* s, const double* partials) {
+__global__ __launch_bounds__(FT) void k_brel_final(BState* s, const double* partials, long nchunks) {
     if (s->skip_rel) return;
     double out[NR];
-    b_final_sum<NR>(partials, out);
+    final_sums<NR>(partials, nchunks, out);
     if (threadIdx.x == 0) {
         for (int q = 0; q < NR; ++q) {
             if (!s->rel[q]) continue;
@@ -332,40 +329,35 @@ __global__ __launch_bounds__(RED_BLOCKS) void k_brel_final(BState* s, const doub
     }
 }
 
+// p = r_s + beta p for the RHS still running
 template <int PR, int NR>
 __global__ void k_bcg_p(const BState* s, typename Prec<PR>::S* p, float* p_sc, const cplx<float>* rs, long nsites,
                         int stride) {
     if (s->all_done) return;
-    BSITE_LOOP(j, nsites) {
+    PAIR_LOOP(j, q, NR, nsites) {
+        if (s->done[q]) continue;
+        const float be = (float)s->beta[q];
+        cplx<float> a[12], r[12];
+        bs_load<PR, NR>(p, p_sc, stride, j, (int)q, a);
+        bg_load<cplx<float>, NR>(rs, stride, j, (int)q, r);
 #pragma unroll
-        for (int q = 0; q < NR; ++q) {
-            if (s->done[q]) continue;
-            const float be = (float)s->beta[q];
-            cplx<float> a[12], r[12];
-            bs_load<PR, NR>(p, p_sc, stride, j, q, a);
-            bg_load<cplx<float>, NR>(rs, stride, j, q, r);
-#pragma unroll
-            for (int k = 0; k < 12; ++k) a[k] = r[k] + be * a[k];
-            bs_store<PR, NR>(p, p_sc, stride, j, q, a);
-        }
+        for (int k = 0; k < 12; ++k) a[k] = r[k] + be * a[k];
+        bs_store<PR, NR>(p, p_sc, stride, j, (int)q, a);
     }
 }
 
 // x += x_s for the RHS that stopped between reliable updates
 template <int NR>
 __global__ void k_baccum(const BState* s, zd* x, const cplx<float>* xs, long nsites, int stride) {
-    BSITE_LOOP(j, nsites) {
+    PAIR_LOOP(j, q, NR, nsites) {
+        if (s->conv[q]) continue;
+        zd a[12];
+        cplx<float> f[12];
+        bg_load<zd, NR>(x, stride, j, (int)q, a);
+        bg_load<cplx<float>, NR>(xs, stride, j, (int)q, f);
 #pragma unroll
-        for (int q = 0; q < NR; ++q) {
-            if (s->conv[q]) continue;
-            zd a[12];
-            cplx<float> f[12];
-            bg_load<zd, NR>(x, stride, j, q, a);
-            bg_load<cplx<float>, NR>(xs, stride, j, q, f);
-#pragma unroll
-            for (int k = 0; k < 12; ++k) a[k] = a[k] + mk((double)f[k].re, (double)f[k].im);
-            bg_store<zd, NR>(x, stride, j, q, a);
-        }
+        for (int k = 0; k < 12; ++k) a[k] = a[k] + mk((double)f[k].re, (double)f[k].im);
+        bg_store<zd, NR>(x, stride, j, (int)q, a);
     }
 }
 
@@ -401,43 +393,48 @@ static inline int bsite_blocks(long n) {
     return (int)(b < 4096 ? b : 4096);
 }
 
+static inline int pair_blocks(long pairs) {
+    long b = (pairs + 255) / 256;
+    return (int)(b < 8192 ? b : 8192);
+}
+
 template <int PR, int NR>
 static void bcg_step_impl(int which, BState* s, const BcgArgs& a, hipStream_t st) {
     typedef typename Prec<PR>::S S;
     S* p = reinterpret_cast<S*>(a.p);
     S* Ap = reinterpret_cast<S*>(a.Ap);
+    const long nch = (a.nsites + BC - 1) / BC;
+    const dim3 cg((unsigned)nch), cb(BC * NR), fb(FT);
+    const int eb = pair_blocks(a.nsites * NR);
     switch (which) {
         case 0:
-            hipLaunchKernelGGL((k_bcg_init<PR, NR>), dim3(RED_BLOCKS), dim3(BR_THREADS), 0, st, a.r, a.rs, a.xs, p,
-                               a.p_sc, a.nsites, a.stride, a.partials);
-            hipLaunchKernelGGL((k_bcg_init_final<NR>), dim3(1), dim3(RED_BLOCKS), 0, st, s, a.partials, a.max_steps);
+            hipLaunchKernelGGL((k_bcg_init<PR, NR>), cg, cb, 0, st, a.r, a.rs, a.xs, p, a.p_sc, a.nsites, a.stride,
+                               a.partials, nch);
+            hipLaunchKernelGGL((k_bcg_init_final<NR>), dim3(1), fb, 0, st, s, a.partials, nch, a.max_steps);
             break;
         case 1:
-            hipLaunchKernelGGL((k_bcg_dot<PR, NR>), dim3(RED_BLOCKS), dim3(BR_THREADS), 0, st, s, p, a.p_sc, Ap,
-                               a.Ap_sc, a.nsites, a.stride, a.partials);
-            hipLaunchKernelGGL((k_bcg_alpha_final<NR>), dim3(1), dim3(RED_BLOCKS), 0, st, s, a.partials);
+            hipLaunchKernelGGL((k_bcg_dot<PR, NR>), cg, cb, 0, st, s, p, a.p_sc, Ap, a.Ap_sc, a.nsites, a.stride,
+                               a.partials, nch);
+            hipLaunchKernelGGL((k_bcg_alpha_final<NR>), dim3(1), fb, 0, st, s, a.partials, nch);
             break;
         case 2:
-            hipLaunchKernelGGL((k_bcg_update<PR, NR>), dim3(RED_BLOCKS), dim3(BR_THREADS), 0, st, s, a.xs, a.rs, p,
-                               a.p_sc, Ap, a.Ap_sc, a.nsites, a.stride, a.partials);
-            hipLaunchKernelGGL((k_bcg_update_final<NR>), dim3(1), dim3(RED_BLOCKS), 0, st, s, a.partials);
+            hipLaunchKernelGGL((k_bcg_update<PR, NR>), cg, cb, 0, st, s, a.xs, a.rs, p, a.p_sc, Ap, a.Ap_sc, a.nsites,
+                               a.stride, a.partials, nch);
+            hipLaunchKernelGGL((k_bcg_update_final<NR>), dim3(1), fb, 0, st, s, a.partials, nch);
             break;
         case 3:
-            hipLaunchKernelGGL((k_brel_accum<NR>), dim3(bsite_blocks(a.nsites)), dim3(256), 0, st, s, a.x, a.xs,
-                               a.nsites, a.stride);
+            hipLaunchKernelGGL((k_brel_accum<NR>), dim3(eb), dim3(256), 0, st, s, a.x, a.xs, a.nsites, a.stride);
             break;
         case 4:
-            hipLaunchKernelGGL((k_brel_resid<NR>), dim3(RED_BLOCKS), dim3(BR_THREADS), 0, st, s, a.r, a.rs, a.b, a.Ax,
-                               a.nsites, a.stride, a.partials);
-            hipLaunchKernelGGL((k_brel_final<NR>), dim3(1), dim3(RED_BLOCKS), 0, st, s, a.partials);
+            hipLaunchKernelGGL((k_brel_resid<NR>), cg, cb, 0, st, s, a.r, a.rs, a.b, a.Ax, a.nsites, a.stride,
+                               a.partials, nch);
+            hipLaunchKernelGGL((k_brel_final<NR>), dim3(1), fb, 0, st, s, a.partials, nch);
             break;
         case 5:
-            hipLaunchKernelGGL((k_bcg_p<PR, NR>), dim3(bsite_blocks(a.nsites)), dim3(256), 0, st, s, p, a.p_sc, a.rs,
-                               a.nsites, a.stride);
+            hipLaunchKernelGGL((k_bcg_p<PR, NR>), dim3(eb), dim3(256), 0, st, s, p, a.p_sc, a.rs, a.nsites, a.stride);
             break;
         default:
-            hipLaunchKernelGGL((k_baccum<NR>), dim3(bsite_blocks(a.nsites)), dim3(256), 0, st, s, a.x, a.xs,
-                               a.nsites, a.stride);
+            hipLaunchKernelGGL((k_baccum<NR>), dim3(eb), dim3(256), 0, st, s, a.x, a.xs, a.nsites, a.stride);
             break;
     }
 }

@@ -156,6 +156,7 @@ struct lym_op {
     double2* tmp2;     // half-spinor scratch for the Schur operator
     // low-precision copies for mixed-precision solvers (sloppy_prec 1 FP32, 2 FP16; 0 none)
     int sloppy_prec;
+    int sloppy_alloc;  // precision the low-precision buffers are allocated for (kept across kappa changes)
     void* Us;          // links
     void* As;          // [2][36][stride] blocks
     float* As_sc;      // [2][stride]

@@ -137,6 +137,7 @@ struct BState {
     double delta2;
     double eps[BMAX], rr_s[BMAX], rr_true[BMAX], maxrr[BMAX], alpha[BMAX], beta[BMAX];
 };
+constexpr int BATCH_CHUNK = 64;  // sites per fixed summation chunk of the batched solver
 struct BcgArgs {
     zd* x;
     zd* r;
@@ -151,7 +152,7 @@ struct BcgArgs {
     long nsites;
     int stride;
     int max_steps;
-    double* partials;  // BMAX * RED_BLOCKS doubles
+    double* partials;  // BMAX * ceil(nsites / BATCH_CHUNK) doubles
 };
 // which: 0 init, 1 alpha, 2 update, 3 reliable accumulate, 4 reliable residual, 5 p update, 6 final accumulate
 bool launch_bcg_step(int which, int prec, int nr, BState* s, const BcgArgs& a, hipStream_t st);

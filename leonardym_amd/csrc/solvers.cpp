@@ -505,7 +505,8 @@ int ensure_batch(lym_solver* s, int nr, int dom, int prec) {
     lym_ctx* c = s->ctx;
     int rc;
     if (!s->bpartials) {
-        if ((rc = dzalloc(c, (void**)&s->bpartials, sizeof(double) * lym::BMAX * lym::RED_BLOCKS))) return rc;
+        const size_t nch = (2 * (size_t)c->g.stride + lym::BATCH_CHUNK - 1) / lym::BATCH_CHUNK;
+        if ((rc = dzalloc(c, (void**)&s->bpartials, sizeof(double) * lym::BMAX * nch))) return rc;
         if ((rc = dzalloc(c, &s->bstate, sizeof(lym::BState)))) return rc;
         if (hipHostMalloc(&s->bhost, sizeof(lym::BState), hipHostMallocDefault) != hipSuccess)
             return fail(LYM_ERR_DEVICE, "hipHostMalloc");
