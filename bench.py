@@ -395,9 +395,9 @@ def serve_bench(lym, lambdas, nreq, workers=4):
             dt = t0 + ta - time.perf_counter()
             if dt > 0:
                 time.sleep(dt)
-            out = outs[li][used[li]]
+            dst = outs[li][used[li]]
             used[li] += 1
-            rids.append(srv.submit(ids[li], k, tol, srcs[li][si], out))
+            rids.append(srv.submit(ids[li], k, tol, srcs[li][si], dst))
         lat, conv = [], 0
         for r in rids:
             info, _ = srv.wait(r)

@@ -193,6 +193,7 @@ lym::DslashParams base_params(const lym_op* op) {
     a.A = reinterpret_cast<const zd*>(op->A);
     a.Ainv = reinterpret_cast<const zd*>(op->Ainv);
     a.kappa = op->kappa;
+    a.kappa_dev = op->dev_kappa;
     a.hop_sign = 1.0;
     a.alpha = lym::mk(0.0, 0.0);
     return a;
@@ -452,7 +453,9 @@ int lym_ctx_create(const int dims[4], int rep, int bc_t, lym_ctx** out) {
     *out = nullptr;
     for (int i = 0; i < 4; ++i)
         if (dims[i] < 2) return fail(LYM_ERR_INVALID, "lattice extent %d must be >= 2", dims[i]);
-    if (dims[3] % 2) return fail(LYM_ERR_INVALID, "Lt = %d must be even for the even-odd device layout", dims[3]);
+    if (LYM_T_SLOW ? dims[0] % 2 : dims[3] % 2)
+        return fail(LYM_ERR_INVALID, "%s = %d must be even for the even-odd device layout", LYM_T_SLOW ? "Lx" : "Lt",
+                    LYM_T_SLOW ? dims[0] : dims[3]);
     if (rep != LYM_REP_SU3_FUND && rep != LYM_REP_SU2_ADJ) return fail(LYM_ERR_INVALID, "unknown representation %d", rep);
     if (bc_t != 1 && bc_t != -1) return fail(LYM_ERR_INVALID, "bc_t must be +1 or -1");
     long V = (long)dims[0] * dims[1] * dims[2] * dims[3];
@@ -467,6 +470,7 @@ int lym_ctx_create(const int dims[4], int rep, int bc_t, lym_ctx** out) {
     }
     c->V = V;
     c->g.Lth = dims[3] / 2;
+    c->g.Lxh = dims[0] / 2;
     c->g.Vh = (int)(V / 2);
     c->g.stride = (int)(((V / 2) + 63) / 64 * 64);
     c->g.bc_t = bc_t;
@@ -711,6 +715,12 @@ int lym_op_create(lym_ctx* c, int kind, unsigned power, double kappa, double csw
     op->csw = (kind == LYM_OP_CLOVER || kind == LYM_OP_EO_CLOVER) ? csw : 0.0;
     op->gamma5 = gamma5 ? 1 : 0;
     op->diag_dirty = 1;
+    int rc = dmalloc(c, (void**)&op->dev_kappa, 256);
+    if (rc) {
+        delete op;
+        return rc;
+    }
+    lym::launch_set_double(op->dev_kappa, kappa, c->stream);
     *out = op;
     return LYM_OK;
 }
@@ -735,6 +745,7 @@ void lym_op_destroy(lym_op* op) {
     if (op->Ainv) (void)hipFree(op->Ainv);
     if (op->tmp) (void)hipFree(op->tmp);
     if (op->tmp2) (void)hipFree(op->tmp2);
+    if (op->dev_kappa) (void)hipFree(op->dev_kappa);
     delete op;
 }
 
@@ -752,7 +763,10 @@ int lym_op_set_gauge(lym_op* op, const lym_gauge* g) {
 
 int lym_op_set_kappa(lym_op* op, double kappa) {
     if (!op) return fail(LYM_ERR_INVALID, "null operator");
-    if (op->kappa != kappa) op->diag_dirty = 1;
+    if (op->kappa != kappa) {
+        op->diag_dirty = 1;
+        lym::launch_set_double(op->dev_kappa, kappa, op->ctx->stream);
+    }
     op->kappa = kappa;
     return LYM_OK;
 }

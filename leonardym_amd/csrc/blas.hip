@@ -179,6 +179,11 @@ void launch_scale_g5(zd* y, int nparity, int stride, hipStream_t st) {
     long n = (long)nparity * 12 * stride;
     hipLaunchKernelGGL(k_g5, dim3(ew_blocks(n)), dim3(256), 0, st, y, n, stride);
 }
+__global__ void k_set_double(double* p, double v) { *p = v; }
+void launch_set_double(double* p, double v, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_double, dim3(1), dim3(1), 0, st, p, v);
+}
+
 void launch_copy(zd* y, const zd* x, long n, hipStream_t st) {
     hipLaunchKernelGGL(k_copy, dim3(ew_blocks(n)), dim3(256), 0, st, y, x, n);
 }

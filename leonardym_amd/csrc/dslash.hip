@@ -177,7 +177,8 @@ __device__ __forceinline__ void dslash_site(const DslashParams& a, int p, int si
         hop_pair<2, PR, LK>(a, p, site, r, c, acc);
         hop_pair<3, PR, LK>(a, p, site, r, c, acc);
     }
-    const T hk = (T)(-a.kappa * a.hop_sign);
+    const double kap = a.kappa_dev ? *a.kappa_dev : a.kappa;
+    const T hk = (T)(-kap * a.hop_sign);
     cplx<T> v[12];
     if constexpr (DIAG == 0) {
 #pragma unroll

@@ -20,7 +20,7 @@ __global__ void k_spinor_to_eo(Geom g, const zd* __restrict__ aos, zd* __restric
     int c[4];
     lex_coords(g, lex, c);
     int p = (c[0] + c[1] + c[2] + c[3]) & 1;
-    int idx = (int)(lex >> 1);
+    int idx = eo_index(g, c[0], c[1], c[2], c[3]);
     zd* o = eo + (size_t)p * 12 * g.stride + idx;
 #pragma unroll
     for (int k = 0; k < 12; ++k) o[(size_t)k * g.stride] = aos[lex * 12 + k];
@@ -32,7 +32,7 @@ __global__ void k_spinor_from_eo(Geom g, const zd* __restrict__ eo, zd* __restri
     int c[4];
     lex_coords(g, lex, c);
     int p = (c[0] + c[1] + c[2] + c[3]) & 1;
-    int idx = (int)(lex >> 1);
+    int idx = eo_index(g, c[0], c[1], c[2], c[3]);
     const zd* s = eo + (size_t)p * 12 * g.stride + idx;
 #pragma unroll
     for (int k = 0; k < 12; ++k) aos[lex * 12 + k] = s[(size_t)k * g.stride];
@@ -46,7 +46,7 @@ __global__ void k_links_to_eo(Geom g, const E* __restrict__ aos, int ncomp, int 
     int c[4];
     lex_coords(g, lex, c);
     int p = (c[0] + c[1] + c[2] + c[3]) & 1;
-    int idx = (int)(lex >> 1);
+    int idx = eo_index(g, c[0], c[1], c[2], c[3]);
     for (int mu = 0; mu < 4; ++mu) {
         const E* s = aos + (lex * 4 + mu) * 9;
         E* o = eo + (size_t)((p * 4 + mu) * ncomp) * g.stride + idx;

@@ -28,9 +28,19 @@
 
 namespace lym {
 
+// Device site order.  0: t fastest (the reference's lexicographic order; the
+// even-odd split needs Lt even).  1: t slowest, x fastest (needs Lx even): the
+// contiguous site range each XCD works on is then a slab of whole t-slices,
+// the longest extent of the usual T > L lattices, so a smaller share of each
+// slab's neighbours lies on other XCDs.
+#ifndef LYM_T_SLOW
+#define LYM_T_SLOW 0
+#endif
+
 struct Geom {
     int L[4];    // Lx, Ly, Lz, Lt
     int Lth;     // Lt / 2
+    int Lxh;     // Lx / 2
     int Vh;      // sites per parity
     int stride;  // component stride (Vh rounded up to 64)
     int bc_t;    // +1 periodic, -1 antiperiodic in time
@@ -86,6 +96,23 @@ __host__ __device__ __forceinline__ cplx<T> mulph(cplx<T> z) {
 }
 
 // ---------------------------------------------------------------- geometry
+#if LYM_T_SLOW
+// device order t slowest, x fastest: dev = ((t*Lz + z)*Ly + y)*Lx + x, eo = dev >> 1
+__device__ __forceinline__ void eo_coords(const Geom& g, int p, int idx, int c[4]) {
+    int xh = idx % g.Lxh;
+    int r = idx / g.Lxh;
+    c[1] = r % g.L[1];
+    r /= g.L[1];
+    c[2] = r % g.L[2];
+    c[3] = r / g.L[2];
+    c[0] = 2 * xh + ((c[1] + c[2] + c[3] + p) & 1);
+}
+
+__device__ __forceinline__ int eo_index(const Geom& g, int x, int y, int z, int t) {
+    return ((t * g.L[2] + z) * g.L[1] + y) * g.Lxh + (x >> 1);
+}
+#else
+// device order t fastest (the reference's lexicographic order), eo = lex >> 1
 __device__ __forceinline__ void eo_coords(const Geom& g, int p, int idx, int c[4]) {
     int th = idx % g.Lth;
     int r = idx / g.Lth;
@@ -99,6 +126,7 @@ __device__ __forceinline__ void eo_coords(const Geom& g, int p, int idx, int c[4
 __device__ __forceinline__ int eo_index(const Geom& g, int x, int y, int z, int t) {
     return ((x * g.L[1] + y) * g.L[2] + z) * g.Lth + (t >> 1);
 }
+#endif
 
 // XCD-aware block order: MI355X deals workgroups round-robin over its 8 XCDs;
 // remapping gives each XCD a contiguous range of sites, so neighbour spinors
@@ -147,6 +175,7 @@ struct lym_op {
     int kind;          // LYM_OP_*
     unsigned power;
     double kappa, csw;
+    double* dev_kappa;  // kappa in device memory, read by the stencil (captured launches stay valid across kappa changes)
     int gamma5;
     const lym_gauge* gauge;
     double2* A;        // [2][36][stride] site-diagonal blocks of D (1 + kappa csw gamma5 B)

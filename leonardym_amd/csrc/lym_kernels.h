@@ -29,6 +29,7 @@ struct DslashParams {
     const void* Ainv;      // [36][stride] inverse blocks (even sites)
     const float* Ainv_sc;  // [stride] (FP16)
     double kappa;
+    const double* kappa_dev;  // when set, kappa is read from device memory instead
     double hop_sign;
     zd alpha;
     int gamma5;
@@ -72,6 +73,7 @@ void launch_axpy(zd* y, zd a, const zd* x, long n, hipStream_t st);         // y
 void launch_xpay(zd* y, const zd* x, zd a, long n, hipStream_t st);         // y = x + a y
 void launch_scale_g5(zd* y, int nparity, int stride, hipStream_t st);       // lower spins negated
 void launch_copy(zd* y, const zd* x, long n, hipStream_t st);
+void launch_set_double(double* p, double v, hipStream_t st);                // *p = v, ordered on st
 void launch_sub(zd* out, const zd* a, const zd* b, long n, hipStream_t st); // out = a - b
 
 // ---------------------------------------------------------------- solver state machines

@@ -188,9 +188,10 @@ struct Key {
     }
 };
 
-// what the operator's launches capture
+// what the operator's launches capture (kappa is read from device memory, so a
+// kappa change keeps the captures valid; csw enters only through the blocks)
 void key_op(Key& k, const lym_op* op) {
-    k << op << op->ctx << op->kind << op->power << op->kappa << op->csw << op->gamma5 << op->gauge << op->gauge->U
+    k << op << op->ctx << op->kind << op->power << op->dev_kappa << (op->csw != 0.0) << op->gamma5 << op->gauge << op->gauge->U
       << op->gauge->compression << op->A << op->Ainv << op->tmp << op->tmp2 << op->sloppy_prec << op->Us << op->As
       << op->As_sc << op->Ainvs << op->Ainvs_sc << op->ctx->g.stride << op->ctx->g.Vh;
 }

@@ -370,3 +370,30 @@ def test_graph_replay_equals_eager(case, kind, op_kind, power, monkeypatch):
     for (xe, se, ce), (xg, sg, cg) in zip(eager, graph):
         assert ce and cg and se == sg
         assert np.array_equal(xe, xg)
+
+
+def test_graph_reuse_across_kappa_change(case, monkeypatch):
+    """kappa lives in device memory, so a captured iteration chunk is replayed
+    after set_kappa; results equal the eager ones and the oracle's at the new kappa."""
+    b = case.gauss()
+    eps = (1e-11 * np.linalg.norm(b)) ** 2
+    k2 = case.kappa * 0.9
+
+    def run():
+        op = case.op(lym.OP_CLOVER)
+        s = lym.Solver(case.ctx, lym.SOLVER_PRECOND_MIXED_CG, precision=eps, max_steps=5000, sloppy=lym.PREC_HALF)
+        X1, X2 = case.spinor(), case.spinor()
+        s.solve(op, case.spinor(b), X1)
+        op.set_kappa(k2)
+        info = s.solve(op, case.spinor(b), X2)
+        return X1.download(), X2.download(), info
+
+    monkeypatch.setenv("LYM_GRAPHS", "0")
+    e1, e2, ie = run()
+    monkeypatch.delenv("LYM_GRAPHS")
+    g1, g2, ig = run()
+    assert np.array_equal(e1, g1) and np.array_equal(e2, g2) and ie.steps == ig.steps
+    fld = OracleField(case.dims, case.orc.U, kappa=k2, csw=case.csw, need_clover=True)
+    xo, oinfo, ok = fld.solve("precond_bicgstab", b, eps, kind=K_CLOVER, max_steps=3000)
+    assert ok
+    assert np.linalg.norm((g2 - xo).ravel()) / np.linalg.norm(xo.ravel()) < 1e-9
