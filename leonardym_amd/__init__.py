@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liblym.so")
+# LYM_LIB selects another build of the library (e.g. a variant under test)
+LIB_PATH = os.environ.get("LYM_LIB") or os.path.join(HERE, "liblym.so")
 
 REP_SU3_FUND, REP_SU2_ADJ = 0, 1
 OP_WILSON, OP_BASIC_WILSON, OP_CLOVER, OP_EO_CLOVER = 0, 1, 2, 3
