@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # LYM_LIB selects another build of the library (e.g. a variant under test)
 LIB_PATH = os.environ.get("LYM_LIB") or os.path.join(HERE, "liblym.so")
 
+OK, ERR_INVALID, ERR_DEVICE, ERR_NOMEM, ERR_UNSUPPORTED, ERR_SINGULAR = 0, 1, 2, 3, 4, 5
 REP_SU3_FUND, REP_SU2_ADJ = 0, 1
 OP_WILSON, OP_BASIC_WILSON, OP_CLOVER, OP_EO_CLOVER = 0, 1, 2, 3
 SOLVER_CG, SOLVER_BICGSTAB, SOLVER_PRECOND_BICGSTAB = 0, 1, 2

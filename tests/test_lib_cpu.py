@@ -112,3 +112,26 @@ def test_invalid_arguments_are_errors():
         lym.point_source((2, 2, 2, 2), (0, 0, 0, 0), 4, 0)
     with pytest.raises(lym.LymError):
         lym.point_source((2, 2, 2, 2), (0, 0, 0, 2), 0, 0)
+
+
+def test_argument_errors_are_codes_before_any_device_work():
+    """Bad arguments come back as LYM_ERR_INVALID with a message, checked
+    before the library touches a device (so they hold on a CPU-only host)."""
+    import ctypes
+    L = lym.lib()
+    out = ctypes.c_void_p()
+    for dims, what in (((4, 4, 4, 3), b"even"), ((4, 4, 4, 1), b">= 2")):
+        d = (ctypes.c_int * 4)(*dims)
+        assert L.lym_ctx_create(d, lym.REP_SU3_FUND, -1, ctypes.byref(out)) == lym.ERR_INVALID
+        assert what in L.lym_last_error()
+    d = (ctypes.c_int * 4)(4, 4, 4, 4)
+    assert L.lym_ctx_create(d, 7, -1, ctypes.byref(out)) == lym.ERR_INVALID
+    assert L.lym_ctx_create(d, lym.REP_SU3_FUND, 0, ctypes.byref(out)) == lym.ERR_INVALID
+    assert L.lym_ctx_create(None, lym.REP_SU3_FUND, -1, ctypes.byref(out)) == lym.ERR_INVALID
+    assert L.lym_op_set_kappa(None, 0.1) == lym.ERR_INVALID
+    assert L.lym_solver_set_rhs_block(None, 4) == lym.ERR_INVALID
+    assert L.lym_solver_solve_batch(None, None, 0, None, None, None, None) == lym.ERR_INVALID
+    assert L.lym_server_create(0, 0, 4, ctypes.byref(out)) == lym.ERR_INVALID
+    assert L.lym_server_create(4, 0, 17, ctypes.byref(out)) == lym.ERR_INVALID
+    assert L.lym_server_submit(None, 0, 0.1, 1e-8, None, None, None) == lym.ERR_INVALID
+    assert L.lym_host_alloc(0, ctypes.byref(out)) == lym.OK and not out.value
