@@ -26,8 +26,14 @@
 #endif
 // both parities of a block range run on the same XCD (links and spinors of a
 // region are then fetched into one L2 for the even and the odd sites)
+// -1: automatic -- one sweeping workgroup per CU for launches of at most
+// LYM_SWEEP_MAX_BLOCKS workgroups (small lattices, where it was ~4 % faster),
+// all workgroups at once above that (large lattices, where sweeping was slower)
 #ifndef LYM_GRID_PER_CU
-#define LYM_GRID_PER_CU 0
+#define LYM_GRID_PER_CU -1
+#endif
+#ifndef LYM_SWEEP_MAX_BLOCKS
+#define LYM_SWEEP_MAX_BLOCKS 1024
 #endif
 // Full-lattice applies: odd-parity sites take each direction's backward hop
 // before its forward hop, so the two reads of a link U_mu(y) -- by y (forward)
@@ -291,7 +297,8 @@ void launch_dslash(DslashParams a, int link_kind, int diag, bool inv, hipStream_
     a.nblocks_per_parity = (a.g.Vh + a.spb - 1) / a.spb;
     int nblocks = a.nblocks_per_parity * (a.parity >= 0 ? 1 : 2);
     // workgroups in flight: LYM_GRID_PER_CU per CU of the 256 CUs, sweeping
-    if (LYM_GRID_PER_CU > 0 && nblocks > 256 * LYM_GRID_PER_CU) nblocks = 256 * LYM_GRID_PER_CU;
+    const int gpc = LYM_GRID_PER_CU >= 0 ? LYM_GRID_PER_CU : (nblocks <= LYM_SWEEP_MAX_BLOCKS ? 1 : 0);
+    if (gpc > 0 && nblocks > 256 * gpc) nblocks = 256 * gpc;
     switch (a.prec) {
         case 0: launch_pr<0>(a, link_kind, diag, inv, nblocks, st); break;
         case 1: launch_pr<1>(a, link_kind, diag, inv, nblocks, st); break;
