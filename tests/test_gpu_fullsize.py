@@ -49,10 +49,9 @@ def test_gamma5_hermiticity_and_square(big):
     Qa, Qb = big.spinor(), big.spinor()
     Q.multiply(Qa, a)
     Q.multiply(Qb, b)
-    l_re, l_im = b.dot(Qa)
-    r_re, r_im = Qb.dot(a)
+    lhs, rhs = b.dot(Qa), Qb.dot(a)
     scale = np.sqrt(a.squared_norm() * Qb.squared_norm())
-    assert abs(l_re - r_re) < 1e-12 * scale and abs(l_im - r_im) < 1e-12 * scale
+    assert abs(lhs - rhs) < 1e-12 * scale
     QQa, Q2a = big.spinor(), big.spinor()
     Q.multiply(QQa, Qa)
     big.op(power=2).multiply(Q2a, a)
