@@ -76,12 +76,20 @@ __device__ __forceinline__ long bidx(int stride, int nr, long j, int k, int r) {
     return ((p * 12 + k) * stride + i) * nr + r;
 }
 
+// parity base of a low-precision nr-field and the lane element of (site j, rhs r)
+template <int PR, int NR>
+__device__ __forceinline__ long bs_parity_off(int stride, long j) {
+    return (j / stride) * 12L * stride * NR;
+}
+
 template <int PR, int NR>
 __device__ __forceinline__ void bs_load(const typename Prec<PR>::S* d, const float* sc, int stride, long j, int r,
                                         cplx<float> s[12]) {
+    typename Prec<PR>::S e[12];
+    load_packed<PR>(d + bs_parity_off<PR, NR>(stride, j), (size_t)stride * NR, (unsigned)((j % stride) * NR + r), e);
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
-        cplx<typename Prec<PR>::T> z = to_c<PR>(d[bidx(stride, NR, j, k, r)]);
+        cplx<typename Prec<PR>::T> z = to_c<PR>(e[k]);
         s[k] = mk((float)z.re, (float)z.im);
     }
     if constexpr (PR == 2) {
@@ -99,12 +107,18 @@ __device__ __forceinline__ void bs_store(typename Prec<PR>::S* d, float* sc, int
 #pragma unroll
         for (int k = 0; k < 12; ++k) m = fmaxf(m, fmaxf(fabsf(s[k].re), fabsf(s[k].im)));
         const float inv = fp16_site_inv(m);
+        typename Prec<PR>::S e[12];
 #pragma unroll
-        for (int k = 0; k < 12; ++k) d[bidx(stride, NR, j, k, r)] = from_c<PR>(inv * s[k]);
+        for (int k = 0; k < 12; ++k) e[k] = from_c<PR>(inv * s[k]);
+        store_packed<PR, false>(d + bs_parity_off<PR, NR>(stride, j), (size_t)stride * NR,
+                                (unsigned)((j % stride) * NR + r), e);
         sc[j * NR + r] = m;
     } else {
+        typename Prec<PR>::S e[12];
 #pragma unroll
-        for (int k = 0; k < 12; ++k) d[bidx(stride, NR, j, k, r)] = from_c<PR>(s[k]);
+        for (int k = 0; k < 12; ++k) e[k] = from_c<PR>(s[k]);
+        store_packed<PR, false>(d + bs_parity_off<PR, NR>(stride, j), (size_t)stride * NR,
+                                (unsigned)((j % stride) * NR + r), e);
     }
 }
 

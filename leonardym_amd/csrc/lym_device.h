@@ -200,6 +200,47 @@ __device__ __forceinline__ void fst(E* ubase, unsigned lane, E e) {
 #endif
 }
 
+// ---------------------------------------------------------------- packed low-precision spinors
+// FP32 and FP16 spinor fields store 16-byte vectors of consecutive
+// spin-colour components: per parity [G][stride * nr] vectors, G = 6 (two
+// float2) or 3 (four half2), lane element (site * nr + rhs).  A lane reads a
+// site's spinor with G 16-byte loads instead of 12 8- or 4-byte ones; the
+// bytes per site are those of the unpacked [12][stride * nr] layout, so the
+// buffers keep their sizes.  FP64 spinors stay [12][stride * nr] double2.
+struct alignas(16) V16 {
+    unsigned w[4];
+};
+template <int PR>
+struct Packed {
+    static constexpr int PER = 16 / (int)sizeof(typename Prec<PR>::S);  // components per vector
+    static constexpr int G = 12 / PER;
+};
+
+// parity base pbase (element type S), group stride ks = stride * nr vectors, lane o
+template <int PR, bool NT = false>
+__device__ __forceinline__ void load_packed(const typename Prec<PR>::S* pbase, size_t ks, unsigned o,
+                                            typename Prec<PR>::S e[12]) {
+    constexpr int PER = Packed<PR>::PER;
+    const V16* b = reinterpret_cast<const V16*>(pbase);
+#pragma unroll
+    for (int g = 0; g < Packed<PR>::G; ++g) {
+        V16 v = fld<V16, NT>(b + g * ks, o);
+        __builtin_memcpy(e + g * PER, &v, 16);
+    }
+}
+template <int PR, bool NT = true>
+__device__ __forceinline__ void store_packed(typename Prec<PR>::S* pbase, size_t ks, unsigned o,
+                                             const typename Prec<PR>::S e[12]) {
+    constexpr int PER = Packed<PR>::PER;
+    V16* b = reinterpret_cast<V16*>(pbase);
+#pragma unroll
+    for (int g = 0; g < Packed<PR>::G; ++g) {
+        V16 v;
+        __builtin_memcpy(&v, e + g * PER, 16);
+        fst<V16, NT>(b + g * ks, o, v);
+    }
+}
+
 // ---------------------------------------------------------------- spinors
 // One parity of a (multi-RHS) spinor field.
 template <int PR>
@@ -218,10 +259,14 @@ __device__ __forceinline__ void load_spinor(SpinorRef<PR> f, int stride, int nr,
                                             cplx<typename Prec<PR>::T> s[12]) {
     const unsigned o = (unsigned)site * (unsigned)nr + (unsigned)r;
     const size_t ks = (size_t)stride * nr;
+    if constexpr (PR == 0) {
 #pragma unroll
-    for (int k = 0; k < 12; ++k) {
-        typename Prec<PR>::S e = fld<typename Prec<PR>::S, NT>(f.base + k * ks, o);
-        s[k] = to_c<PR>(e);
+        for (int k = 0; k < 12; ++k) s[k] = to_c<PR>(fld<typename Prec<PR>::S, NT>(f.base + k * ks, o));
+    } else {
+        typename Prec<PR>::S e[12];
+        load_packed<PR, NT>(f.base, ks, o, e);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) s[k] = to_c<PR>(e[k]);
     }
     if constexpr (PR == 2) {
         const float sc = fld<float>(f.scale, o);
@@ -253,9 +298,16 @@ __device__ __forceinline__ void store_spinor(SpinorOut<PR> f, int stride, int nr
 #pragma unroll
         for (int k = 0; k < 12; ++k) m = fmaxf(m, fmaxf(fabsf(v[k].re), fabsf(v[k].im)));
         const float inv = fp16_site_inv(m);
+        typename Prec<PR>::S e[12];
 #pragma unroll
-        for (int k = 0; k < 12; ++k) fst(f.base + k * ks, o, from_c<PR>(inv * v[k]));
+        for (int k = 0; k < 12; ++k) e[k] = from_c<PR>(inv * v[k]);
+        store_packed<PR>(f.base, ks, o, e);
         fst<float, false>(f.scale, o, m);
+    } else if constexpr (PR == 1) {
+        typename Prec<PR>::S e[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) e[k] = from_c<PR>(v[k]);
+        store_packed<PR>(f.base, ks, o, e);
     } else {
 #pragma unroll
         for (int k = 0; k < 12; ++k) fst(f.base + k * ks, o, v[k]);

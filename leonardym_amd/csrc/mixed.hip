@@ -78,10 +78,11 @@ struct SVec {
 template <int PR>
 __device__ __forceinline__ void sv_load(const SVec<PR>& v, int stride, long j, cplx<float> s[12]) {
     const long p = j / stride, i = j % stride;
-    const typename Prec<PR>::S* b = v.d + p * 12 * stride + i;
+    typename Prec<PR>::S e[12];
+    load_packed<PR>(v.d + p * 12 * stride, (size_t)stride, (unsigned)i, e);
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
-        cplx<typename Prec<PR>::T> z = to_c<PR>(b[(long)k * stride]);
+        cplx<typename Prec<PR>::T> z = to_c<PR>(e[k]);
         s[k] = mk((float)z.re, (float)z.im);
     }
     if constexpr (PR == 2) {
@@ -94,18 +95,21 @@ __device__ __forceinline__ void sv_load(const SVec<PR>& v, int stride, long j, c
 template <int PR>
 __device__ __forceinline__ void sv_store(const SVec<PR>& v, int stride, long j, const cplx<float> s[12]) {
     const long p = j / stride, i = j % stride;
-    typename Prec<PR>::S* b = v.d + p * 12 * stride + i;
+    typename Prec<PR>::S* b = v.d + p * 12 * stride;
+    typename Prec<PR>::S e[12];
     if constexpr (PR == 2) {
         float m = 0.f;
 #pragma unroll
         for (int k = 0; k < 12; ++k) m = fmaxf(m, fmaxf(fabsf(s[k].re), fabsf(s[k].im)));
         const float inv = fp16_site_inv(m);
 #pragma unroll
-        for (int k = 0; k < 12; ++k) b[(long)k * stride] = from_c<PR>(inv * s[k]);
+        for (int k = 0; k < 12; ++k) e[k] = from_c<PR>(inv * s[k]);
+        store_packed<PR, false>(b, (size_t)stride, (unsigned)i, e);
         v.sc[j] = m;
     } else {
 #pragma unroll
-        for (int k = 0; k < 12; ++k) b[(long)k * stride] = from_c<PR>(s[k]);
+        for (int k = 0; k < 12; ++k) e[k] = from_c<PR>(s[k]);
+        store_packed<PR, false>(b, (size_t)stride, (unsigned)i, e);
     }
 }
 
