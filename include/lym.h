@@ -167,6 +167,11 @@ int lym_solver_set_even_odd(lym_solver* s, int use_even_odd);   /* setUseEvenOdd
  * which the sloppy residual must fall before an FP64 reliable update (default 0.1) */
 int lym_solver_set_sloppy_precision(lym_solver* s, int prec);
 int lym_solver_set_reliable_delta(lym_solver* s, double delta);
+/* BiConjugateGradient's shifted overload (BiConjugateGradient.cpp:510-690):
+ * plain BiCGStab (LYM_SOLVER_BICGSTAB) on Op + shift, every apply being
+ * multiplyAdd(out, v, v, shift).  Full-lattice operators only (the even-odd
+ * operator has no multiplyAdd); 0 + 0i restores the plain solver. */
+int lym_solver_set_shift(lym_solver* s, double shift_re, double shift_im);
 /* solve(dirac, source, solution, initial_guess) -- x0 may be NULL */
 int lym_solver_solve(lym_solver* s, lym_op* op, const lym_spinor* source, lym_spinor* solution,
                      const lym_spinor* initial_guess, lym_solve_info* info);

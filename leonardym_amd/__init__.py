@@ -86,6 +86,7 @@ SIGNATURES = {
     "lym_solver_set_even_odd": (_I, [_P, _I]),
     "lym_solver_set_sloppy_precision": (_I, [_P, _I]),
     "lym_solver_set_reliable_delta": (_I, [_P, _D]),
+    "lym_solver_set_shift": (_I, [_P, _D, _D]),
     "lym_solver_solve": (_I, [_P, _P, _P, _P, _P, ctypes.POINTER(SolveInfo)]),
     "lym_solver_solve_batch": (_I, [_P, _P, _U, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_D),
                                     ctypes.POINTER(SolveInfo)]),
@@ -382,7 +383,8 @@ class Operator:
 
 
 class Solver:
-    def __init__(self, ctx, kind, precision=None, max_steps=None, even_odd=True, sloppy=None, delta=None):
+    def __init__(self, ctx, kind, precision=None, max_steps=None, even_odd=True, sloppy=None, delta=None,
+                 shift=None):
         self.ctx = ctx
         h = ctypes.c_void_p()
         _check(lib().lym_solver_create(ctx.h, kind, ctypes.byref(h)))
@@ -396,6 +398,8 @@ class Solver:
         if max_steps is not None:
             _check(lib().lym_solver_set_max_steps(self.h, max_steps))
         _check(lib().lym_solver_set_even_odd(self.h, int(even_odd)))
+        if shift is not None:
+            _check(lib().lym_solver_set_shift(self.h, complex(shift).real, complex(shift).imag))
 
     def solve(self, op, source, solution, guess=None):
         info = SolveInfo()

@@ -297,6 +297,9 @@ void apply_schur(lym_op* op, zd* out_o, const zd* in_o, int gamma5, const int* s
 void apply_full(lym_op* op, zd* out, const zd* in, int gamma5, const int* skip) {
     apply_full1(op, out, in, nullptr, lym::mk(0.0, 0.0), gamma5, skip);
 }
+void apply_full_add(lym_op* op, zd* out, const zd* in, const zd* v2, zd alpha, int gamma5, const int* skip) {
+    apply_full1(op, out, in, v2, alpha, gamma5, skip);
+}
 int link_kind_of(const lym_gauge* g) { return link_kind(g); }
 
 int op_sloppy(lym_op* op, int prec) { return op_sloppy_data(op, prec); }

@@ -34,6 +34,7 @@ int fail(int code, const char* fmt, ...);
 int op_ready(lym_op* op, bool force_blocks);
 void apply_schur(lym_op* op, zd* out_o, const zd* in_o, int gamma5, const int* skip);
 void apply_full(lym_op* op, zd* out, const zd* in, int gamma5, const int* skip);
+void apply_full_add(lym_op* op, zd* out, const zd* in, const zd* v2, zd alpha, int gamma5, const int* skip);
 int link_kind_of(const lym_gauge* g);
 int op_sloppy(lym_op* op, int prec);
 void apply_full_sloppy(lym_op* op, int prec, void* out, float* out_sc, const void* in, const float* in_sc, int gamma5,
@@ -84,6 +85,8 @@ struct lym_solver {
     zd** bptr_dev;
     zd** bptr_host;
     zd* bzero;              // zero source / dummy destination for padded lanes
+    // plain BiCGStab on A + shift (the reference's shifted overload)
+    double shift_re, shift_im;
     // captured iteration chunks (see queue_iterations)
     uint64_t graph_key[4];
     hipGraphExec_t graph_exec[4];
@@ -111,17 +114,23 @@ struct LinOp {
     int gamma5;
     unsigned power;
     zd* scratch;  // one vector of the map's domain
+    zd shift{0.0, 0.0};  // full-lattice maps only: apply A + shift (fused into the last stencil launch)
+    bool shifted() const { return shift.re != 0.0 || shift.im != 0.0; }
+    void last(zd* out, const zd* in, const zd* v, const int* skip) const {
+        if (shifted()) lym_internal::apply_full_add(op, out, in, v, shift, gamma5, skip);
+        else lym_internal::apply_full(op, out, in, gamma5, skip);
+    }
     void apply(zd* out, const zd* in, const int* skip) const {
         if (power == 1) {
             if (half) lym_internal::apply_schur(op, out, in, gamma5, skip);
-            else lym_internal::apply_full(op, out, in, gamma5, skip);
+            else last(out, in, in, skip);
         } else {
             if (half) {
                 lym_internal::apply_schur(op, scratch, in, gamma5, skip);
                 lym_internal::apply_schur(op, out, scratch, gamma5, skip);
             } else {
                 lym_internal::apply_full(op, scratch, in, gamma5, skip);
-                lym_internal::apply_full(op, out, scratch, gamma5, skip);
+                last(out, scratch, in, skip);
             }
         }
     }
@@ -331,7 +340,7 @@ int run_bicgstab(lym_solver* s, const LinOp& L, const zd* b, zd* xout, const zd*
     Key key;
     key << 2 << s << n << b;
     key_op(key, L.op);
-    key << L.half << L.gamma5 << L.power << L.scratch;
+    key << L.half << L.gamma5 << L.power << L.scratch << L.shift.re << L.shift.im;
     auto body = [&](unsigned) {
         lym::launch_bicg_rho(S, rh, r, n, c->red_partials, st);
         lym::launch_bicg_p(S, p, r, nu, n, st);
@@ -892,6 +901,13 @@ int lym_solver_set_reliable_delta(lym_solver* s, double delta) {
     return LYM_OK;
 }
 
+int lym_solver_set_shift(lym_solver* s, double re, double im) {
+    if (!s) return fail(LYM_ERR_INVALID, "null solver");
+    s->shift_re = re;
+    s->shift_im = im;
+    return LYM_OK;
+}
+
 int lym_solver_set_even_odd(lym_solver* s, int use) {
     if (!s) return fail(LYM_ERR_INVALID, "null solver");
     s->even_odd = use ? 1 : 0;
@@ -913,6 +929,10 @@ int lym_solver_solve(lym_solver* s, lym_op* op, const lym_spinor* source, lym_sp
     bool eo_driver = precond_kind && s->even_odd && op->power == 1 &&
                      (op->kind == LYM_OP_CLOVER || op->kind == LYM_OP_EO_CLOVER) && c->dims[0] % 2 == 0 &&
                      c->dims[1] % 2 == 0 && c->dims[2] % 2 == 0 && c->dims[3] % 2 == 0;
+    if ((s->shift_re != 0.0 || s->shift_im != 0.0) &&
+        (s->kind != LYM_SOLVER_BICGSTAB || op->kind == LYM_OP_EO_CLOVER))
+        return fail(LYM_ERR_UNSUPPORTED, "a shift is supported by plain BiCGStab on full-lattice operators only "
+                                         "(the even-odd operator has no multiplyAdd)");
     int rc = lym_internal::op_ready(op, eo_driver);
     if (rc) return rc;
     rc = ensure_work(s);
@@ -987,6 +1007,7 @@ int lym_solver_solve(lym_solver* s, lym_op* op, const lym_spinor* source, lym_sp
         } else if (s->kind == LYM_SOLVER_CG || s->kind == LYM_SOLVER_PRECOND_CG) {
             rc = run_cg(s, L, bb, xx, g0, n, info);
         } else {
+            L.shift = lym::mk(s->shift_re, s->shift_im);
             rc = run_bicgstab(s, L, bb, xx, g0, n, half ? read_norm2(c, b, half_n(c)) : 0.0, info);
         }
         if (rc) return rc;

@@ -744,11 +744,32 @@ int orc_cg(const orc_op_desc* d, const double* bsrc, double* xout, const double*
     return ok;
 }
 
-// BiConjugateGradient::solve, plain BiCGStab (BiConjugateGradient.cpp:321-507).
+// BiConjugateGradient::solve, plain BiCGStab (BiConjugateGradient.cpp:321-507),
+// and the shifted overload (:510-690), which is the same iteration on
+// A + shift (every A v there is dirac->multiplyAdd(out, v, v, shift)).
+static int bicgstab_impl(const orc_op_desc* d, const double* bsrc, double* xout, const double* x0,
+                         double precision, unsigned max_steps, cd shift, orc_solve_info* info);
+
 int orc_bicgstab(const orc_op_desc* d, const double* bsrc, double* xout, const double* x0, double precision,
                  unsigned max_steps, orc_solve_info* info) {
-    Op op = make_op(d->dims, d->kind, d->power, d->kappa, d->csw, d->gamma5, d->links, d->F, d->inv);
-    const long n = op.g->V * 12;
+    return bicgstab_impl(d, bsrc, xout, x0, precision, max_steps, cd(0, 0), info);
+}
+
+int orc_bicgstab_shift(const orc_op_desc* d, const double* bsrc, double* xout, const double* x0, double precision,
+                       unsigned max_steps, double shift_re, double shift_im, orc_solve_info* info) {
+    return bicgstab_impl(d, bsrc, xout, x0, precision, max_steps, cd(shift_re, shift_im), info);
+}
+
+static int bicgstab_impl(const orc_op_desc* d, const double* bsrc, double* xout, const double* x0,
+                         double precision, unsigned max_steps, cd shift, orc_solve_info* info) {
+    Op op0 = make_op(d->dims, d->kind, d->power, d->kappa, d->csw, d->gamma5, d->links, d->F, d->inv);
+    const long n = op0.g->V * 12;
+    auto apply_op = [&](const Op& o, cd* out, const cd* in, std::vector<cd>& scr) {
+        ::apply_op(o, out, in, scr);
+        if (shift != cd(0, 0))
+            for (long i = 0; i < n; ++i) out[i] += shift * in[i];
+    };
+    const Op& op = op0;
     const cd* b = reinterpret_cast<const cd*>(bsrc);
     std::vector<cd> x(n), r(n), rh(n), p(n), nu(n), s(n), t(n), scratch;
     if (x0) {

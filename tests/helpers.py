@@ -73,6 +73,9 @@ def oracle():
         f = getattr(lib, name)
         f.argtypes = [ctypes.POINTER(OpDesc), P, P, P, ctypes.c_double, ctypes.c_uint, ctypes.POINTER(SolveInfo)]
         f.restype = ctypes.c_int
+    lib.orc_bicgstab_shift.argtypes = [ctypes.POINTER(OpDesc), P, P, P, ctypes.c_double, ctypes.c_uint,
+                                       ctypes.c_double, ctypes.c_double, ctypes.POINTER(SolveInfo)]
+    lib.orc_bicgstab_shift.restype = ctypes.c_int
     lib.orc_num_threads.restype = ctypes.c_int
     _oracle = lib
     return lib
@@ -141,10 +144,16 @@ class OracleField:
                               float(np.imag(alpha)))
         return out
 
-    def solve(self, which, b, eps, max_steps=5000, kind=K_CLOVER, power=1, gamma5=True, x0=None):
+    def solve(self, which, b, eps, max_steps=5000, kind=K_CLOVER, power=1, gamma5=True, x0=None, shift=None):
         d = self.desc(kind, power, gamma5)
         x = np.zeros_like(b)
         info = SolveInfo()
+        if shift is not None:  # BiConjugateGradient.cpp:510-690, A + shift
+            assert which == "bicgstab"
+            ok = oracle().orc_bicgstab_shift(ctypes.byref(d), _p(np.ascontiguousarray(b)), _p(x), _p(x0), eps,
+                                             max_steps, complex(shift).real, complex(shift).imag,
+                                             ctypes.byref(info))
+            return x, info, ok
         fn = getattr(oracle(), "orc_" + which)
         ok = fn(ctypes.byref(d), _p(np.ascontiguousarray(b)), _p(x), _p(x0), eps, max_steps, ctypes.byref(info))
         return x, info, ok

@@ -397,3 +397,32 @@ def test_graph_reuse_across_kappa_change(case, monkeypatch):
     xo, oinfo, ok = fld.solve("precond_bicgstab", b, eps, kind=K_CLOVER, max_steps=3000)
     assert ok
     assert np.linalg.norm((g2 - xo).ravel()) / np.linalg.norm(xo.ravel()) < 1e-9
+
+
+@pytest.mark.parametrize("kind,power,gamma5", [(lym.OP_CLOVER, 1, False), (lym.OP_CLOVER, 1, True),
+                                               (lym.OP_WILSON, 2, True)])
+def test_shifted_bicgstab(case, kind, power, gamma5):
+    """BiConjugateGradient's shifted overload (BiConjugateGradient.cpp:510-690):
+    plain BiCGStab on Op + shift, against the oracle's, with (Op + shift) x = b."""
+    b = case.gauss()
+    shift = 0.05 - 0.02j
+    eps = (1e-11 * np.linalg.norm(b)) ** 2
+    s = lym.Solver(case.ctx, lym.SOLVER_BICGSTAB, precision=eps, max_steps=3000, shift=shift)
+    op = case.op(kind, power=power, gamma5=gamma5)
+    X = case.spinor()
+    info = s.solve(op, case.spinor(b), X)
+    assert info.converged
+    x = X.download()
+    okind = K_CLOVER if kind == lym.OP_CLOVER else K_WILSON
+    Ax = case.orc.apply(okind, x, power=power, gamma5=gamma5)
+    assert np.linalg.norm((Ax + shift * x - b).ravel()) / np.linalg.norm(b.ravel()) <= 1e-10
+    xo, oinfo, ok = case.orc.solve("bicgstab", b, eps, kind=okind, power=power, gamma5=gamma5, max_steps=3000,
+                                   shift=shift)
+    assert ok
+    assert np.linalg.norm((x - xo).ravel()) / np.linalg.norm(xo.ravel()) < 1e-9
+    # other solver kinds and the even-odd operator refuse a shift
+    with pytest.raises(lym.LymError):
+        lym.Solver(case.ctx, lym.SOLVER_CG, shift=0.1).solve(case.op(lym.OP_CLOVER, power=2), case.spinor(b),
+                                                             case.spinor())
+    with pytest.raises(lym.LymError):
+        s.solve(case.op(lym.OP_EO_CLOVER), case.spinor(b), case.spinor())

@@ -418,6 +418,26 @@ def _np_bicgstab(A, b, x0, eps, max_steps):
     return x, max_steps
 
 
+@pytest.mark.parametrize("shift", [0.3 - 0.2j, -0.05])
+def test_shifted_bicgstab_matches_dense(dense_case, shift):
+    """Shifted BiCGStab (BiConjugateGradient.cpp:510-690) solves (D + shift) x = b;
+    its iterates after fixed steps are the textbook recurrences on D + shift,
+    and shift 0 is the plain overload (to the run-to-run rounding of the
+    oracle's threaded sums)."""
+    fld, D, b, x = dense_case
+    A = D + shift * np.eye(D.shape[0])
+    xs = np.linalg.solve(A, b.ravel())
+    eps = (1e-12 * np.linalg.norm(b)) ** 2
+    sol, info, ok = fld.solve("bicgstab", b, eps, kind=K_CLOVER, power=1, gamma5=False, shift=shift)
+    assert ok and _relerr(sol.ravel(), xs) < 1e-9
+    sol, info, ok = fld.solve("bicgstab", b, 1e-300, kind=K_CLOVER, power=1, gamma5=False, max_steps=3, shift=shift)
+    xn, _ = _np_bicgstab(A, b.ravel(), b.ravel().copy(), 1e-300, 3)
+    assert not ok and _relerr(sol.ravel(), xn) < 1e-12
+    s0, i0, ok0 = fld.solve("bicgstab", b, eps, kind=K_CLOVER, power=1, gamma5=False, shift=0.0)
+    s1, i1, ok1 = fld.solve("bicgstab", b, eps, kind=K_CLOVER, power=1, gamma5=False)
+    assert _relerr(s0, s1) < 1e-13 and abs(int(i0.steps) - int(i1.steps)) <= 1
+
+
 def test_cg_iteration_path_matches_textbook(dense_case):
     """Pins the CG recurrences themselves (alpha, beta, x0 = b), which the
     converged solution alone does not see."""
