@@ -35,6 +35,11 @@
 #ifndef LYM_SWEEP_MAX_BLOCKS
 #define LYM_SWEEP_MAX_BLOCKS 1024
 #endif
+// 1: each XCD works through its own contiguous range of site blocks;
+// 0: all XCDs advance together through the lattice in block order
+#ifndef LYM_XCD_MAP
+#define LYM_XCD_MAP 1
+#endif
 // Full-lattice applies: odd-parity sites take each direction's backward hop
 // before its forward hop, so the two reads of a link U_mu(y) -- by y (forward)
 // and by y + mu (backward), sites of opposite parity -- happen at the same
@@ -238,12 +243,19 @@ template <int PR, int LK, int DIAG, bool INV>
 __global__ __launch_bounds__(LYM_DSLASH_THREADS) LYM_OCC void k_dslash(DslashParams a) {
     if (a.skip && *a.skip) return;
     const int nbt = a.nblocks_per_parity * (a.parity >= 0 ? 1 : 2);
+#if LYM_XCD_MAP
     const int nx = 8;
     const int xcd = blockIdx.x % nx, k = blockIdx.x / nx;
     const int nphys = ((int)gridDim.x - xcd + nx - 1) / nx;  // workgroups on this XCD
     const int q = nbt / nx, rr = nbt % nx;
     const int lo = xcd < rr ? xcd * (q + 1) : rr * (q + 1) + (xcd - rr) * q;
     const int hi = lo + (xcd < rr ? q + 1 : q);
+#else
+    // plain order: consecutive workgroups (dealt round-robin over the XCDs) take
+    // consecutive site blocks, so all XCDs advance one front through the lattice
+    // and the neighbour slices they re-read are recent in the shared Infinity Cache
+    const int k = blockIdx.x, nphys = (int)gridDim.x, lo = 0, hi = nbt;
+#endif
     const int ls = (int)threadIdx.x / a.nr, r = (int)threadIdx.x - ls * a.nr;
     if (ls >= a.spb) return;
     if (a.lane_flags && a.lane_flags[r] == a.lane_skip_value) return;
