@@ -39,6 +39,8 @@ struct DslashParams {
     const int* lane_flags; // optional per-RHS device flags: lane r does nothing when
     int lane_skip_value;   //   lane_flags[r] == lane_skip_value (an RHS that is done or needs no update)
     int threads;           // workgroup size (0: LYM_DSLASH_THREADS); a workgroup holds threads / nr sites
+    int xslice_blocks;     // set by the launcher: site blocks per x-slice and parity when an x-slice
+                           // is a whole number of blocks, else 0
 };
 
 // link_kind: 0 complex 18, 1 complex 12, 2 real 9, 3 real 6; diag: 0 none, 1 identity, 2 clover
