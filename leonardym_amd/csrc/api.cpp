@@ -476,13 +476,6 @@ int lym_ctx_create(const int dims[4], int rep, int bc_t, lym_ctx** out) {
     c->g.Lxh = dims[0] / 2;
     c->g.Vh = (int)(V / 2);
     c->g.stride = (int)(((V / 2) + 63) / 64 * 64);
-    // Optional padding of the component stride (sites, rounded up to 64): a
-    // stride of a large power of two puts the component planes of one site
-    // on the same L2 sets.
-    if (const char* pad = std::getenv("LYM_STRIDE_PAD")) {
-        long p = std::atol(pad);
-        if (p > 0 && p <= 65536) c->g.stride += (int)((p + 63) / 64 * 64);
-    }
     c->g.bc_t = bc_t;
     c->rep = rep;
     (void)hipGetDevice(&c->device);

@@ -68,9 +68,11 @@
 #define LYM_PIN_HOPS 2
 #endif
 // the same for FP32/FP16 storage, whose hops need far fewer registers
-// (8: no barrier between hops at all)
+// (8: no barrier between hops at all).  4 (125-150 VGPRs for FP16): the
+// mixed-precision even-odd CG ran 5-14 % faster than with 1; 8 slowed the
+// FP32 stencil down.
 #ifndef LYM_PIN_HOPS_LOWP
-#define LYM_PIN_HOPS_LOWP 1
+#define LYM_PIN_HOPS_LOWP 4
 #endif
 #ifdef LYM_WAVES_PER_EU
 #define LYM_OCC __attribute__((amdgpu_waves_per_eu(LYM_WAVES_PER_EU, LYM_WAVES_PER_EU)))
