@@ -462,7 +462,8 @@ int lym_ctx_create(const int dims[4], int rep, int bc_t, lym_ctx** out) {
     if (rep != LYM_REP_SU3_FUND && rep != LYM_REP_SU2_ADJ) return fail(LYM_ERR_INVALID, "unknown representation %d", rep);
     if (bc_t != 1 && bc_t != -1) return fail(LYM_ERR_INVALID, "bc_t must be +1 or -1");
     long V = (long)dims[0] * dims[1] * dims[2] * dims[3];
-    if (V / 2 > (1L << 30)) return fail(LYM_ERR_INVALID, "lattice too large");
+    // a field component is read through 32-bit buffer offsets below 2 GiB
+    if (V / 2 > (1L << 27) - 64) return fail(LYM_ERR_INVALID, "lattice too large (at most 2^28 sites)");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(LYM_ERR_DEVICE, "no HIP device available");
     lym_ctx* c = new lym_ctx();

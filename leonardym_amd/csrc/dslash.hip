@@ -254,49 +254,67 @@ __device__ __forceinline__ void dslash_site(const DslashParams& a, int p, int si
 // in order, so at any time an XCD works on one compact slab and the
 // neighbour spinors and links of that slab are reused from its L2.  A
 // workgroup holds `spb` sites x `nr` right-hand sides, rhs fastest.
-template <int PR, int LK, int DIAG, bool INV>
-__global__ __launch_bounds__(LYM_DSLASH_THREADS) LYM_OCC void k_dslash(DslashParams a) {
-    if (a.skip && *a.skip) return;
+struct BlockSweep {
+    int lo, hi, k, nphys;
+};
+__device__ __forceinline__ BlockSweep block_sweep(const DslashParams& a) {
     const int nbt = a.nblocks_per_parity * (a.parity >= 0 ? 1 : 2);
+    BlockSweep b;
 #if LYM_XCD_MAP
     const int nx = 8;
-    const int xcd = blockIdx.x % nx, k = blockIdx.x / nx;
-    const int nphys = ((int)gridDim.x - xcd + nx - 1) / nx;  // workgroups on this XCD
+    const int xcd = blockIdx.x % nx;
+    b.k = blockIdx.x / nx;
+    b.nphys = ((int)gridDim.x - xcd + nx - 1) / nx;  // workgroups on this XCD
     const int q = nbt / nx, rr = nbt % nx;
-    const int lo = xcd < rr ? xcd * (q + 1) : rr * (q + 1) + (xcd - rr) * q;
-    const int hi = lo + (xcd < rr ? q + 1 : q);
+    b.lo = xcd < rr ? xcd * (q + 1) : rr * (q + 1) + (xcd - rr) * q;
+    b.hi = b.lo + (xcd < rr ? q + 1 : q);
 #else
     // plain order: consecutive workgroups (dealt round-robin over the XCDs) take
     // consecutive site blocks, so all XCDs advance one front through the lattice
     // and the neighbour slices they re-read are recent in the shared Infinity Cache
-    const int k = blockIdx.x, nphys = (int)gridDim.x, lo = 0, hi = nbt;
+    b.k = blockIdx.x;
+    b.nphys = (int)gridDim.x;
+    b.lo = 0;
+    b.hi = nbt;
 #endif
+    return b;
+}
+
+// logical block lb of the sweep -> parity p and site block bi of that parity
+__device__ __forceinline__ void block_of(const DslashParams& a, const BlockSweep& b, int lb, int& p, int& bi) {
+    if (a.parity >= 0) {
+        p = a.parity;
+        bi = lb;
+    } else {
+        p = lb & 1;
+        bi = lb >> 1;
+    }
+#if LYM_XSWEEP && LYM_XCD_MAP
+    {
+        // the range holds whole x-slices: renumber its blocks x-slice fastest
+        const int W = a.xslice_blocks;
+        const int b0 = a.parity >= 0 ? b.lo : b.lo >> 1;
+        const int D = a.parity >= 0 ? b.hi - b.lo : (b.hi - b.lo) >> 1;
+        const bool whole = a.parity >= 0 || ((b.lo | b.hi) & 1) == 0;
+        if (W > 0 && whole && b0 % W == 0 && D % W == 0 && D > W) {
+            const int u = bi - b0, nxs = D / W;
+            const int j = u / nxs;
+            bi = b0 + (u - j * nxs) * W + j;
+        }
+    }
+#endif
+}
+
+template <int PR, int LK, int DIAG, bool INV>
+__global__ __launch_bounds__(LYM_DSLASH_THREADS) LYM_OCC void k_dslash(DslashParams a) {
+    if (a.skip && *a.skip) return;
+    const BlockSweep b = block_sweep(a);
     const int ls = (int)threadIdx.x / a.nr, r = (int)threadIdx.x - ls * a.nr;
     if (ls >= a.spb) return;
     if (a.lane_flags && a.lane_flags[r] == a.lane_skip_value) return;
-    for (int lb = lo + k; lb < hi; lb += nphys) {
+    for (int lb = b.lo + b.k; lb < b.hi; lb += b.nphys) {
         int p, bi;
-        if (a.parity >= 0) {
-            p = a.parity;
-            bi = lb;
-        } else {
-            p = lb & 1;
-            bi = lb >> 1;
-        }
-#if LYM_XSWEEP && LYM_XCD_MAP
-        {
-            // the range holds whole x-slices: renumber its blocks x-slice fastest
-            const int W = a.xslice_blocks;
-            const int b0 = a.parity >= 0 ? lo : lo >> 1;
-            const int D = a.parity >= 0 ? hi - lo : (hi - lo) >> 1;
-            const bool whole = a.parity >= 0 || ((lo | hi) & 1) == 0;
-            if (W > 0 && whole && b0 % W == 0 && D % W == 0 && D > W) {
-                const int u = bi - b0, nxs = D / W;
-                const int j = u / nxs;
-                bi = b0 + (u - j * nxs) * W + j;
-            }
-        }
-#endif
+        block_of(a, b, lb, p, bi);
         const int site = bi * a.spb + ls;
         if (site < a.g.Vh) dslash_site<PR, LK, DIAG, INV>(a, p, site, r);
     }

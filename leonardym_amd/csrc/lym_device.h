@@ -116,7 +116,7 @@ __device__ __forceinline__ typename Prec<PR>::S from_c(cplx<typename Prec<PR>::T
 // offset shared by all streams of a site index.  Plain pointer arithmetic
 // leaves a 64-bit VGPR address per stream (v_lshl_add_u64 for each of the ~230
 // loads of a stencil site), which costs registers and occupancy.  Each
-// component stream is < 4 GiB.  LYM_BUFFER_LOADS=0 keeps plain loads.
+// component stream is < 2 GiB.  LYM_BUFFER_LOADS=0 keeps plain loads.
 #ifndef LYM_BUFFER_LOADS
 #define LYM_BUFFER_LOADS 1
 #endif
@@ -131,7 +131,7 @@ __device__ __forceinline__ E* lane_at(E* ubase, unsigned lane_elems) {
 }
 
 #if LYM_BUFFER_LOADS
-// raw buffer resource for gfx9-family (gfx950): stride 0, 4 GiB range
+// raw buffer resource for gfx9-family (gfx950): stride 0, 2 GiB range
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_of(const void* base) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, 0x7fffffff, 0x00020000);
 }

@@ -526,8 +526,11 @@ int ensure_batch(lym_solver* s, int nr, int dom, int prec) {
         if ((rc = dzalloc(c, (void**)&s->bzero, sizeof(zd) * full_n(c)))) return rc;
     }
     if (s->bnr >= nr && s->bdom >= dom && s->bprec == prec) return LYM_OK;
-    free_batch(s);
     const size_t S = c->g.stride;
+    // one component of a batched field is read through 32-bit buffer offsets
+    if (S * (size_t)nr * sizeof(zd) >= (size_t(1) << 31))
+        return fail(LYM_ERR_INVALID, "%d right-hand sides per batch exceed the 2 GiB component stream on this lattice", nr);
+    free_batch(s);
     const size_t fullb = 24 * S * nr * sizeof(zd), domb = 12 * S * dom * nr * sizeof(zd);
     for (int i = 0; i < 10; ++i)
         if ((rc = dzalloc(c, (void**)&s->bw[i], i < 2 ? fullb : domb))) return rc;
