@@ -169,7 +169,14 @@ int op_sloppy_data(lym_op* op, int prec) {
     if (!op->Us) rc = dmalloc(c, &op->Us, 2L * 4 * 9 * S * le);  // room for 18-real links
     if (rc) return rc;
     op->sloppy_alloc = prec;
-    lym::launch_convert_links(prec, g->U, op->Us, nl, g->real_links, c->stream);
+    if (g->rows_rebuild) {
+        // rows 0 and 1 in 16/8/4-byte vectors, row 2 rebuilt in the stencil
+        lym::launch_pack_links(prec, g->U, ncomp, g->real_links, op->Us, (int)S, c->stream);
+        op->sloppy_lk = g->real_links ? 3 : 1;
+    } else {
+        lym::launch_convert_links(prec, g->U, op->Us, nl, g->real_links, c->stream);
+        op->sloppy_lk = link_kind(g);
+    }
     if (op->A) {
         if (!op->As) rc = dmalloc(c, &op->As, 2L * 36 * S * se);
         if (!rc && !op->As_sc) rc = dmalloc(c, (void**)&op->As_sc, 2L * S * sizeof(float));
@@ -347,7 +354,7 @@ void apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* 
     a.skip = skip;
     a.lane_flags = lane_flags;
     a.lane_skip_value = lane_skip_value;
-    lym::launch_dslash(a, link_kind(op->gauge), op_has_clover(op) ? 2 : 1, false, c->stream);
+    lym::launch_dslash(a, prec ? op->sloppy_lk : link_kind(op->gauge), op_has_clover(op) ? 2 : 1, false, c->stream);
 }
 
 // Schur complement on odd-half nr-fields, storage precision prec, explicit even intermediate
@@ -355,7 +362,7 @@ void apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, floa
                     const float* in_sc, void* tmp_e, float* tmp_sc, int gamma5, const int* skip,
                     const int* lane_flags, int lane_skip_value) {
     lym_ctx* c = op->ctx;
-    int lk = link_kind(op->gauge);
+    const int lk = prec ? op->sloppy_lk : link_kind(op->gauge);
     lym::DslashParams a = base_params(op);
     set_op_data(a, op, prec);
     a.nr = nr;
@@ -556,10 +563,24 @@ int lym_gauge_create(lym_ctx* c, const double* links, int folded, int storage, l
         return fail(LYM_ERR_DEVICE, "upload links: %s", hipGetErrorString(e));
     }
     lym::launch_links_to_eo(c->g, c->staging, g->real_links, ncomp, folded, g->U, c->stream);
-    e = hipStreamSynchronize(c->stream);
+    // 12/6-real storage holds rows 0, 1 by construction; 18/9-real links are
+    // checked once here (low-precision copies then keep rows 0, 1 only)
+    g->rows_rebuild = ncomp == 6;
+    static_assert(sizeof(double) * lym::RED_BLOCKS <= 4096, "pinned readback holds the partials");
+    if (ncomp == 9) lym::launch_link_row2_err(g->U, g->real_links, c->g.stride, c->red_partials, c->stream);
+    e = ncomp == 9 ? hipMemcpyAsync(c->host_pinned, c->red_partials, sizeof(double) * lym::RED_BLOCKS,
+                                    hipMemcpyDeviceToHost, c->stream)
+                   : hipSuccess;
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         lym_gauge_destroy(g);
         return fail(LYM_ERR_DEVICE, "links layout: %s", hipGetErrorString(e));
+    }
+    if (ncomp == 9) {
+        const double* m = reinterpret_cast<const double*>(c->host_pinned);
+        double worst = 0;
+        for (int k = 0; k < lym::RED_BLOCKS; ++k) worst = std::fmax(worst, m[k]);
+        g->rows_rebuild = worst < 1e-6;
     }
     *out = g;
     return LYM_OK;

@@ -163,6 +163,8 @@ struct lym_gauge {
     int real_links;    // 1 for SU(2) adjoint
     void* U;           // links, see layout above
     double2* B;        // kappa-independent clover blocks (hermitian, packed), built lazily
+    int rows_rebuild;  // 1: row 2 of every link is the one rebuilt from rows 0 and 1 (SU(3) /
+                       // SO(3) links), so the low-precision copies keep rows 0 and 1 only
 };
 
 struct lym_spinor {
@@ -186,6 +188,7 @@ struct lym_op {
     // low-precision copies for mixed-precision solvers (sloppy_prec 1 FP32, 2 FP16; 0 none)
     int sloppy_prec;
     int sloppy_alloc;  // precision the low-precision buffers are allocated for (kept across kappa changes)
+    int sloppy_lk;     // link kind of the low-precision links
     void* Us;          // links
     void* As;          // [2][36][stride] blocks
     float* As_sc;      // [2][stride]

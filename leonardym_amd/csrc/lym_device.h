@@ -15,6 +15,8 @@
 // transaction) and NR consecutive spinor elements.
 #pragma once
 
+#include <type_traits>
+
 #include "lym_kernels.h"
 
 #ifndef LYM_NT_LINKS
@@ -328,11 +330,77 @@ struct LinkMat<2, T> { T m[9]; };
 template <class T>
 struct LinkMat<3, T> { T m[9]; };
 
+// ---------------------------------------------------------------- packed low-precision links
+// FP32/FP16 links of kinds 1 and 3 hold rows 0 and 1 only (6 entries; row 2 is
+// rebuilt: conj(row0 x row1) for SU(3), row0 x row1 for the real adjoint
+// links), packed into 16-, 8- and 4-byte vectors.  Group g of the field is an
+// array [2 parity][4 mu][stride] of vectors of its size; the groups follow
+// one another.  Bytes per link: FP32 complex 48 (16+16+16), FP16 complex 24
+// (16+8), FP32 real 24 (16+8), FP16 real 12 (8+4).
+struct alignas(8) V8 {
+    unsigned w[2];
+};
+template <int PR, bool REAL>
+struct PackedLink {
+    static_assert(PR >= 1, "packed links are low-precision only");
+    typedef typename std::conditional<REAL, typename Prec<PR>::R, typename Prec<PR>::S>::type E;
+    static constexpr int BYTES = 6 * (int)sizeof(E);
+    static constexpr int N16 = BYTES / 16;
+    static constexpr bool H8 = (BYTES - 16 * N16) >= 8;
+    static constexpr bool H4 = (BYTES - 16 * N16) % 8 == 4;
+};
+
+template <int PR, bool REAL>
+__device__ __forceinline__ void load_link_rows(const void* base, int stride, int p, int mu, int idx,
+                                               typename PackedLink<PR, REAL>::E e[6]) {
+    typedef PackedLink<PR, REAL> P;
+    const char* gb = reinterpret_cast<const char*>(base);
+    const size_t plane = (size_t)(p * 4 + mu) * stride;
+    unsigned char buf[P::BYTES];
+    int off = 0;
+#pragma unroll
+    for (int g = 0; g < P::N16; ++g) {
+        V16 v = fld<V16, LYM_NT_LINKS != 0>(reinterpret_cast<const V16*>(gb) + plane, (unsigned)idx);
+        __builtin_memcpy(buf + off, &v, 16);
+        off += 16;
+        gb += (size_t)8 * stride * 16;
+    }
+    if constexpr (P::H8) {
+        V8 v = fld<V8, LYM_NT_LINKS != 0>(reinterpret_cast<const V8*>(gb) + plane, (unsigned)idx);
+        __builtin_memcpy(buf + off, &v, 8);
+        off += 8;
+        gb += (size_t)8 * stride * 8;
+    }
+    if constexpr (P::H4) {
+        unsigned v = fld<unsigned, LYM_NT_LINKS != 0>(reinterpret_cast<const unsigned*>(gb) + plane, (unsigned)idx);
+        __builtin_memcpy(buf + off, &v, 4);
+    }
+    __builtin_memcpy(e, buf, P::BYTES);
+}
+
 template <int PR, int LK>
 __device__ __forceinline__ void load_link(const void* base, int stride, int p, int mu, int idx,
                                           LinkMat<LK, typename Prec<PR>::T>& L) {
     typedef typename Prec<PR>::T T;
-    if constexpr (LK == 0 || LK == 1) {
+    if constexpr (PR >= 1 && (LK == 1 || LK == 3)) {
+        constexpr bool REAL = LK == 3;
+        typename PackedLink<PR, REAL>::E e[6];
+        load_link_rows<PR, REAL>(base, stride, p, mu, idx, e);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            if constexpr (REAL) L.m[k] = (T)e[k];
+            else L.m[k] = to_c<PR>(e[k]);
+        }
+        if constexpr (REAL) {
+            L.m[6] = L.m[1] * L.m[5] - L.m[2] * L.m[4];
+            L.m[7] = L.m[2] * L.m[3] - L.m[0] * L.m[5];
+            L.m[8] = L.m[0] * L.m[4] - L.m[1] * L.m[3];
+        } else {
+            L.m[6] = conj(L.m[1] * L.m[5] - L.m[2] * L.m[4]);
+            L.m[7] = conj(L.m[2] * L.m[3] - L.m[0] * L.m[5]);
+            L.m[8] = conj(L.m[0] * L.m[4] - L.m[1] * L.m[3]);
+        }
+    } else if constexpr (LK == 0 || LK == 1) {
         constexpr int NC = (LK == 0) ? 9 : 6;
         const typename Prec<PR>::S* b =
             reinterpret_cast<const typename Prec<PR>::S*>(base) + (size_t)((p * 4 + mu) * NC) * stride;

@@ -119,9 +119,17 @@ struct MCGState {
 // low-precision copies of operator data (mixed.hip); prec 1 FP32, 2 FP16
 void launch_convert_links(int prec, const void* src, void* dst, long n, int real_links, hipStream_t st);
 void launch_convert_blocks(int prec, const zd* src, void* dst, float* sc, int nparity, int stride, hipStream_t st);
+// rows 0 and 1 of FP64 links [2][4][nc][stride] into the packed low-precision
+// layout of link kinds 1 / 3 (row 2 rebuilt in the stencil)
+void launch_pack_links(int prec, const void* src, int nc, int real_links, void* dst, int stride, hipStream_t st);
+// per-workgroup maxima (RED_BLOCKS of them) of the deviation of row 2 of
+// 18-real / 9-real FP64 links from the row rebuilt out of rows 0 and 1
+void launch_link_row2_err(const void* src, int real_links, int stride, double* partials, hipStream_t st);
 // mixed-precision CG with reliable updates; vectors span nsites = nparity * stride sites
 void launch_mcg_init(int prec, MCGState* s, const zd* r, cplx<float>* rs, cplx<float>* xs, void* p, float* p_sc,
                      long nsites, int stride, double* partials, double eps, double delta, int max_steps, hipStream_t st);
+// <p, Ap> partials into partials[0, RED_BLOCKS); launch_mcg_update folds them into
+// alpha and needs partials[RED_BLOCKS, 2 RED_BLOCKS) for |r_s|^2
 void launch_mcg_alpha(int prec, MCGState* s, void* p, float* p_sc, void* Ap, float* Ap_sc, long nsites, int stride,
                       double* partials, hipStream_t st);
 void launch_mcg_update(int prec, MCGState* s, cplx<float>* xs, cplx<float>* rs, void* p, float* p_sc, void* Ap,

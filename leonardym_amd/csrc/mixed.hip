@@ -65,6 +65,36 @@ __device__ __forceinline__ void m_final_sum(const double* partials, double out[N
         }
 }
 
+// The same total as m_final_sum, folded by a workgroup of any size (a
+// multiple of 64, at most RED_BLOCKS): every workgroup that calls it gets the
+// identical value.  Result in out[] of every thread.
+template <int NV>
+__device__ __forceinline__ void m_fold_partials(const double* partials, double out[NV]) {
+    __shared__ double sh[NV][RED_BLOCKS / 64];
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        v[k] = 0;
+        for (int j = threadIdx.x; j < RED_BLOCKS; j += blockDim.x) v[k] += partials[k * RED_BLOCKS + j];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] += __shfl_xor(v[k], off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) sh[k][w] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double t = 0;
+        for (int j = 0; j < nw; ++j) t += sh[k][j];
+        out[k] = t;
+    }
+    __syncthreads();
+}
+
 #define SITE_LOOP(j, n) for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < (n); j += (long)gridDim.x * blockDim.x)
 
 // A sloppy spinor vector over nsites = nparity * stride sites: site j lives in
@@ -149,6 +179,79 @@ __global__ void k_convert_links(const void* src, void* dst, long n, int real_lin
     }
 }
 
+// FP64 links [2][4][nc][stride] (nc = 9 or 6; entries 0..5 are rows 0 and 1)
+// -> rows 0, 1 in the packed low-precision layout (lym_device.h)
+template <int PR, bool REAL>
+__global__ void k_pack_links(const void* src, int nc, void* dst, int stride) {
+    typedef PackedLink<PR, REAL> P;
+    typedef typename P::E E;
+    const long n = 8L * stride;
+    SITE_LOOP(j, n) {
+        const long plane = j / stride, i = j % stride;
+        E e[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const long o = (plane * nc + k) * stride + i;
+            if constexpr (REAL) e[k] = (E)reinterpret_cast<const double*>(src)[o];
+            else {
+                const zd z = reinterpret_cast<const zd*>(src)[o];
+                e[k] = from_c<PR>(mk((typename Prec<PR>::T)z.re, (typename Prec<PR>::T)z.im));
+            }
+        }
+        unsigned char buf[P::BYTES];
+        __builtin_memcpy(buf, e, P::BYTES);
+        char* gb = reinterpret_cast<char*>(dst);
+        int off = 0;
+        for (int g = 0; g < P::N16; ++g) {
+            __builtin_memcpy(gb + (size_t)j * 16, buf + off, 16);
+            off += 16;
+            gb += (size_t)n * 16;
+        }
+        if constexpr (P::H8) {
+            __builtin_memcpy(gb + (size_t)j * 8, buf + off, 8);
+            off += 8;
+            gb += (size_t)n * 8;
+        }
+        if constexpr (P::H4) __builtin_memcpy(gb + (size_t)j * 4, buf + off, 4);
+    }
+}
+
+// largest |U_2j - rebuilt U_2j| over the 18-real (9-real) FP64 links: row 2
+// rebuilt as conj(row0 x row1) (complex) or row0 x row1 (real); one maximum
+// per workgroup into partials[blockIdx.x]
+__global__ __launch_bounds__(MR_THREADS) void k_link_row2_err(const void* src, int real_links, int stride,
+                                                              double* partials) {
+    double m = 0;
+    const long n = 8L * stride;
+    SITE_LOOP(j, n) {
+        const long plane = j / stride, i = j % stride;
+        if (real_links) {
+            const double* u = reinterpret_cast<const double*>(src) + plane * 9 * stride + i;
+            double a[9];
+            for (int k = 0; k < 9; ++k) a[k] = u[(long)k * stride];
+            const double r6 = a[1] * a[5] - a[2] * a[4], r7 = a[2] * a[3] - a[0] * a[5], r8 = a[0] * a[4] - a[1] * a[3];
+            m = fmax(m, fmax(fabs(a[6] - r6), fmax(fabs(a[7] - r7), fabs(a[8] - r8))));
+        } else {
+            const zd* u = reinterpret_cast<const zd*>(src) + plane * 9 * stride + i;
+            zd a[9];
+            for (int k = 0; k < 9; ++k) a[k] = u[(long)k * stride];
+            const zd r[3] = {conj(a[1] * a[5] - a[2] * a[4]), conj(a[2] * a[3] - a[0] * a[5]),
+                             conj(a[0] * a[4] - a[1] * a[3])};
+            for (int k = 0; k < 3; ++k) m = fmax(m, fmax(fabs(a[6 + k].re - r[k].re), fabs(a[6 + k].im - r[k].im)));
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    __shared__ double sh[MR_THREADS / 64];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int w = 0; w < MR_THREADS / 64; ++w) t = fmax(t, sh[w]);
+        partials[blockIdx.x] = t;
+    }
+}
+
 // blocks [nparity][36][stride] -> PR storage with one scale per site
 template <int PR>
 __global__ void k_convert_blocks(const zd* src, typename Prec<PR>::S* dst, float* sc, int nparity, int stride) {
@@ -181,6 +284,21 @@ static inline int site_blocks(long n) {
 void launch_convert_links(int prec, const void* src, void* dst, long n, int real_links, hipStream_t st) {
     if (prec == 1) hipLaunchKernelGGL(k_convert_links<1>, dim3(site_blocks(n)), dim3(256), 0, st, src, dst, n, real_links);
     else hipLaunchKernelGGL(k_convert_links<2>, dim3(site_blocks(n)), dim3(256), 0, st, src, dst, n, real_links);
+}
+
+void launch_pack_links(int prec, const void* src, int nc, int real_links, void* dst, int stride, hipStream_t st) {
+    const long n = 8L * stride;
+    if (prec == 1) {
+        if (real_links) hipLaunchKernelGGL((k_pack_links<1, true>), dim3(site_blocks(n)), dim3(256), 0, st, src, nc, dst, stride);
+        else hipLaunchKernelGGL((k_pack_links<1, false>), dim3(site_blocks(n)), dim3(256), 0, st, src, nc, dst, stride);
+    } else {
+        if (real_links) hipLaunchKernelGGL((k_pack_links<2, true>), dim3(site_blocks(n)), dim3(256), 0, st, src, nc, dst, stride);
+        else hipLaunchKernelGGL((k_pack_links<2, false>), dim3(site_blocks(n)), dim3(256), 0, st, src, nc, dst, stride);
+    }
+}
+
+void launch_link_row2_err(const void* src, int real_links, int stride, double* partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_link_row2_err, dim3(RED_BLOCKS), dim3(MR_THREADS), 0, st, src, real_links, stride, partials);
 }
 
 void launch_convert_blocks(int prec, const zd* src, void* dst, float* sc, int nparity, int stride, hipStream_t st) {
@@ -253,20 +371,19 @@ __global__ __launch_bounds__(MR_THREADS) void k_mcg_dot(const MCGState* s, SVec<
     m_block_store<1>(v, partials);
 }
 
-__global__ __launch_bounds__(RED_BLOCKS) void k_mcg_alpha_final(MCGState* s, const double* partials) {
-    if (s->done) return;
-    double out[1];
-    m_final_sum<1>(partials, out);
-    if (threadIdx.x == 0) s->alpha = s->rr_s / out[0];
-}
-
-// x_s += alpha p, r_s -= alpha Ap, partial |r_s|^2
+// alpha = rr_s / <p, Ap>, then x_s += alpha p, r_s -= alpha Ap, partial |r_s|^2.
+// Every workgroup folds the <p, Ap> partials itself (in one fixed order, so
+// all get the same alpha) instead of a separate one-workgroup kernel.
 template <int PR>
-__global__ __launch_bounds__(MR_THREADS) void k_mcg_update(const MCGState* s, cplx<float>* xs, cplx<float>* rs,
+__global__ __launch_bounds__(MR_THREADS) void k_mcg_update(MCGState* s, cplx<float>* xs, cplx<float>* rs,
                                                            SVec<PR> p, SVec<PR> Ap, long nsites, int stride,
-                                                           double* partials) {
+                                                           const double* dot_partials, double* partials) {
     if (s->done) return;
-    const float al = (float)s->alpha;
+    double pap[1];
+    m_fold_partials<1>(dot_partials, pap);
+    const double alpha = s->rr_s / pap[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) s->alpha = alpha;
+    const float al = (float)alpha;
     double v[1] = {0};
     SITE_LOOP(j, nsites) {
         cplx<float> x[12], r[12], a[12], b[12];
@@ -421,18 +538,19 @@ void launch_mcg_alpha(int prec, MCGState* s, void* p, float* p_sc, void* Ap, flo
     else
         hipLaunchKernelGGL(k_mcg_dot<2>, dim3(RED_BLOCKS), dim3(MR_THREADS), 0, st, s, sv<2>(p, p_sc), sv<2>(Ap, Ap_sc),
                            nsites, stride, partials);
-    hipLaunchKernelGGL(k_mcg_alpha_final, dim3(1), dim3(RED_BLOCKS), 0, st, s, partials);
 }
 
 void launch_mcg_update(int prec, MCGState* s, cplx<float>* xs, cplx<float>* rs, void* p, float* p_sc, void* Ap,
                        float* Ap_sc, long nsites, int stride, double* partials, hipStream_t st) {
+    // <p, Ap> partials (launch_mcg_alpha) in the first RED_BLOCKS slots, |r_s|^2 in the next
+    double* rpart = partials + RED_BLOCKS;
     if (prec == 1)
         hipLaunchKernelGGL(k_mcg_update<1>, dim3(RED_BLOCKS), dim3(MR_THREADS), 0, st, s, xs, rs, sv<1>(p, p_sc),
-                           sv<1>(Ap, Ap_sc), nsites, stride, partials);
+                           sv<1>(Ap, Ap_sc), nsites, stride, partials, rpart);
     else
         hipLaunchKernelGGL(k_mcg_update<2>, dim3(RED_BLOCKS), dim3(MR_THREADS), 0, st, s, xs, rs, sv<2>(p, p_sc),
-                           sv<2>(Ap, Ap_sc), nsites, stride, partials);
-    hipLaunchKernelGGL(k_mcg_update_final, dim3(1), dim3(RED_BLOCKS), 0, st, s, partials);
+                           sv<2>(Ap, Ap_sc), nsites, stride, partials, rpart);
+    hipLaunchKernelGGL(k_mcg_update_final, dim3(1), dim3(RED_BLOCKS), 0, st, s, rpart);
 }
 
 void launch_rel_accum(MCGState* s, zd* x, cplx<float>* xs, long nsites, int stride, hipStream_t st) {
