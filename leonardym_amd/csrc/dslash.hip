@@ -214,10 +214,15 @@ __device__ __forceinline__ void dslash_site(const DslashParams& a, int p, int si
         load_spinor<PR>(SpinorRef<PR>{reinterpret_cast<const S*>(a.diag_in[p]), a.diag_sc[p]}, ST, a.nr, site, r, s);
         if constexpr (DIAG == 2) {
             cplx<T> t[12];
-            const S* Ab = reinterpret_cast<const S*>(a.A) + (size_t)p * 36 * ST;
-            const T sc = PR == 2 ? (T)a.A_sc[(size_t)p * ST + site] : (T)1;
-            herm6<PR>(Ab, ST, site, sc, s, t);
-            herm6<PR>(Ab + (size_t)18 * ST, ST, site, sc, s + 6, t + 6);
+            if constexpr (PR == 0) {
+                const S* Ab = reinterpret_cast<const S*>(a.A) + (size_t)p * 36 * ST;
+                herm6<PR>(Ab, ST, site, (T)1, s, t);
+                herm6<PR>(Ab + (size_t)18 * ST, ST, site, (T)1, s + 6, t + 6);
+            } else {
+                const T sc = PR == 2 ? (T)a.A_sc[(size_t)p * ST + site] : (T)1;
+                herm6_packed<PR>(a.A, 4 * (size_t)ST, (size_t)(2 * p) * ST, site, sc, s, t);
+                herm6_packed<PR>(a.A, 4 * (size_t)ST, (size_t)(2 * p + 1) * ST, site, sc, s + 6, t + 6);
+            }
 #pragma unroll
             for (int k = 0; k < 12; ++k) v[k] = t[k] + hk * acc[k];
         } else {
@@ -227,10 +232,15 @@ __device__ __forceinline__ void dslash_site(const DslashParams& a, int p, int si
     }
     if constexpr (INV) {
         cplx<T> t[12];
-        const S* Ai = reinterpret_cast<const S*>(a.Ainv);
-        const T sc = PR == 2 ? (T)a.Ainv_sc[site] : (T)1;
-        herm6<PR>(Ai, ST, site, sc, v, t);
-        herm6<PR>(Ai + (size_t)18 * ST, ST, site, sc, v + 6, t + 6);
+        if constexpr (PR == 0) {
+            const S* Ai = reinterpret_cast<const S*>(a.Ainv);
+            herm6<PR>(Ai, ST, site, (T)1, v, t);
+            herm6<PR>(Ai + (size_t)18 * ST, ST, site, (T)1, v + 6, t + 6);
+        } else {
+            const T sc = PR == 2 ? (T)a.Ainv_sc[site] : (T)1;
+            herm6_packed<PR>(a.Ainv, 2 * (size_t)ST, 0, site, sc, v, t);
+            herm6_packed<PR>(a.Ainv, 2 * (size_t)ST, (size_t)ST, site, sc, v + 6, t + 6);
+        }
 #pragma unroll
         for (int k = 0; k < 12; ++k) v[k] = t[k];
     }

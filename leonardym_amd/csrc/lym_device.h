@@ -495,6 +495,73 @@ __device__ __forceinline__ void herm6(const typename Prec<PR>::S* A, int stride,
     }
 }
 
+// The same product from the block's 18 entries already in registers (scaled).
+template <class T>
+__device__ __forceinline__ void herm6_mul(const cplx<T> a[18], const cplx<T> v[6], cplx<T> o[6]) {
+    T d[6];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        d[2 * k] = a[k].re;
+        d[2 * k + 1] = a[k].im;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) o[i] = d[i] * v[i];
+    int k = 3;
+#pragma unroll
+    for (int i = 1; i < 6; ++i) {
+#pragma unroll
+        for (int j = 0; j < i; ++j) {
+            cfma(o[i], a[k], v[j]);
+            cfmac(o[j], a[k], v[i]);
+            ++k;
+        }
+    }
+}
+
+// Low-precision clover blocks: the 18 entries of each hermitian 6x6 block
+// (3 diagonal pairs, 15 off-diagonal) packed into 16- and 8-byte vectors.
+// Group g is an array [nplanes][stride] of vectors (plane = 2 parity + block
+// for the blocks of D, block for R^-1), the groups one after the other.
+// FP32: 144 B = 9 x 16; FP16: 72 B = 4 x 16 + 8.
+template <int PR>
+struct PackedBlock {
+    static_assert(PR >= 1, "packed blocks are low-precision only");
+    typedef typename Prec<PR>::S E;
+    static constexpr int BYTES = 18 * (int)sizeof(E);
+    static constexpr int N16 = BYTES / 16;
+    static constexpr bool H8 = BYTES - 16 * N16 >= 8;
+};
+
+// o = sc * A v for the block at plane_off = plane * stride of a packed field
+// whose group arrays hold ngroup vectors each
+template <int PR>
+__device__ __forceinline__ void herm6_packed(const void* base, size_t ngroup, size_t plane_off, int idx,
+                                             typename Prec<PR>::T sc, const cplx<typename Prec<PR>::T> v[6],
+                                             cplx<typename Prec<PR>::T> o[6]) {
+    typedef PackedBlock<PR> P;
+    typedef typename Prec<PR>::T T;
+    const char* gb = reinterpret_cast<const char*>(base);
+    unsigned char buf[P::BYTES];
+    int off = 0;
+#pragma unroll
+    for (int g = 0; g < P::N16; ++g) {
+        V16 w = fld<V16, true>(reinterpret_cast<const V16*>(gb) + plane_off, (unsigned)idx);
+        __builtin_memcpy(buf + off, &w, 16);
+        off += 16;
+        gb += ngroup * 16;
+    }
+    if constexpr (P::H8) {
+        V8 w = fld<V8, true>(reinterpret_cast<const V8*>(gb) + plane_off, (unsigned)idx);
+        __builtin_memcpy(buf + off, &w, 8);
+    }
+    typename P::E e[18];
+    __builtin_memcpy(e, buf, P::BYTES);
+    cplx<T> a[18];
+#pragma unroll
+    for (int k = 0; k < 18; ++k) a[k] = sc * to_c<PR>(e[k]);
+    herm6_mul<T>(a, v, o);
+}
+
 // FP64 form used by the clover builder and the EO driver
 __device__ __forceinline__ void herm6(const cplx<double>* A, int stride, int idx, const cplx<double> v[6],
                                       cplx<double> o[6]) {

@@ -253,12 +253,15 @@ __global__ __launch_bounds__(MR_THREADS) void k_link_row2_err(const void* src, i
 }
 
 // blocks [nparity][36][stride] -> PR storage with one scale per site
+// blocks [nparity][36][stride] (FP64) -> the packed low-precision layout of
+// lym_device.h (planes 2 p + block), FP16 with one scale per site
 template <int PR>
-__global__ void k_convert_blocks(const zd* src, typename Prec<PR>::S* dst, float* sc, int nparity, int stride) {
+__global__ void k_convert_blocks(const zd* src, void* dst, float* sc, int nparity, int stride) {
+    typedef PackedBlock<PR> P;
+    const long ngroup = 2L * nparity * stride;
     SITE_LOOP(j, (long)nparity * stride) {
         const long p = j / stride, i = j % stride;
         const zd* s = src + p * 36 * stride + i;
-        typename Prec<PR>::S* d = dst + p * 36 * stride + i;
         float m = 1.f;
         if constexpr (PR == 2) {
             m = 0.f;
@@ -269,9 +272,23 @@ __global__ void k_convert_blocks(const zd* src, typename Prec<PR>::S* dst, float
         }
         const float inv = fp16_site_inv(m);
         if constexpr (PR == 2) sc[j] = m;
-        for (int k = 0; k < 36; ++k) {
-            zd z = s[(long)k * stride];
-            d[(long)k * stride] = from_c<PR>(mk((typename Prec<PR>::T)(inv * z.re), (typename Prec<PR>::T)(inv * z.im)));
+        for (int blk = 0; blk < 2; ++blk) {
+            typename P::E e[18];
+            for (int k = 0; k < 18; ++k) {
+                zd z = s[(long)(18 * blk + k) * stride];
+                e[k] = from_c<PR>(mk((typename Prec<PR>::T)(inv * z.re), (typename Prec<PR>::T)(inv * z.im)));
+            }
+            unsigned char buf[P::BYTES];
+            __builtin_memcpy(buf, e, P::BYTES);
+            const long el = (2 * p + blk) * stride + i;
+            char* gb = reinterpret_cast<char*>(dst);
+            int off = 0;
+            for (int g = 0; g < P::N16; ++g) {
+                __builtin_memcpy(gb + el * 16, buf + off, 16);
+                off += 16;
+                gb += ngroup * 16;
+            }
+            if constexpr (P::H8) __builtin_memcpy(gb + el * 8, buf + off, 8);
         }
     }
 }
@@ -304,11 +321,9 @@ void launch_link_row2_err(const void* src, int real_links, int stride, double* p
 void launch_convert_blocks(int prec, const zd* src, void* dst, float* sc, int nparity, int stride, hipStream_t st) {
     long n = (long)nparity * stride;
     if (prec == 1)
-        hipLaunchKernelGGL(k_convert_blocks<1>, dim3(site_blocks(n)), dim3(256), 0, st, src,
-                           (typename Prec<1>::S*)dst, sc, nparity, stride);
+        hipLaunchKernelGGL(k_convert_blocks<1>, dim3(site_blocks(n)), dim3(256), 0, st, src, dst, sc, nparity, stride);
     else
-        hipLaunchKernelGGL(k_convert_blocks<2>, dim3(site_blocks(n)), dim3(256), 0, st, src,
-                           (typename Prec<2>::S*)dst, sc, nparity, stride);
+        hipLaunchKernelGGL(k_convert_blocks<2>, dim3(site_blocks(n)), dim3(256), 0, st, src, dst, sc, nparity, stride);
 }
 
 // ---------------------------------------------------------------- mixed CG
