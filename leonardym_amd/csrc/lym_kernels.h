@@ -114,6 +114,7 @@ namespace lym {
 struct MCGState {
     int done, converged, step, max_steps, rel, skip_rel, nrel, pad;
     double rr_s, rr_true, maxrr, alpha, beta, eps, delta2;
+    double rr_prev;  // rr_s at the start of the iteration (set by the update, read by the p update)
 };
 
 // low-precision copies of operator data (mixed.hip); prec 1 FP32, 2 FP16
@@ -138,8 +139,10 @@ void launch_rel_accum(MCGState* s, zd* x, cplx<float>* xs, long nsites, int stri
 void launch_accum_xs(zd* x, const cplx<float>* xs, long nsites, int stride, hipStream_t st);
 void launch_rel_resid(MCGState* s, zd* r, cplx<float>* rs, const zd* b, const zd* Ax, long nsites, int stride,
                       double* partials, hipStream_t st);
+// p = r_s + beta p after launch_mcg_update, with beta and the state step taken
+// from the update's |r_s|^2 partials
 void launch_mcg_p(int prec, MCGState* s, void* p, float* p_sc, const cplx<float>* rs, long nsites, int stride,
-                  hipStream_t st);
+                  const double* partials, hipStream_t st);
 
 // ---------------------------------------------------------------- multi-RHS mixed CG (batch.hip)
 constexpr int BMAX = 16;

@@ -397,7 +397,10 @@ __global__ __launch_bounds__(MR_THREADS) void k_mcg_update(MCGState* s, cplx<flo
     double pap[1];
     m_fold_partials<1>(dot_partials, pap);
     const double alpha = s->rr_s / pap[0];
-    if (blockIdx.x == 0 && threadIdx.x == 0) s->alpha = alpha;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        s->alpha = alpha;
+        s->rr_prev = s->rr_s;  // the p update of this iteration divides by it
+    }
     const float al = (float)alpha;
     double v[1] = {0};
     SITE_LOOP(j, nsites) {
@@ -416,26 +419,6 @@ __global__ __launch_bounds__(MR_THREADS) void k_mcg_update(MCGState* s, cplx<flo
         f_store(rs, stride, j, r);
     }
     m_block_store<1>(v, partials);
-}
-
-__global__ __launch_bounds__(RED_BLOCKS) void k_mcg_update_final(MCGState* s, const double* partials) {
-    if (s->done) return;
-    double out[1];
-    m_final_sum<1>(partials, out);
-    if (threadIdx.x == 0) {
-        const double rn = out[0];
-        if (rn > s->maxrr) s->maxrr = rn;
-        // a reliable update is due; it runs at the next reliable-update slot
-        // (the iteration itself completes in sloppy precision)
-        if (rn < s->delta2 * s->maxrr || rn < s->eps) {
-            s->rel = 1;
-            s->skip_rel = 0;
-        }
-        s->beta = rn / s->rr_s;
-        s->rr_s = rn;
-        s->step += 1;
-        if (s->step >= s->max_steps) s->done = 1;
-    }
 }
 
 // reliable update, part 1: x += x_s, x_s = 0
@@ -512,11 +495,37 @@ __global__ __launch_bounds__(RED_BLOCKS) void k_rel_final(MCGState* s, const dou
     }
 }
 
-// p = r_s + beta p
+// beta = |r_s|^2 / rr_prev (every workgroup folds the |r_s|^2 partials of the
+// update itself), p = r_s + beta p; workgroup 0 also advances the state: the
+// step count, the largest residual since the last reliable update, and the
+// reliable-update flag (the update itself runs at the next reliable-update
+// slot, the iteration completes in sloppy precision).  Only workgroup 0 writes
+// the state and no workgroup reads what it writes, except `done` at the start
+// (read once per workgroup: a solve stopped here by max_steps may leave p
+// partly updated, which nothing reads).
 template <int PR>
-__global__ void k_mcg_p(const MCGState* s, SVec<PR> p, const cplx<float>* rs, long nsites, int stride) {
-    if (s->done) return;
-    const float be = (float)s->beta;
+__global__ __launch_bounds__(MR_THREADS) void k_mcg_p(MCGState* s, SVec<PR> p, const cplx<float>* rs, long nsites,
+                                                      int stride, const double* rpart) {
+    __shared__ int stop;
+    if (threadIdx.x == 0) stop = s->done;
+    __syncthreads();
+    if (stop) return;
+    double rr[1];
+    m_fold_partials<1>(rpart, rr);
+    const double rn = rr[0];
+    const double beta = rn / s->rr_prev;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (rn > s->maxrr) s->maxrr = rn;
+        if (rn < s->delta2 * s->maxrr || rn < s->eps) {
+            s->rel = 1;
+            s->skip_rel = 0;
+        }
+        s->beta = beta;
+        s->rr_s = rn;
+        s->step += 1;
+        if (s->step >= s->max_steps) s->done = 1;
+    }
+    const float be = (float)beta;
     SITE_LOOP(j, nsites) {
         cplx<float> a[12], r[12];
         sv_load<PR>(p, stride, j, a);
@@ -565,7 +574,7 @@ void launch_mcg_update(int prec, MCGState* s, cplx<float>* xs, cplx<float>* rs, 
     else
         hipLaunchKernelGGL(k_mcg_update<2>, dim3(RED_BLOCKS), dim3(MR_THREADS), 0, st, s, xs, rs, sv<2>(p, p_sc),
                            sv<2>(Ap, Ap_sc), nsites, stride, partials, rpart);
-    hipLaunchKernelGGL(k_mcg_update_final, dim3(1), dim3(RED_BLOCKS), 0, st, s, rpart);
+
 }
 
 void launch_rel_accum(MCGState* s, zd* x, cplx<float>* xs, long nsites, int stride, hipStream_t st) {
@@ -583,11 +592,14 @@ void launch_rel_resid(MCGState* s, zd* r, cplx<float>* rs, const zd* b, const zd
 }
 
 void launch_mcg_p(int prec, MCGState* s, void* p, float* p_sc, const cplx<float>* rs, long nsites, int stride,
-                  hipStream_t st) {
+                  const double* partials, hipStream_t st) {
+    const double* rpart = partials + RED_BLOCKS;  // |r_s|^2 partials of launch_mcg_update
     if (prec == 1)
-        hipLaunchKernelGGL(k_mcg_p<1>, dim3(site_blocks(nsites)), dim3(256), 0, st, s, sv<1>(p, p_sc), rs, nsites, stride);
+        hipLaunchKernelGGL(k_mcg_p<1>, dim3(RED_BLOCKS), dim3(MR_THREADS), 0, st, s, sv<1>(p, p_sc), rs, nsites, stride,
+                           rpart);
     else
-        hipLaunchKernelGGL(k_mcg_p<2>, dim3(site_blocks(nsites)), dim3(256), 0, st, s, sv<2>(p, p_sc), rs, nsites, stride);
+        hipLaunchKernelGGL(k_mcg_p<2>, dim3(RED_BLOCKS), dim3(MR_THREADS), 0, st, s, sv<2>(p, p_sc), rs, nsites, stride,
+                           rpart);
 }
 
 }  // namespace lym

@@ -457,7 +457,7 @@ int run_mixed_cg(lym_solver* s, const LinOp& L, const SloppyOp& SL, const zd* b,
         SL.apply(Ap, Ap_sc, p, p_sc, &S->done);
         lym::launch_mcg_alpha(prec, S, p, p_sc, Ap, Ap_sc, nsites, c->g.stride, c->red_partials, st);
         lym::launch_mcg_update(prec, S, s->xs, s->rs, p, p_sc, Ap, Ap_sc, nsites, c->g.stride, c->red_partials, st);
-        lym::launch_mcg_p(prec, S, p, p_sc, s->rs, nsites, c->g.stride, st);
+        lym::launch_mcg_p(prec, S, p, p_sc, s->rs, nsites, c->g.stride, c->red_partials, st);
         if (rel_slot(it)) {
             lym::launch_rel_accum(S, x, s->xs, nsites, c->g.stride, st);
             L.apply(Ax, x, &S->skip_rel);
