@@ -164,7 +164,7 @@ int lym_solver_set_max_steps(lym_solver* s, unsigned max_steps);
 int lym_solver_set_even_odd(lym_solver* s, int use_even_odd);   /* setUseEvenOddPreconditioning */
 /* mixed-precision solvers (no counterpart in the reference): sloppy storage
  * precision (LYM_PREC_SINGLE or LYM_PREC_HALF, default HALF) and the factor by
- * which the sloppy residual must fall before an FP64 reliable update (default 0.1) */
+ * which the sloppy residual must fall before an FP64 reliable update (default 0.05) */
 int lym_solver_set_sloppy_precision(lym_solver* s, int prec);
 int lym_solver_set_reliable_delta(lym_solver* s, double delta);
 /* BiConjugateGradient's shifted overload (BiConjugateGradient.cpp:510-690):

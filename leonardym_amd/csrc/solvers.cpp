@@ -849,7 +849,7 @@ int lym_solver_create(lym_ctx* c, int kind, lym_solver** out) {
     s->max_steps = cg ? 3000 : 1000;
     s->even_odd = 1;
     s->sloppy_prec = LYM_PREC_HALF;
-    s->delta = 0.1;
+    s->delta = 0.05;
     *out = s;
     return LYM_OK;
 }
