@@ -65,34 +65,32 @@ __device__ __forceinline__ void m_final_sum(const double* partials, double out[N
         }
 }
 
-// The same total as m_final_sum, folded by a workgroup of any size (a
-// multiple of 64, at most RED_BLOCKS): every workgroup that calls it gets the
-// identical value.  Result in out[] of every thread.
-template <int NV>
-__device__ __forceinline__ void m_fold_partials(const double* partials, double out[NV]) {
-    __shared__ double sh[NV][RED_BLOCKS / 64];
-    double v[NV];
+// The same total as m_final_sum, folded by a workgroup of MR_THREADS threads:
+// every workgroup that calls it gets the identical value.  Split in two so a
+// kernel can issue the partial loads, then its first streaming loads, and only
+// then wait for the partials: m_partials_load gives each thread its share (in
+// a fixed order), m_fold_block folds the shares over the workgroup (result in
+// every thread).
+constexpr int PART_PER_THREAD = RED_BLOCKS / MR_THREADS;
+static_assert(RED_BLOCKS % MR_THREADS == 0, "partials per thread");
+__device__ __forceinline__ void m_partials_load(const double* partials, double pl[PART_PER_THREAD]) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        v[k] = 0;
-        for (int j = threadIdx.x; j < RED_BLOCKS; j += blockDim.x) v[k] += partials[k * RED_BLOCKS + j];
-    }
+    for (int i = 0; i < PART_PER_THREAD; ++i) pl[i] = partials[threadIdx.x + i * MR_THREADS];
+}
+__device__ __forceinline__ double m_fold_block(const double pl[PART_PER_THREAD]) {
+    __shared__ double sh[MR_THREADS / 64];
+    double v = 0;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
+    for (int i = 0; i < PART_PER_THREAD; ++i) v += pl[i];
 #pragma unroll
-        for (int k = 0; k < NV; ++k) v[k] += __shfl_xor(v[k], off, 64);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    if (lane == 0)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) sh[k][w] = v[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
+    double t = 0;
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        double t = 0;
-        for (int j = 0; j < nw; ++j) t += sh[k][j];
-        out[k] = t;
-    }
+    for (int w = 0; w < MR_THREADS / 64; ++w) t += sh[w];
     __syncthreads();
+    return t;
 }
 
 #define SITE_LOOP(j, n) for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < (n); j += (long)gridDim.x * blockDim.x)
@@ -394,21 +392,26 @@ __global__ __launch_bounds__(MR_THREADS) void k_mcg_update(MCGState* s, cplx<flo
                                                            SVec<PR> p, SVec<PR> Ap, long nsites, int stride,
                                                            const double* dot_partials, double* partials) {
     if (s->done) return;
-    double pap[1];
-    m_fold_partials<1>(dot_partials, pap);
-    const double alpha = s->rr_s / pap[0];
+    double pl[PART_PER_THREAD];
+    m_partials_load(dot_partials, pl);
+    // the first site's vectors load while the partials are folded
+    long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long jstep = (long)gridDim.x * blockDim.x;
+    cplx<float> x[12], r[12], a[12], b[12];
+    if (j < nsites) {
+        f_load(xs, stride, j, x);
+        f_load(rs, stride, j, r);
+        sv_load<PR>(p, stride, j, a);
+        sv_load<PR>(Ap, stride, j, b);
+    }
+    const double alpha = s->rr_s / m_fold_block(pl);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         s->alpha = alpha;
         s->rr_prev = s->rr_s;  // the p update of this iteration divides by it
     }
     const float al = (float)alpha;
     double v[1] = {0};
-    SITE_LOOP(j, nsites) {
-        cplx<float> x[12], r[12], a[12], b[12];
-        f_load(xs, stride, j, x);
-        f_load(rs, stride, j, r);
-        sv_load<PR>(p, stride, j, a);
-        sv_load<PR>(Ap, stride, j, b);
+    while (j < nsites) {
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
             x[k] = x[k] + al * a[k];
@@ -417,6 +420,13 @@ __global__ __launch_bounds__(MR_THREADS) void k_mcg_update(MCGState* s, cplx<flo
         }
         f_store(xs, stride, j, x);
         f_store(rs, stride, j, r);
+        j += jstep;
+        if (j < nsites) {
+            f_load(xs, stride, j, x);
+            f_load(rs, stride, j, r);
+            sv_load<PR>(p, stride, j, a);
+            sv_load<PR>(Ap, stride, j, b);
+        }
     }
     m_block_store<1>(v, partials);
 }
@@ -510,9 +520,17 @@ __global__ __launch_bounds__(MR_THREADS) void k_mcg_p(MCGState* s, SVec<PR> p, c
     if (threadIdx.x == 0) stop = s->done;
     __syncthreads();
     if (stop) return;
-    double rr[1];
-    m_fold_partials<1>(rpart, rr);
-    const double rn = rr[0];
+    double pl[PART_PER_THREAD];
+    m_partials_load(rpart, pl);
+    // the first site's vectors load while the partials are folded
+    long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long jstep = (long)gridDim.x * blockDim.x;
+    cplx<float> a[12], r[12];
+    if (j < nsites) {
+        sv_load<PR>(p, stride, j, a);
+        f_load(rs, stride, j, r);
+    }
+    const double rn = m_fold_block(pl);
     const double beta = rn / s->rr_prev;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (rn > s->maxrr) s->maxrr = rn;
@@ -526,13 +544,15 @@ __global__ __launch_bounds__(MR_THREADS) void k_mcg_p(MCGState* s, SVec<PR> p, c
         if (s->step >= s->max_steps) s->done = 1;
     }
     const float be = (float)beta;
-    SITE_LOOP(j, nsites) {
-        cplx<float> a[12], r[12];
-        sv_load<PR>(p, stride, j, a);
-        f_load(rs, stride, j, r);
+    while (j < nsites) {
 #pragma unroll
         for (int k = 0; k < 12; ++k) a[k] = r[k] + be * a[k];
         sv_store<PR>(p, stride, j, a);
+        j += jstep;
+        if (j < nsites) {
+            sv_load<PR>(p, stride, j, a);
+            f_load(rs, stride, j, r);
+        }
     }
 }
 
