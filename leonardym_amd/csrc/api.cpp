@@ -142,20 +142,38 @@ int op_prepare(lym_op* op, bool force_blocks) {
 }
 
 void free_sloppy(lym_op* op) {
-    void* ptrs[5] = {op->Us, op->As, op->As_sc, op->Ainvs, op->Ainvs_sc};
+    void* ptrs[6] = {op->Us, op->Usf, op->As, op->As_sc, op->Ainvs, op->Ainvs_sc};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
-    op->Us = op->As = op->Ainvs = nullptr;
+    op->Us = op->Usf = op->As = op->Ainvs = nullptr;
     op->As_sc = op->Ainvs_sc = nullptr;
     op->sloppy_prec = 0;
     op->sloppy_alloc = 0;
 }
 
+// all three rows of 18-real (9-real) links in low precision, for multi-RHS
+// applies: there the NR lanes of a site would each rebuild row 2 of the same
+// link, and the batched stencil is bound by arithmetic, not bandwidth
+int op_sloppy_full_rows(lym_op* op, int prec) {
+    lym_ctx* c = op->ctx;
+    const lym_gauge* g = op->gauge;
+    const int ncomp = g->real_links ? g->compression : g->compression / 2;
+    if (!g->rows_rebuild || ncomp != 9 || op->Usf) return LYM_OK;  // Us already has every row, or Usf is current
+    const long S = c->g.stride;
+    const size_t le = g->real_links ? (prec == 1 ? 4 : 2) : (prec == 1 ? 8 : 4);
+    int rc = dmalloc(c, &op->Usf, 2L * 4 * 9 * S * le);
+    if (rc) return rc;
+    lym::launch_convert_links(prec, g->U, op->Usf, 2L * 4 * 9 * S, g->real_links, c->stream);
+    CHECK_LAUNCH();
+    return LYM_OK;
+}
+
 // low-precision copies of links and blocks for a mixed-precision solve
-int op_sloppy_data(lym_op* op, int prec) {
+// (full_rows: also the full-row links of multi-RHS applies)
+int op_sloppy_data(lym_op* op, int prec, bool full_rows) {
     lym_ctx* c = op->ctx;
     if (prec != 1 && prec != 2) return fail(LYM_ERR_INVALID, "sloppy precision must be 1 (FP32) or 2 (FP16)");
-    if (op->sloppy_prec == prec) return LYM_OK;
+    if (op->sloppy_prec == prec) return full_rows ? op_sloppy_full_rows(op, prec) : LYM_OK;
     // buffers stay allocated across kappa/csw/gauge changes (a free would
     // synchronise the device); only a change of precision reallocates
     if (op->sloppy_alloc != prec) free_sloppy(op);
@@ -177,6 +195,7 @@ int op_sloppy_data(lym_op* op, int prec) {
         lym::launch_convert_links(prec, g->U, op->Us, nl, g->real_links, c->stream);
         op->sloppy_lk = link_kind(g);
     }
+    if (op->Usf) lym::launch_convert_links(prec, g->U, op->Usf, nl, g->real_links, c->stream);
     if (op->A) {
         if (!op->As) rc = dmalloc(c, &op->As, 2L * 36 * S * se);
         if (!rc && !op->As_sc) rc = dmalloc(c, (void**)&op->As_sc, 2L * S * sizeof(float));
@@ -189,7 +208,7 @@ int op_sloppy_data(lym_op* op, int prec) {
     }
     CHECK_LAUNCH();
     op->sloppy_prec = prec;
-    return LYM_OK;
+    return full_rows ? op_sloppy_full_rows(op, prec) : LYM_OK;
 }
 
 lym::DslashParams base_params(const lym_op* op) {
@@ -309,34 +328,40 @@ void apply_full_add(lym_op* op, zd* out, const zd* in, const zd* v2, zd alpha, i
 }
 int link_kind_of(const lym_gauge* g) { return link_kind(g); }
 
-int op_sloppy(lym_op* op, int prec) { return op_sloppy_data(op, prec); }
+int op_sloppy(lym_op* op, int prec, bool full_rows) { return op_sloppy_data(op, prec, full_rows); }
 
 static size_t selem(int prec) { return prec == 0 ? 16 : (prec == 1 ? 8 : 4); }
 
-// operator data of a storage precision: 0 FP64 (the operator's own), 1/2 the sloppy copies
-static void set_op_data(lym::DslashParams& a, const lym_op* op, int prec) {
+// operator data of a storage precision: 0 FP64 (the operator's own), 1/2 the
+// sloppy copies (batch_links: the full-row links of the batched solver when
+// there are such, for every batch width, so an RHS gives the same bits in a
+// batch of any size); returns the link kind
+static int set_op_data(lym::DslashParams& a, const lym_op* op, int prec, bool batch_links) {
     a.prec = prec;
     if (prec == 0) {
         a.links = op->gauge->U;
         a.A = op->A;
         a.Ainv = op->Ainv;
-    } else {
-        a.links = op->Us;
-        a.A = op->As;
-        a.A_sc = op->As_sc;
-        a.Ainv = op->Ainvs;
-        a.Ainv_sc = op->Ainvs_sc;
+        return link_kind(op->gauge);
     }
+    const bool full = batch_links && op->Usf;
+    a.links = full ? op->Usf : op->Us;
+    a.A = op->As;
+    a.A_sc = op->As_sc;
+    a.Ainv = op->Ainvs;
+    a.Ainv_sc = op->Ainvs_sc;
+    return full ? link_kind(op->gauge) : op->sloppy_lk;
 }
 
 // full lattice, nr right-hand sides, storage precision prec: out = Op in
 void apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* out_sc, const void* in,
-                   const float* in_sc, int gamma5, const int* skip, const int* lane_flags, int lane_skip_value) {
+                   const float* in_sc, int gamma5, const int* skip, const int* lane_flags, int lane_skip_value,
+                   bool batch_links) {
     lym_ctx* c = op->ctx;
     const size_t po = 12 * (size_t)c->g.stride * nr * selem(prec);
     const size_t so = (size_t)c->g.stride * nr;
     lym::DslashParams a = base_params(op);
-    set_op_data(a, op, prec);
+    const int lk = set_op_data(a, op, prec, batch_links);
     a.nr = nr;
     a.threads = threads;
     for (int p = 0; p < 2; ++p) {
@@ -354,17 +379,16 @@ void apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* 
     a.skip = skip;
     a.lane_flags = lane_flags;
     a.lane_skip_value = lane_skip_value;
-    lym::launch_dslash(a, prec ? op->sloppy_lk : link_kind(op->gauge), op_has_clover(op) ? 2 : 1, false, c->stream);
+    lym::launch_dslash(a, lk, op_has_clover(op) ? 2 : 1, false, c->stream);
 }
 
 // Schur complement on odd-half nr-fields, storage precision prec, explicit even intermediate
 void apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, float* out_sc, const void* in_o,
                     const float* in_sc, void* tmp_e, float* tmp_sc, int gamma5, const int* skip,
-                    const int* lane_flags, int lane_skip_value) {
+                    const int* lane_flags, int lane_skip_value, bool batch_links) {
     lym_ctx* c = op->ctx;
-    const int lk = prec ? op->sloppy_lk : link_kind(op->gauge);
     lym::DslashParams a = base_params(op);
-    set_op_data(a, op, prec);
+    const int lk = set_op_data(a, op, prec, batch_links);
     a.nr = nr;
     a.threads = threads;
     a.skip = skip;
@@ -391,12 +415,12 @@ void apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, floa
 
 void apply_full_sloppy(lym_op* op, int prec, void* out, float* out_sc, const void* in, const float* in_sc, int gamma5,
                        const int* skip) {
-    apply_full_nr(op, prec, 1, 0, out, out_sc, in, in_sc, gamma5, skip, nullptr, 0);
+    apply_full_nr(op, prec, 1, 0, out, out_sc, in, in_sc, gamma5, skip, nullptr, 0, false);
 }
 
 void apply_schur_sloppy(lym_op* op, int prec, void* out_o, float* out_sc, const void* in_o, const float* in_sc,
                         void* tmp_e, float* tmp_sc, int gamma5, const int* skip) {
-    apply_schur_nr(op, prec, 1, 0, out_o, out_sc, in_o, in_sc, tmp_e, tmp_sc, gamma5, skip, nullptr, 0);
+    apply_schur_nr(op, prec, 1, 0, out_o, out_sc, in_o, in_sc, tmp_e, tmp_sc, gamma5, skip, nullptr, 0, false);
 }
 
 // the EO driver's source and reconstruction steps on nr-fields (FP64)

@@ -96,8 +96,11 @@ struct Dir<3> { static constexpr int P0 = 2, C0 = 2, P1 = 3, C1 = 2; };
 
 // forward:  h_a = psi_a + c_a psi_P, acc_a += U h_a, acc_P += conj(c_a) U h_a
 // backward: h_a = psi_a - c_a psi_P, acc_a += U^+ h_a, acc_P -= conj(c_a) U^+ h_a
-template <int MU, bool FWD, int LK, class T>
-__device__ __forceinline__ void hop_dir(cplx<T> acc[12], const cplx<T> psi[12], const LinkMat<LK, T>& L, bool flip) {
+// SCALED: psi is an unscaled FP16 spinor and sc its site scale, applied (with
+// the boundary sign) to the two colour vectors after the link multiply
+template <int MU, bool FWD, int LK, class T, bool SCALED = false>
+__device__ __forceinline__ void hop_dir(cplx<T> acc[12], const cplx<T> psi[12], const LinkMat<LK, T>& L, bool flip,
+                                        T sc = (T)1) {
     using D = Dir<MU>;
     constexpr int PH0 = FWD ? D::C0 : D::C0 + 2;
     constexpr int PH1 = FWD ? D::C1 : D::C1 + 2;
@@ -108,9 +111,11 @@ __device__ __forceinline__ void hop_dir(cplx<T> acc[12], const cplx<T> psi[12], 
     for (int c = 0; c < 3; ++c) {
         h0[c] = psi[c] + mulph<PH0>(psi[3 * D::P0 + c]);
         h1[c] = psi[3 + c] + mulph<PH1>(psi[3 * D::P1 + c]);
-        if (flip) {
-            h0[c] = -h0[c];
-            h1[c] = -h1[c];
+        if constexpr (!SCALED) {
+            if (flip) {
+                h0[c] = -h0[c];
+                h1[c] = -h1[c];
+            }
         }
     }
     if constexpr (FWD) {
@@ -119,6 +124,14 @@ __device__ __forceinline__ void hop_dir(cplx<T> acc[12], const cplx<T> psi[12], 
     } else {
         matvec_dag<LK, T>(L, h0, r0);
         matvec_dag<LK, T>(L, h1, r1);
+    }
+    if constexpr (SCALED) {
+        const T f = flip ? -sc : sc;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            r0[c] = f * r0[c];
+            r1[c] = f * r1[c];
+        }
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -155,29 +168,32 @@ __device__ __forceinline__ void hop_pair(const DslashParams& a, int p, int site,
     const int id = eo_index(g, cd[0], cd[1], cd[2], cd[3]);
     const bool anti = MU == 3 && g.bc_t < 0;
     const SpinorRef<PR> src{reinterpret_cast<const typename Prec<PR>::S*>(a.in[q]), a.in_sc[q]};
+    constexpr bool SC = PR == 2;
     cplx<T> psi[12];
+    T sc;
     LinkMat<LK, T> L;
     constexpr int PH = PR == 0 ? LYM_PIN_HOPS : LYM_PIN_HOPS_LOWP;
     constexpr bool pin_mid = PH == 1;
     constexpr bool pin_end = PH <= 2 || (PH == 4 && (MU & 1));
     if constexpr (!BWD_FIRST) {
-        load_spinor<PR>(src, g.stride, a.nr, iu, r, psi);
+        load_spinor_unscaled<PR>(src, g.stride, a.nr, iu, r, psi, sc);
         load_link<PR, LK>(a.links, g.stride, p, MU, site, L);
-        hop_dir<MU, true, LK, T>(acc, psi, L, anti && c[3] == g.L[3] - 1);
+        hop_dir<MU, true, LK, T, SC>(acc, psi, L, anti && c[3] == g.L[3] - 1, sc);
         if constexpr (pin_mid) pin(acc);
     }
     {
         cplx<T> psi2[12];
+        T sc2;
         LinkMat<LK, T> L2;
-        load_spinor<PR>(src, g.stride, a.nr, id, r, psi2);
+        load_spinor_unscaled<PR>(src, g.stride, a.nr, id, r, psi2, sc2);
         load_link<PR, LK>(a.links, g.stride, q, MU, id, L2);
-        hop_dir<MU, false, LK, T>(acc, psi2, L2, anti && c[3] == 0);
+        hop_dir<MU, false, LK, T, SC>(acc, psi2, L2, anti && c[3] == 0, sc2);
     }
     if constexpr (BWD_FIRST) {
         if constexpr (pin_mid) pin(acc);
-        load_spinor<PR>(src, g.stride, a.nr, iu, r, psi);
+        load_spinor_unscaled<PR>(src, g.stride, a.nr, iu, r, psi, sc);
         load_link<PR, LK>(a.links, g.stride, p, MU, site, L);
-        hop_dir<MU, true, LK, T>(acc, psi, L, anti && c[3] == g.L[3] - 1);
+        hop_dir<MU, true, LK, T, SC>(acc, psi, L, anti && c[3] == g.L[3] - 1, sc);
     }
     if constexpr (pin_end) pin(acc);
 }
@@ -209,20 +225,25 @@ __device__ __forceinline__ void dslash_site(const DslashParams& a, int p, int si
     if constexpr (DIAG == 0) {
 #pragma unroll
         for (int k = 0; k < 12; ++k) v[k] = hk * acc[k];
+    } else if constexpr (DIAG == 2 && PR >= 1) {
+        // A (sc_psi psi) = (sc_A sc_psi) (A psi): both FP16 scales on the 12 outputs
+        cplx<T> s[12], t[12];
+        T scp;
+        load_spinor_unscaled<PR>(SpinorRef<PR>{reinterpret_cast<const S*>(a.diag_in[p]), a.diag_sc[p]}, ST, a.nr,
+                                 site, r, s, scp);
+        const T sc = PR == 2 ? (T)a.A_sc[(size_t)p * ST + site] * scp : (T)1;
+        herm6_packed<PR>(a.A, 4 * (size_t)ST, (size_t)(2 * p) * ST, site, sc, s, t);
+        herm6_packed<PR>(a.A, 4 * (size_t)ST, (size_t)(2 * p + 1) * ST, site, sc, s + 6, t + 6);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) v[k] = t[k] + hk * acc[k];
     } else {
         cplx<T> s[12];
         load_spinor<PR>(SpinorRef<PR>{reinterpret_cast<const S*>(a.diag_in[p]), a.diag_sc[p]}, ST, a.nr, site, r, s);
         if constexpr (DIAG == 2) {
             cplx<T> t[12];
-            if constexpr (PR == 0) {
-                const S* Ab = reinterpret_cast<const S*>(a.A) + (size_t)p * 36 * ST;
-                herm6<PR>(Ab, ST, site, (T)1, s, t);
-                herm6<PR>(Ab + (size_t)18 * ST, ST, site, (T)1, s + 6, t + 6);
-            } else {
-                const T sc = PR == 2 ? (T)a.A_sc[(size_t)p * ST + site] : (T)1;
-                herm6_packed<PR>(a.A, 4 * (size_t)ST, (size_t)(2 * p) * ST, site, sc, s, t);
-                herm6_packed<PR>(a.A, 4 * (size_t)ST, (size_t)(2 * p + 1) * ST, site, sc, s + 6, t + 6);
-            }
+            const S* Ab = reinterpret_cast<const S*>(a.A) + (size_t)p * 36 * ST;
+            herm6<PR>(Ab, ST, site, (T)1, s, t);
+            herm6<PR>(Ab + (size_t)18 * ST, ST, site, (T)1, s + 6, t + 6);
 #pragma unroll
             for (int k = 0; k < 12; ++k) v[k] = t[k] + hk * acc[k];
         } else {

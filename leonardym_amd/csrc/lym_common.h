@@ -189,6 +189,7 @@ struct lym_op {
     int sloppy_prec;
     int sloppy_alloc;  // precision the low-precision buffers are allocated for (kept across kappa changes)
     int sloppy_lk;     // link kind of the low-precision links
+    void* Usf;         // all rows of the links in low precision (multi-RHS applies), when Us holds rows 0, 1
     void* Us;          // links
     void* As;          // [2][36][stride] blocks
     float* As_sc;      // [2][stride]

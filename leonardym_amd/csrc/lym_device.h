@@ -277,6 +277,28 @@ __device__ __forceinline__ void load_spinor(SpinorRef<PR> f, int stride, int nr,
     }
 }
 
+// The same without the FP16 per-site scale: sc receives it (1 for FP64/FP32),
+// so a caller can apply it to a product of fewer components (the stencil
+// scales the two projected colour vectors after the link multiply: 12 real
+// multiplies per hop instead of 24).
+template <int PR>
+__device__ __forceinline__ void load_spinor_unscaled(SpinorRef<PR> f, int stride, int nr, int site, int r,
+                                                     cplx<typename Prec<PR>::T> s[12], typename Prec<PR>::T& sc) {
+    const unsigned o = (unsigned)site * (unsigned)nr + (unsigned)r;
+    const size_t ks = (size_t)stride * nr;
+    if constexpr (PR == 0) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) s[k] = fld<typename Prec<PR>::S>(f.base + k * ks, o);
+    } else {
+        typename Prec<PR>::S e[12];
+        load_packed<PR>(f.base, ks, o, e);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) s[k] = to_c<PR>(e[k]);
+    }
+    if constexpr (PR == 2) sc = fld<float>(f.scale, o);
+    else sc = (typename Prec<PR>::T)1;
+}
+
 // Reciprocal of a per-site FP16 scale m (the site's largest |component|).  A
 // site whose m is below the smallest normal float is stored as zero: 1/m of a
 // subnormal overflows to inf, and inf * 0 = NaN would poison the vector.  Such
@@ -558,8 +580,13 @@ __device__ __forceinline__ void herm6_packed(const void* base, size_t ngroup, si
     __builtin_memcpy(e, buf, P::BYTES);
     cplx<T> a[18];
 #pragma unroll
-    for (int k = 0; k < 18; ++k) a[k] = sc * to_c<PR>(e[k]);
+    for (int k = 0; k < 18; ++k) a[k] = to_c<PR>(e[k]);
     herm6_mul<T>(a, v, o);
+    if constexpr (PR == 2) {
+        // the block's scale on the 6 outputs instead of its 18 entries
+#pragma unroll
+        for (int i = 0; i < 6; ++i) o[i] = sc * o[i];
+    }
 }
 
 // FP64 form used by the clover builder and the EO driver

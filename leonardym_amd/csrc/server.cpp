@@ -117,12 +117,13 @@ struct lym_server {
 namespace {
 
 // device bytes of one worker's state for a lattice, for batches up to nr RHS:
-// operator (blocks, inverse, scratch, low-precision copies), solver (8 FP64
-// work vectors, mixed and batch buffers), nr source and solution spinors
+// operator (blocks, inverse, scratch, low-precision copies: links as rows 0, 1
+// and as all rows for batches, blocks), solver (8 FP64 work vectors, mixed and
+// batch buffers), nr source and solution spinors
 size_t state_bytes(const Lattice& L, unsigned nr) {
     const double V = (double)L.dims[0] * L.dims[1] * L.dims[2] * L.dims[3];
     const double full = V * 12 * 16;  // FP64 full spinor
-    const double op = V * (36 * 16 + 18 * 16) + 2 * full + V * 4 * 18 * 4 + V * 54 * 4;
+    const double op = V * (36 * 16 + 18 * 16) + 2 * full + 2 * V * 4 * 9 * 8 + V * 54 * 4;
     const double solver = 8 * full + 6 * full / 2;
     const double batch = nr * (2 * full + 8 * full / 2 + 4 * V * 12 * 4 + 2 * V * 6 * 8);
     const double spinors = 2.0 * nr * full + full;  // + staging

@@ -36,17 +36,19 @@ void apply_schur(lym_op* op, zd* out_o, const zd* in_o, int gamma5, const int* s
 void apply_full(lym_op* op, zd* out, const zd* in, int gamma5, const int* skip);
 void apply_full_add(lym_op* op, zd* out, const zd* in, const zd* v2, zd alpha, int gamma5, const int* skip);
 int link_kind_of(const lym_gauge* g);
-int op_sloppy(lym_op* op, int prec);
+int op_sloppy(lym_op* op, int prec, bool full_rows = false);
 void apply_full_sloppy(lym_op* op, int prec, void* out, float* out_sc, const void* in, const float* in_sc, int gamma5,
                        const int* skip);
 void apply_schur_sloppy(lym_op* op, int prec, void* out_o, float* out_sc, const void* in_o, const float* in_sc,
                         void* tmp_e, float* tmp_sc, int gamma5, const int* skip);
+// batch_links: the batched solver's low-precision links (all rows where the
+// single-RHS copy keeps rows 0, 1)
 void apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* out_sc, const void* in,
                    const float* in_sc, int gamma5, const int* skip, const int* lane_flags = nullptr,
-                   int lane_skip_value = 0);
+                   int lane_skip_value = 0, bool batch_links = false);
 void apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, float* out_sc, const void* in_o,
                     const float* in_sc, void* tmp_e, float* tmp_sc, int gamma5, const int* skip,
-                    const int* lane_flags = nullptr, int lane_skip_value = 0);
+                    const int* lane_flags = nullptr, int lane_skip_value = 0, bool batch_links = false);
 void eo_source_nr(lym_op* op, int nr, const zd* b_e, const zd* b_o, zd* te, zd* src_o);
 void eo_reconstruct_nr(lym_op* op, int nr, const zd* b_e, const zd* x_o, zd* x_e);
 }  // namespace lym_internal
@@ -201,7 +203,7 @@ struct Key {
 // kappa change keeps the captures valid; csw enters only through the blocks)
 void key_op(Key& k, const lym_op* op) {
     k << op << op->ctx << op->kind << op->power << op->dev_kappa << (op->csw != 0.0) << op->gamma5 << op->gauge << op->gauge->U
-      << op->gauge->compression << op->A << op->Ainv << op->tmp << op->tmp2 << op->sloppy_prec << op->Us << op->As
+      << op->gauge->compression << op->A << op->Ainv << op->tmp << op->tmp2 << op->sloppy_prec << op->Us << op->Usf << op->As
       << op->As_sc << op->Ainvs << op->Ainvs_sc << op->ctx->g.stride << op->ctx->g.Vh;
 }
 
@@ -564,10 +566,10 @@ struct BatchOp {
         auto one = [&](void* o, float* osc, const void* i, const float* isc) {
             if (half)
                 lym_internal::apply_schur_nr(op, prec, nr, threads, o, osc, i, isc, tmp, tmp_sc, gamma5, skip,
-                                             lane_flags, lane_skip_value);
+                                             lane_flags, lane_skip_value, true);
             else
                 lym_internal::apply_full_nr(op, prec, nr, threads, o, osc, i, isc, gamma5, skip, lane_flags,
-                                            lane_skip_value);
+                                            lane_skip_value, true);
         };
         if (power == 1) {
             one(out, out_sc, in, in_sc);
@@ -808,7 +810,7 @@ int lym_solver_solve_batch(lym_solver* s, lym_op* op, unsigned nrhs, const lym_s
                     c->dims[1] % 2 == 0 && c->dims[2] % 2 == 0 && c->dims[3] % 2 == 0;
     const bool half = op->kind == LYM_OP_EO_CLOVER;
     int rc = lym_internal::op_ready(op, eo);
-    if (!rc) rc = lym_internal::op_sloppy(op, s->sloppy_prec);
+    if (!rc) rc = lym_internal::op_sloppy(op, s->sloppy_prec, true);
     if (rc) return rc;
     int nr = s->rhs_block, thr = 0;
     if (nr == 0) {
