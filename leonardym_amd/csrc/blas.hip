@@ -73,6 +73,44 @@ __device__ __forceinline__ void final_sum(const double* partials, double out[NV]
         }
 }
 
+// The total of final_sum, folded by every workgroup of RB_THREADS threads that
+// needs it (the same fixed order in each, so all get the same value): the
+// partials are loaded first (fold_load), the caller may then issue its own
+// loads, and fold_block folds across the workgroup (result in every thread).
+constexpr int FOLD_PER_THREAD = RED_BLOCKS / RB_THREADS;
+static_assert(RED_BLOCKS % RB_THREADS == 0, "partials per thread");
+template <int NV>
+__device__ __forceinline__ void fold_load(const double* partials, double pl[NV][FOLD_PER_THREAD]) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+        for (int i = 0; i < FOLD_PER_THREAD; ++i) pl[k][i] = partials[k * RED_BLOCKS + threadIdx.x + i * RB_THREADS];
+}
+template <int NV>
+__device__ __forceinline__ void fold_block(const double pl[NV][FOLD_PER_THREAD], double out[NV]) {
+    __shared__ double sh[NV][RB_THREADS / 64];
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        v[k] = 0;
+#pragma unroll
+        for (int i = 0; i < FOLD_PER_THREAD; ++i) v[k] += pl[k][i];
+    }
+    wave_sum<NV>(v);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) sh[k][threadIdx.x >> 6] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double t = 0;
+#pragma unroll
+        for (int j = 0; j < RB_THREADS / 64; ++j) t += sh[k][j];
+        out[k] = t;
+    }
+    __syncthreads();
+}
+
 #define GRID_LOOP(i, n) for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (long)gridDim.x * blockDim.x)
 
 // ---------------------------------------------------------------- plain reductions
@@ -228,72 +266,110 @@ __global__ __launch_bounds__(RB_THREADS) void k_cg_dot(const CGState* s, const z
     block_store<2>(v, partials);
 }
 
-// alpha = |r|^2 / <p, A p>  (complex, ConjugateGradient.cpp:36)
-__global__ __launch_bounds__(RED_BLOCKS) void k_cg_alpha_final(CGState* s, const double* partials) {
-    if (s->done) return;
-    double out[2];
-    final_sum<2>(partials, out);
-    if (threadIdx.x == 0) {
-        double den = out[0] * out[0] + out[1] * out[1];
-        s->alpha_re = s->rr * out[0] / den;
-        s->alpha_im = -s->rr * out[1] / den;
-    }
-}
-
 void launch_cg_alpha(CGState* s, const zd* p, const zd* Ap, long n, double* partials, hipStream_t st) {
     hipLaunchKernelGGL(k_cg_dot, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, s, p, Ap, n, partials);
-    hipLaunchKernelGGL(k_cg_alpha_final, dim3(1), dim3(RED_BLOCKS), 0, st, s, partials);
 }
 
-// x += alpha p, r -= alpha Ap, partial |r|^2
-__global__ __launch_bounds__(RB_THREADS) void k_cg_update(const CGState* s, zd* __restrict__ x, zd* __restrict__ r,
+// alpha = |r|^2 / <p, A p> (complex, ConjugateGradient.cpp:36), folded by
+// every workgroup from the <p, Ap> partials; then x += alpha p, r -= alpha Ap,
+// partial |r|^2.  Workgroup 0 records alpha and the |r|^2 the p update
+// divides by.
+__global__ __launch_bounds__(RB_THREADS) void k_cg_update(CGState* s, zd* __restrict__ x, zd* __restrict__ r,
                                                           const zd* __restrict__ p, const zd* __restrict__ Ap, long n,
-                                                          double* partials) {
+                                                          const double* dot_partials, double* partials) {
     if (s->done) return;
-    const zd al = mk(s->alpha_re, s->alpha_im);
+    double pl[2][FOLD_PER_THREAD];
+    fold_load<2>(dot_partials, pl);
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long istep = (long)gridDim.x * blockDim.x;
+    zd xv, rv, pv, av;
+    if (i < n) {
+        xv = x[i];
+        rv = r[i];
+        pv = p[i];
+        av = Ap[i];
+    }
+    double d[2];
+    fold_block<2>(pl, d);
+    const double den = d[0] * d[0] + d[1] * d[1];
+    const zd al = mk(s->rr * d[0] / den, -s->rr * d[1] / den);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        s->alpha_re = al.re;
+        s->alpha_im = al.im;
+        s->rr_prev = s->rr;
+    }
     double v[1] = {0};
-    GRID_LOOP(i, n) {
-        x[i] = x[i] + al * p[i];
-        zd rn = r[i] - al * Ap[i];
+    while (i < n) {
+        x[i] = xv + al * pv;
+        const zd rn = rv - al * av;
         r[i] = rn;
         v[0] += rn.re * rn.re + rn.im * rn.im;
+        i += istep;
+        if (i < n) {
+            xv = x[i];
+            rv = r[i];
+            pv = p[i];
+            av = Ap[i];
+        }
     }
     block_store<1>(v, partials);
 }
 
-__global__ __launch_bounds__(RED_BLOCKS) void k_cg_update_final(CGState* s, const double* partials) {
-    if (s->done) return;
-    double out[1];
-    final_sum<1>(partials, out);
-    if (threadIdx.x == 0) {
-        double rn = out[0];
-        s->rr_new = rn;
-        if (rn < s->eps) {
+void launch_cg_update(CGState* s, zd* x, zd* r, const zd* p, const zd* Ap, long n, double* partials,
+                      hipStream_t st) {
+    // <p, Ap> partials in slots 0 and 1 (launch_cg_alpha), |r|^2 in slot 2
+    hipLaunchKernelGGL(k_cg_update, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, s, x, r, p, Ap, n, partials,
+                       partials + 2 * RED_BLOCKS);
+}
+
+// |r|^2 folded by every workgroup from the update's partials; workgroup 0
+// advances the state (ConjugateGradient.cpp:49-50: stop at |r|^2 < eps, else
+// beta = |r|^2 / |r_prev|^2, step + 1); then p = r + beta p.  Only workgroup 0
+// writes the state and no workgroup reads what it writes except `done` at the
+// start (read once per workgroup: after a stop decided here p may be left
+// partly updated, which nothing reads).
+__global__ __launch_bounds__(RB_THREADS) void k_cg_p(CGState* s, zd* __restrict__ p, const zd* __restrict__ r, long n,
+                                                     const double* rpart) {
+    __shared__ int stop;
+    if (threadIdx.x == 0) stop = s->done;
+    __syncthreads();
+    if (stop) return;
+    double pl[1][FOLD_PER_THREAD];
+    fold_load<1>(rpart, pl);
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long istep = (long)gridDim.x * blockDim.x;
+    zd pv, rv;
+    if (i < n) {
+        pv = p[i];
+        rv = r[i];
+    }
+    double rn[1];
+    fold_block<1>(pl, rn);
+    const double b = rn[0] / s->rr_prev;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        s->rr_new = rn[0];
+        if (rn[0] < s->eps) {
             s->done = 1;
             s->converged = 1;
         } else {
-            s->beta = rn / s->rr;
-            s->rr = rn;
+            s->beta = b;
+            s->rr = rn[0];
             s->step += 1;
             if (s->step >= s->max_steps) s->done = 1;
         }
     }
+    while (i < n) {
+        p[i] = rv + b * pv;
+        i += istep;
+        if (i < n) {
+            pv = p[i];
+            rv = r[i];
+        }
+    }
 }
 
-void launch_cg_update(CGState* s, zd* x, zd* r, const zd* p, const zd* Ap, long n, double* partials,
-                      hipStream_t st) {
-    hipLaunchKernelGGL(k_cg_update, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, s, x, r, p, Ap, n, partials);
-    hipLaunchKernelGGL(k_cg_update_final, dim3(1), dim3(RED_BLOCKS), 0, st, s, partials);
-}
-
-__global__ void k_cg_p(const CGState* s, zd* __restrict__ p, const zd* __restrict__ r, long n) {
-    if (s->done) return;
-    const double b = s->beta;
-    GRID_LOOP(i, n) { p[i] = r[i] + b * p[i]; }
-}
-
-void launch_cg_p(CGState* s, zd* p, const zd* r, long n, hipStream_t st) {
-    hipLaunchKernelGGL(k_cg_p, dim3(ew_blocks(n)), dim3(256), 0, st, s, p, r, n);
+void launch_cg_p(CGState* s, zd* p, const zd* r, long n, double* partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_cg_p, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, s, p, r, n, partials + 2 * RED_BLOCKS);
 }
 
 // ---------------------------------------------------------------- BiCGStab

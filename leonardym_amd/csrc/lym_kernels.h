@@ -82,6 +82,7 @@ void launch_sub(zd* out, const zd* a, const zd* b, long n, hipStream_t st); // o
 struct CGState {
     int done, converged, step, max_steps;
     double rr, rr_new, alpha_re, alpha_im, beta, eps;
+    double rr_prev;  // rr at the start of the iteration (set by the update, read by the p update)
 };
 
 struct BiCGState {
@@ -92,10 +93,13 @@ struct BiCGState {
 };
 
 void launch_cg_init(CGState* s, const zd* r, long n, double* partials, double eps, int max_steps, hipStream_t st);
+// one CG iteration after Ap = A p: <p, Ap> partials (slots 0, 1 of partials),
+// the update folding them into alpha (|r|^2 partials into slot 2), the p update
+// folding those into beta and the stopping rule
 void launch_cg_alpha(CGState* s, const zd* p, const zd* Ap, long n, double* partials, hipStream_t st);
 void launch_cg_update(CGState* s, zd* x, zd* r, const zd* p, const zd* Ap, long n, double* partials,
                       hipStream_t st);
-void launch_cg_p(CGState* s, zd* p, const zd* r, long n, hipStream_t st);
+void launch_cg_p(CGState* s, zd* p, const zd* r, long n, double* partials, hipStream_t st);
 
 void launch_bicg_init(BiCGState* s, double precision, int max_steps, hipStream_t st);
 // r = b - Ax, rh = b + Ax, p = nu = 0

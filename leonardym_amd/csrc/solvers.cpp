@@ -295,7 +295,7 @@ int run_cg(lym_solver* s, const LinOp& L, const zd* b, zd* xout, const zd* x0, l
         L.apply(Ap, p, &S->done);
         lym::launch_cg_alpha(S, p, Ap, n, c->red_partials, st);
         lym::launch_cg_update(S, x, r, p, Ap, n, c->red_partials, st);
-        lym::launch_cg_p(S, p, r, n, st);
+        lym::launch_cg_p(S, p, r, n, c->red_partials, st);
     };
     while (!H->done && queued <= s->max_steps) {
         unsigned k = next_chunk(r0, H->rr, (unsigned)H->step, s->precision, s->max_steps + 1 - queued);
