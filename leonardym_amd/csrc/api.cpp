@@ -362,6 +362,7 @@ void apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* 
     const size_t so = (size_t)c->g.stride * nr;
     lym::DslashParams a = base_params(op);
     const int lk = set_op_data(a, op, prec, batch_links);
+    a.allow_split = !batch_links;  // batches keep one form for every width (same bits)
     a.nr = nr;
     a.threads = threads;
     for (int p = 0; p < 2; ++p) {
@@ -389,6 +390,7 @@ void apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, floa
     lym_ctx* c = op->ctx;
     lym::DslashParams a = base_params(op);
     const int lk = set_op_data(a, op, prec, batch_links);
+    a.allow_split = !batch_links;  // batches keep one form for every width (same bits)
     a.nr = nr;
     a.threads = threads;
     a.skip = skip;

@@ -212,6 +212,9 @@ __device__ __forceinline__ void fst(E* ubase, unsigned lane, E e) {
 struct alignas(16) V16 {
     unsigned w[4];
 };
+struct alignas(8) V8 {
+    unsigned w[2];
+};
 template <int PR>
 struct Packed {
     static constexpr int PER = 16 / (int)sizeof(typename Prec<PR>::S);  // components per vector
@@ -352,6 +355,68 @@ struct LinkMat<2, T> { T m[9]; };
 template <class T>
 struct LinkMat<3, T> { T m[9]; };
 
+// ---------------------------------------------------------------- one chirality of a low-precision spinor
+// Components 6h .. 6h+5 (spins 2h, 2h+1) of a packed FP32/FP16 spinor, for
+// the two-threads-per-site stencil; h must be wave-uniform.  FP32 vectors hold
+// components (2g, 2g+1): a chirality is three whole vectors.  FP16 vectors
+// hold (4g .. 4g+3): chirality 0 is vector 0 and the low half of vector 1,
+// chirality 1 the high half of vector 1 and vector 2.  No scale applied.
+template <int PR, bool NT = false>
+__device__ __forceinline__ void load_half_lp(const typename Prec<PR>::S* pbase, size_t ks, unsigned o, int h,
+                                             typename Prec<PR>::S e[6]) {
+    static_assert(PR >= 1, "low precision only");
+    const V16* b = reinterpret_cast<const V16*>(pbase);
+    if constexpr (PR == 1) {
+#pragma unroll
+        for (int g = 0; g < 3; ++g) {
+            V16 v = fld<V16, NT>(b + (3 * h + g) * ks, o);
+            __builtin_memcpy(e + 2 * g, &v, 16);
+        }
+    } else {
+        const V8* b8 = reinterpret_cast<const V8*>(b + ks);  // vector 1 as 8-byte halves
+        if (h == 0) {
+            V16 v = fld<V16, NT>(b, o);
+            V8 w = fld<V8, NT>(b8, 2 * o);
+            __builtin_memcpy(e, &v, 16);
+            __builtin_memcpy(e + 4, &w, 8);
+        } else {
+            V8 w = fld<V8, NT>(b8, 2 * o + 1);
+            V16 v = fld<V16, NT>(b + 2 * ks, o);
+            __builtin_memcpy(e, &w, 8);
+            __builtin_memcpy(e + 2, &v, 16);
+        }
+    }
+}
+template <int PR>
+__device__ __forceinline__ void store_half_lp(typename Prec<PR>::S* pbase, size_t ks, unsigned o, int h,
+                                              const typename Prec<PR>::S e[6]) {
+    static_assert(PR >= 1, "low precision only");
+    V16* b = reinterpret_cast<V16*>(pbase);
+    if constexpr (PR == 1) {
+#pragma unroll
+        for (int g = 0; g < 3; ++g) {
+            V16 v;
+            __builtin_memcpy(&v, e + 2 * g, 16);
+            fst<V16>(b + (3 * h + g) * ks, o, v);
+        }
+    } else {
+        V8* b8 = reinterpret_cast<V8*>(b + ks);
+        V16 v;
+        V8 w;
+        if (h == 0) {
+            __builtin_memcpy(&v, e, 16);
+            __builtin_memcpy(&w, e + 4, 8);
+            fst<V16>(b, o, v);
+            fst<V8>(b8, 2 * o, w);
+        } else {
+            __builtin_memcpy(&w, e, 8);
+            __builtin_memcpy(&v, e + 2, 16);
+            fst<V8>(b8, 2 * o + 1, w);
+            fst<V16>(b + 2 * ks, o, v);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- packed low-precision links
 // FP32/FP16 links of kinds 1 and 3 hold rows 0 and 1 only (6 entries; row 2 is
 // rebuilt: conj(row0 x row1) for SU(3), row0 x row1 for the real adjoint
@@ -359,9 +424,6 @@ struct LinkMat<3, T> { T m[9]; };
 // array [2 parity][4 mu][stride] of vectors of its size; the groups follow
 // one another.  Bytes per link: FP32 complex 48 (16+16+16), FP16 complex 24
 // (16+8), FP32 real 24 (16+8), FP16 real 12 (8+4).
-struct alignas(8) V8 {
-    unsigned w[2];
-};
 template <int PR, bool REAL>
 struct PackedLink {
     static_assert(PR >= 1, "packed links are low-precision only");

@@ -41,6 +41,8 @@ struct DslashParams {
     int threads;           // workgroup size (0: LYM_DSLASH_THREADS); a workgroup holds threads / nr sites
     int xslice_blocks;     // set by the launcher: site blocks per x-slice and parity when an x-slice
                            // is a whole number of blocks, else 0
+    int allow_split;       // the caller allows the two-threads-per-site form (low precision)
+    int split;             // set by the launcher: threads per half of that form, or 0
 };
 
 // link_kind: 0 complex 18, 1 complex 12, 2 real 9, 3 real 6; diag: 0 none, 1 identity, 2 clover
