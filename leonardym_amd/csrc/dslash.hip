@@ -59,6 +59,14 @@
 #ifndef LYM_SPLIT_MAX_SITES
 #define LYM_SPLIT_MAX_SITES 131072
 #endif
+// 1: XCDs 2i and 2i+1 share the x-slices of both their ranges, each taking half
+// of every slice's site blocks, so an XCD's domain is 4 x-slices by half the
+// (y, z) blocks instead of 2 whole x-slices: 3/4 of the halo spinors and
+// links.  Measured on the 16^3x32 FP64 clover apply: 252 -> 221 MB read,
+// 47.1 -> 44.6 us; even-odd launches 68.4 -> 65.8 us.
+#ifndef LYM_XCD_PAIRS
+#define LYM_XCD_PAIRS 1
+#endif
 // Full-lattice applies: odd-parity sites take each direction's backward hop
 // before its forward hop, so the two reads of a link U_mu(y) -- by y (forward)
 // and by y + mu (backward), sites of opposite parity -- happen at the same
@@ -336,6 +344,17 @@ __device__ __forceinline__ void block_of(const DslashParams& a, const BlockSweep
         const int b0 = a.parity >= 0 ? b.lo : b.lo >> 1;
         const int D = a.parity >= 0 ? b.hi - b.lo : (b.hi - b.lo) >> 1;
         const bool whole = a.parity >= 0 || ((b.lo | b.hi) & 1) == 0;
+#if LYM_XCD_PAIRS
+        // XCDs 2i and 2i+1 share the x-slices of both their ranges, each taking
+        // half of every slice's blocks (a 2-D domain: shorter x halo)
+        const int nbt = a.nblocks_per_parity * (a.parity >= 0 ? 1 : 2);
+        if (W > 0 && whole && W % 2 == 0 && nbt % 16 == 0 && b0 % W == 0 && D % W == 0) {
+            const int x1 = (blockIdx.x % 8) & 1;
+            const int b0p = b0 - x1 * D, nxs2 = 2 * D / W, Wh = W / 2;
+            const int u = bi - b0, j = u / nxs2;
+            bi = b0p + (u - j * nxs2) * W + x1 * Wh + j;
+        } else
+#endif
         if (W > 0 && whole && b0 % W == 0 && D % W == 0 && D > W) {
             const int u = bi - b0, nxs = D / W;
             const int j = u / nxs;
