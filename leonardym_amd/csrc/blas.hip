@@ -425,69 +425,76 @@ __global__ __launch_bounds__(RB_THREADS) void k_state_dot(const S* s, const zd* 
     block_store<2>(v, partials);
 }
 
-// rho_next = <rh, r>; breakdown if zero; beta = (rho_next / rho)(alpha / omega)
-__global__ __launch_bounds__(RED_BLOCKS) void k_bicg_rho_final(BiCGState* s, const double* partials) {
-    if (s->done) return;
+void launch_bicg_rho(BiCGState* s, const zd* rh, const zd* r, long n, double* partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_state_dot<BiCGState>, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, s, rh, r, n, partials);
+}
+
+// Each of the p, s and x/r updates below folds the partial sums of the dot
+// product before it in every workgroup (a fixed order, so all get the same
+// value); workgroup 0 writes the state, and no workgroup reads what it
+// writes except `done` at the start (read once per workgroup: after a stop
+// decided there a vector may be left partly updated, which nothing reads).
+#define BICG_STOP_CHECK()                        \
+    __shared__ int stop_;                        \
+    if (threadIdx.x == 0) stop_ = s->done;      \
+    __syncthreads();                             \
+    if (stop_) return
+
+// rho_next = <rh, r>; breakdown if zero; beta = (rho_next / rho)(alpha / omega);
+// p = r + beta (p - omega nu)
+__global__ __launch_bounds__(RB_THREADS) void k_bicg_p(BiCGState* s, zd* __restrict__ p, const zd* __restrict__ r,
+                                                       const zd* __restrict__ nu, long n, const double* partials) {
+    BICG_STOP_CHECK();
+    double pl[2][FOLD_PER_THREAD];
+    fold_load<2>(partials, pl);
     double out[2];
-    final_sum<2>(partials, out);
-    if (threadIdx.x == 0) {
-        zd rn = mk(out[0], out[1]);
+    fold_block<2>(pl, out);
+    const zd rn = mk(out[0], out[1]);
+    const bool brk = rn.re * rn.re + rn.im * rn.im == 0.0;
+    const zd beta = zdiv(rn, mk(s->rho_re, s->rho_im)) * zdiv(mk(s->alpha_re, s->alpha_im), mk(s->omega_re, s->omega_im));
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         s->rhon_re = rn.re;
         s->rhon_im = rn.im;
-        if (rn.re * rn.re + rn.im * rn.im == 0.0) {
+        if (brk) {
             s->done = 1;
             s->breakdown = 1;
             s->converged = 0;
-            return;
+        } else {
+            s->beta_re = beta.re;
+            s->beta_im = beta.im;
         }
-        zd beta = zdiv(rn, mk(s->rho_re, s->rho_im)) * zdiv(mk(s->alpha_re, s->alpha_im), mk(s->omega_re, s->omega_im));
-        s->beta_re = beta.re;
-        s->beta_im = beta.im;
     }
-}
-
-void launch_bicg_rho(BiCGState* s, const zd* rh, const zd* r, long n, double* partials, hipStream_t st) {
-    hipLaunchKernelGGL(k_state_dot<BiCGState>, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, s, rh, r, n, partials);
-    hipLaunchKernelGGL(k_bicg_rho_final, dim3(1), dim3(RED_BLOCKS), 0, st, s, partials);
-}
-
-// p = r + beta (p - omega nu)
-__global__ void k_bicg_p(const BiCGState* s, zd* __restrict__ p, const zd* __restrict__ r, const zd* __restrict__ nu,
-                         long n) {
-    if (s->done) return;
-    const zd beta = mk(s->beta_re, s->beta_im), om = mk(s->omega_re, s->omega_im);
+    if (brk) return;
+    const zd om = mk(s->omega_re, s->omega_im);
     GRID_LOOP(i, n) { p[i] = r[i] + beta * (p[i] - om * nu[i]); }
 }
 
-void launch_bicg_p(BiCGState* s, zd* p, const zd* r, const zd* nu, long n, hipStream_t st) {
-    hipLaunchKernelGGL(k_bicg_p, dim3(ew_blocks(n)), dim3(256), 0, st, s, p, r, nu, n);
-}
-
-__global__ __launch_bounds__(RED_BLOCKS) void k_bicg_alpha_final(BiCGState* s, const double* partials) {
-    if (s->done) return;
-    double out[2];
-    final_sum<2>(partials, out);
-    if (threadIdx.x == 0) {
-        zd a = zdiv(mk(s->rhon_re, s->rhon_im), mk(out[0], out[1]));
-        s->alpha_re = a.re;
-        s->alpha_im = a.im;
-    }
+void launch_bicg_p(BiCGState* s, zd* p, const zd* r, const zd* nu, long n, const double* partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_bicg_p, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, s, p, r, nu, n, partials);
 }
 
 void launch_bicg_alpha(BiCGState* s, const zd* rh, const zd* nu, long n, double* partials, hipStream_t st) {
     hipLaunchKernelGGL(k_state_dot<BiCGState>, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, s, rh, nu, n, partials);
-    hipLaunchKernelGGL(k_bicg_alpha_final, dim3(1), dim3(RED_BLOCKS), 0, st, s, partials);
 }
 
-__global__ void k_bicg_s(const BiCGState* s, zd* __restrict__ sv, const zd* __restrict__ r, const zd* __restrict__ nu,
-                         long n) {
-    if (s->done) return;
-    const zd al = mk(s->alpha_re, s->alpha_im);
+// alpha = rho_next / <rh, nu>; s = r - alpha nu
+__global__ __launch_bounds__(RB_THREADS) void k_bicg_s(BiCGState* s, zd* __restrict__ sv, const zd* __restrict__ r,
+                                                       const zd* __restrict__ nu, long n, const double* partials) {
+    BICG_STOP_CHECK();
+    double pl[2][FOLD_PER_THREAD];
+    fold_load<2>(partials, pl);
+    double out[2];
+    fold_block<2>(pl, out);
+    const zd al = zdiv(mk(s->rhon_re, s->rhon_im), mk(out[0], out[1]));
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        s->alpha_re = al.re;
+        s->alpha_im = al.im;
+    }
     GRID_LOOP(i, n) { sv[i] = r[i] - al * nu[i]; }
 }
 
-void launch_bicg_s(BiCGState* s, zd* sv, const zd* r, const zd* nu, long n, hipStream_t st) {
-    hipLaunchKernelGGL(k_bicg_s, dim3(ew_blocks(n)), dim3(256), 0, st, s, sv, r, nu, n);
+void launch_bicg_s(BiCGState* s, zd* sv, const zd* r, const zd* nu, long n, const double* partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_bicg_s, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, s, sv, r, nu, n, partials);
 }
 
 __global__ __launch_bounds__(RB_THREADS) void k_bicg_ts(const BiCGState* s, const zd* __restrict__ t,
@@ -503,39 +510,36 @@ __global__ __launch_bounds__(RB_THREADS) void k_bicg_ts(const BiCGState* s, cons
     block_store<4>(v, partials);
 }
 
-// omega = <t,s>/<t,t>; <t,t> == 0 -> solution = source (BiConjugateGradient.cpp:449-458)
-__global__ __launch_bounds__(RED_BLOCKS) void k_bicg_omega_final(BiCGState* s, const double* partials) {
-    if (s->done) return;
+void launch_bicg_omega(BiCGState* s, const zd* t, const zd* sv, long n, double* partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_bicg_ts, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, s, t, sv, n, partials);
+}
+
+// omega = <t,s>/<t,t>; <t,t> == 0 -> solution = source (BiConjugateGradient.cpp:449-458);
+// x += alpha p + omega s, r = s - omega t, partial |r|^2 (into rpart)
+__global__ __launch_bounds__(RB_THREADS) void k_bicg_update(BiCGState* s, zd* __restrict__ x, zd* __restrict__ r,
+                                                            const zd* __restrict__ p, const zd* __restrict__ sv,
+                                                            const zd* __restrict__ t, const zd* __restrict__ b,
+                                                            long n, const double* partials, double* rpart) {
+    BICG_STOP_CHECK();
+    double pl[4][FOLD_PER_THREAD];
+    fold_load<4>(partials, pl);
     double out[4];
-    final_sum<4>(partials, out);
-    if (threadIdx.x == 0) {
-        if (out[2] == 0.0) {
+    fold_block<4>(pl, out);
+    if (out[2] == 0.0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
             s->copy_b = 1;
             s->done = 1;
             s->converged = 1;
-            return;
         }
-        s->omega_re = out[0] / out[2];
-        s->omega_im = out[1] / out[2];
-    }
-}
-
-void launch_bicg_omega(BiCGState* s, const zd* t, const zd* sv, long n, double* partials, hipStream_t st) {
-    hipLaunchKernelGGL(k_bicg_ts, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, s, t, sv, n, partials);
-    hipLaunchKernelGGL(k_bicg_omega_final, dim3(1), dim3(RED_BLOCKS), 0, st, s, partials);
-}
-
-// x += alpha p + omega s, r = s - omega t, partial |r|^2
-__global__ __launch_bounds__(RB_THREADS) void k_bicg_update(const BiCGState* s, zd* __restrict__ x,
-                                                            zd* __restrict__ r, const zd* __restrict__ p,
-                                                            const zd* __restrict__ sv, const zd* __restrict__ t,
-                                                            const zd* __restrict__ b, long n, double* partials) {
-    if (s->copy_b) {
         GRID_LOOP(i, n) { x[i] = b[i]; }
         return;
     }
-    if (s->done) return;
-    const zd al = mk(s->alpha_re, s->alpha_im), om = mk(s->omega_re, s->omega_im);
+    const zd om = mk(out[0] / out[2], out[1] / out[2]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        s->omega_re = om.re;
+        s->omega_im = om.im;
+    }
+    const zd al = mk(s->alpha_re, s->alpha_im);
     double v[1] = {0};
     GRID_LOOP(i, n) {
         zd sv_i = sv[i];
@@ -544,7 +548,7 @@ __global__ __launch_bounds__(RB_THREADS) void k_bicg_update(const BiCGState* s, 
         r[i] = rn;
         v[0] += rn.re * rn.re + rn.im * rn.im;
     }
-    block_store<1>(v, partials);
+    block_store<1>(v, rpart);
 }
 
 __global__ __launch_bounds__(RED_BLOCKS) void k_bicg_update_final(BiCGState* s, const double* partials) {
@@ -567,8 +571,11 @@ __global__ __launch_bounds__(RED_BLOCKS) void k_bicg_update_final(BiCGState* s, 
 
 void launch_bicg_update(BiCGState* s, zd* x, zd* r, const zd* p, const zd* sv, const zd* t, const zd* b, long n,
                         double* partials, hipStream_t st) {
-    hipLaunchKernelGGL(k_bicg_update, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, s, x, r, p, sv, t, b, n, partials);
-    hipLaunchKernelGGL(k_bicg_update_final, dim3(1), dim3(RED_BLOCKS), 0, st, s, partials);
+    // <t,s>, <t,t> partials (launch_bicg_omega) in slots 0..3, |r|^2 in slot 4
+    double* rpart = partials + 4 * RED_BLOCKS;
+    hipLaunchKernelGGL(k_bicg_update, dim3(RED_BLOCKS), dim3(RB_THREADS), 0, st, s, x, r, p, sv, t, b, n, partials,
+                       rpart);
+    hipLaunchKernelGGL(k_bicg_update_final, dim3(1), dim3(RED_BLOCKS), 0, st, s, rpart);
 }
 
 }  // namespace lym

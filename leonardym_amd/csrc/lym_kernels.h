@@ -107,9 +107,11 @@ void launch_bicg_init(BiCGState* s, double precision, int max_steps, hipStream_t
 // r = b - Ax, rh = b + Ax, p = nu = 0
 void launch_bicg_setup(zd* r, zd* rh, zd* p, zd* nu, const zd* b, const zd* Ax, long n, hipStream_t st);
 void launch_bicg_rho(BiCGState* s, const zd* rh, const zd* r, long n, double* partials, hipStream_t st);
-void launch_bicg_p(BiCGState* s, zd* p, const zd* r, const zd* nu, long n, hipStream_t st);
+// the p, s and x/r updates fold the partials of the dot launched before them
+// (rho, alpha, omega) into their scalars
+void launch_bicg_p(BiCGState* s, zd* p, const zd* r, const zd* nu, long n, const double* partials, hipStream_t st);
 void launch_bicg_alpha(BiCGState* s, const zd* rh, const zd* nu, long n, double* partials, hipStream_t st);
-void launch_bicg_s(BiCGState* s, zd* sv, const zd* r, const zd* nu, long n, hipStream_t st);
+void launch_bicg_s(BiCGState* s, zd* sv, const zd* r, const zd* nu, long n, const double* partials, hipStream_t st);
 void launch_bicg_omega(BiCGState* s, const zd* t, const zd* sv, long n, double* partials, hipStream_t st);
 void launch_bicg_update(BiCGState* s, zd* x, zd* r, const zd* p, const zd* sv, const zd* t, const zd* b, long n,
                         double* partials, hipStream_t st);

@@ -345,10 +345,10 @@ int run_bicgstab(lym_solver* s, const LinOp& L, const zd* b, zd* xout, const zd*
     key << L.half << L.gamma5 << L.power << L.scratch << L.shift.re << L.shift.im;
     auto body = [&](unsigned) {
         lym::launch_bicg_rho(S, rh, r, n, c->red_partials, st);
-        lym::launch_bicg_p(S, p, r, nu, n, st);
+        lym::launch_bicg_p(S, p, r, nu, n, c->red_partials, st);
         L.apply(nu, p, &S->done);
         lym::launch_bicg_alpha(S, rh, nu, n, c->red_partials, st);
-        lym::launch_bicg_s(S, sv, r, nu, n, st);
+        lym::launch_bicg_s(S, sv, r, nu, n, c->red_partials, st);
         L.apply(t, sv, &S->done);
         lym::launch_bicg_omega(S, t, sv, n, c->red_partials, st);
         lym::launch_bicg_update(S, x, r, p, sv, t, b, n, c->red_partials, st);
