@@ -587,9 +587,40 @@ struct TuneKey {
     bool operator<(const TuneKey& o) const { return std::memcmp(this, &o, sizeof(*this)) < 0; }
 };
 // cached choice: (nr, threads, largest nr the choice was tuned over)
+// per candidate batch width: the fastest workgroup size and the time of one
+// sloppy apply of a full batch with it (0: not measured)
+constexpr int N_TUNE = 5;
+constexpr int TUNE_NR[N_TUNE] = {4, 6, 8, 12, 16};
 struct TuneVal {
-    int nr, threads, maxnr;
+    int maxnr;
+    int threads[N_TUNE];
+    double ms[N_TUNE];
 };
+
+// The batch width for a call of nrhs RHS: least total apply time over the
+// batches it makes (full batches of nr, and the last batch of the remainder
+// padded to the next supported width, at the time of the smallest measured
+// width that holds it), among the widths <= maxnr.
+void choose_batch(const TuneVal& tv, unsigned nrhs, int maxnr, int* nr_out, int* thr_out) {
+    double best = 1e300;
+    for (int i = 0; i < N_TUNE; ++i) {
+        const int nr = TUNE_NR[i];
+        if (nr > maxnr || tv.ms[i] <= 0) continue;
+        const unsigned full = nrhs / (unsigned)nr, rem = nrhs % (unsigned)nr;
+        double cost = full * tv.ms[i];
+        if (rem) {
+            int j = 0;
+            while (j < N_TUNE && (TUNE_NR[j] < (int)rem || tv.ms[j] <= 0)) ++j;
+            cost += j < N_TUNE ? tv.ms[j] : tv.ms[i];
+        }
+        if (nrhs == 0) cost = tv.ms[i] / nr;
+        if (cost < best * 0.98) {
+            best = cost;
+            *nr_out = nr;
+            *thr_out = tv.threads[i];
+        }
+    }
+}
 std::map<TuneKey, TuneVal>& tune_cache() {
     static std::map<TuneKey, TuneVal> m;
     return m;
@@ -607,9 +638,8 @@ int autotune_batch(lym_solver* s, lym_op* op, bool half, int prec, unsigned nrhs
     key.lk = lym_internal::link_kind_of(op->gauge);
     key.prec = prec;
     key.half = half;
-    const int cands[5] = {4, 6, 8, 12, 16};
     int maxnr = 4;
-    for (int k : cands)
+    for (int k : TUNE_NR)
         if ((unsigned)k <= std::max(nrhs, 4u)) maxnr = k;
     {
         std::lock_guard<std::mutex> lk(tune_mutex());
@@ -617,17 +647,22 @@ int autotune_batch(lym_solver* s, lym_op* op, bool half, int prec, unsigned nrhs
         // a choice tuned over fewer RHS is retuned
         auto it = tune_cache().find(key);
         if (it != tune_cache().end() && it->second.maxnr >= maxnr) {
-            *nr_out = std::min(it->second.nr, maxnr);
-            *thr_out = it->second.threads;
+            *nr_out = 4;
+            *thr_out = 256;
+            choose_batch(it->second, nrhs, maxnr, nr_out, thr_out);
             return LYM_OK;
         }
     }
     int rc = ensure_batch(s, maxnr, half ? 1 : 2, prec);
     if (rc) return rc;
-    double best = 1e300;
-    int bnr = 4, bthr = 256;
-    for (int nr : cands) {
-        if (nr > maxnr) break;
+    TuneVal tv;
+    tv.maxnr = maxnr;
+    for (int i = 0; i < N_TUNE; ++i) {
+        const int nr = TUNE_NR[i];
+        tv.ms[i] = 0;
+        tv.threads[i] = 256;
+        if (nr > maxnr) continue;
+        double best = 1e300;
         for (int thr : {64, 128, 256}) {
             if (thr < nr) continue;
             BatchOp B{op, half, 1, 1, nr, thr, s->bsv[2], s->bsvsc[2], s->bsv[3], s->bsvsc[3]};
@@ -638,20 +673,20 @@ int autotune_batch(lym_solver* s, lym_op* op, bool half, int prec, unsigned nrhs
             HIPCHECK(hipEventSynchronize(c->ev_stop));
             float ms = 0;
             (void)hipEventElapsedTime(&ms, c->ev_start, c->ev_stop);
-            double per = ms / (3.0 * nr);
-            if (per < best * 0.98) {
-                best = per;
-                bnr = nr;
-                bthr = thr;
+            if (ms / 3.0 < best * 0.98) {
+                best = ms / 3.0;
+                tv.ms[i] = best;
+                tv.threads[i] = thr;
             }
         }
     }
     {
         std::lock_guard<std::mutex> lk(tune_mutex());
-        tune_cache()[key] = TuneVal{bnr, bthr, maxnr};
+        tune_cache()[key] = tv;
     }
-    *nr_out = bnr;
-    *thr_out = bthr;
+    *nr_out = 4;
+    *thr_out = 256;
+    choose_batch(tv, nrhs, maxnr, nr_out, thr_out);
     return LYM_OK;
 }
 
