@@ -195,7 +195,16 @@ int op_sloppy_data(lym_op* op, int prec, bool full_rows) {
         lym::launch_convert_links(prec, g->U, op->Us, nl, g->real_links, c->stream);
         op->sloppy_lk = link_kind(g);
     }
-    if (op->Usf) lym::launch_convert_links(prec, g->U, op->Usf, nl, g->real_links, c->stream);
+    if (op->Usf) {
+        if (g->rows_rebuild && ncomp == 9) {
+            lym::launch_convert_links(prec, g->U, op->Usf, nl, g->real_links, c->stream);
+        } else {
+            // the new gauge field's Us already serves batches (compressed
+            // storage, or rows not rebuilt): a stale full-row copy must not be used
+            (void)hipFree(op->Usf);
+            op->Usf = nullptr;
+        }
+    }
     if (op->A) {
         if (!op->As) rc = dmalloc(c, &op->As, 2L * 36 * S * se);
         if (!rc && !op->As_sc) rc = dmalloc(c, (void**)&op->As_sc, 2L * S * sizeof(float));

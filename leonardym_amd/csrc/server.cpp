@@ -190,6 +190,10 @@ WState* admit(lym_server* S, int w, int li, std::unique_lock<std::mutex>& lk, in
         return &it->second;
     }
     const size_t need = state_bytes(*S->lattices[li], S->max_batch);
+    if (need > S->budget) {  // never fits, whatever is released: fail now rather than wait
+        *rc = fail(LYM_ERR_NOMEM, "lattice state of %zu bytes exceeds the HBM budget of %zu", need, S->budget);
+        return nullptr;
+    }
     for (;;) {
         while (S->used + need > S->budget) {
             int vw = -1, vl = -1;

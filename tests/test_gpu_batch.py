@@ -154,3 +154,27 @@ def test_point_sources_16x16x16x32():
             assert true_residual(case, xs[q], b) <= 1e-10
             assert np.linalg.norm((xs[q] - x64).ravel()) / np.linalg.norm(x64.ravel()) < 1e-9
             assert np.linalg.norm((X1.download() - x64).ravel()) / np.linalg.norm(x64.ravel()) < 1e-9
+
+
+def test_batch_after_gauge_storage_change():
+    """An operator that ran a batch on 18-real links and is then given the same
+    field in 12-real storage solves exactly as an operator made on the 12-real
+    field (the batch links follow the new field)."""
+    if lym.device_count() < 1:
+        pytest.fail("no HIP device visible")
+    dims = (4, 4, 4, 8)
+    full = Case(dims, seed=17)
+    comp = Case(dims, seed=17, storage=lym.LINKS_RECONSTRUCT)
+    srcs = [full.gauss() for _ in range(3)]
+    eps = [(1e-11 * np.linalg.norm(b)) ** 2 for b in srcs]
+    op = full.op(lym.OP_CLOVER)
+    x18, i18, _ = solve_batch(full, op, lym.SOLVER_PRECOND_MIXED_CG, srcs, 2, eps, max_steps=2000)
+    op.set_gauge(comp.gauge)
+    xsw, isw, _ = solve_batch(full, op, lym.SOLVER_PRECOND_MIXED_CG, srcs, 2, eps, max_steps=2000)
+    xfr, ifr, _ = solve_batch(comp, comp.op(lym.OP_CLOVER), lym.SOLVER_PRECOND_MIXED_CG, srcs, 2, eps,
+                              max_steps=2000)
+    for q, b in enumerate(srcs):
+        assert i18[q].converged and isw[q].converged and ifr[q].converged
+        assert isw[q].steps == ifr[q].steps
+        assert np.array_equal(xsw[q], xfr[q])
+        assert true_residual(full, xsw[q], b) <= 1e-10

@@ -96,11 +96,14 @@ def test_admission_releases_idle_state(fields):
         budget, used = srv.memory()
         assert used <= budget
     srv.close()
-    # a budget below one lattice's state is refused per request, with a code
-    tiny, ids = make_server(fields[:1], workers=1, hbm_budget=1 << 20, max_batch=1)
-    rid = tiny.submit(ids[0], 0.125, 1e-8, lym.gaussian_spinor(int(np.prod(LATTICES[0])), 3))
-    with pytest.raises(lym.LymError):
-        tiny.wait(rid)
+    # a budget below one lattice's state is refused per request, with a code,
+    # also when several workers ask for state at the same time
+    tiny, ids = make_server(fields[:1], workers=2, hbm_budget=1 << 20, max_batch=1)
+    rids = [tiny.submit(ids[0], kappa, 1e-8, lym.gaussian_spinor(int(np.prod(LATTICES[0])), 3))
+            for kappa in (0.125, 0.126, 0.127)]
+    for rid in rids:
+        with pytest.raises(lym.LymError):
+            tiny.wait(rid, timeout=60.0)
     tiny.close()
 
 
