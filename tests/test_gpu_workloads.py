@@ -1,0 +1,77 @@
+"""The two reference-sized workloads that the CPU oracle finishes in seconds,
+solved on the GPU and checked against the oracle's own solves:
+
+* SU(3) Wilson fermions on 8^4, links exp(i 0.4 H), kappa 0.12, periodic in
+  space and antiperiodic in time: FP64 CG on D^dagger D (ConjugateGradient.cpp:10-74
+  on the squared operator) to relative residual 1e-10;
+* SU(3) Wilson-clover, csw 1.0, on 16^3x32, links exp(i 0.5 H), kappa 0.13:
+  the even-odd driver (PreconditionedBiCGStab.cpp:43-91) with BiCGStab and
+  with CG inside, FP64 and mixed precision, to true residual 1e-10.
+"""
+import numpy as np
+import pytest
+
+from helpers import K_CLOVER, K_WILSON
+
+import leonardym_amd as lym
+from test_gpu_parity import Case
+
+pytestmark = pytest.mark.gpu
+
+
+def _need_gpu():
+    if lym.device_count() < 1:
+        pytest.fail("no HIP device visible")
+
+
+def test_wilson_8x8x8x8_cg_on_dagger_d():
+    _need_gpu()
+    case = Case((8, 8, 8, 8), eps=0.4, kappa=0.12, csw=0.0, seed=2024)
+    b = case.gauss()
+    eps = (1e-11 * np.linalg.norm(b)) ** 2
+    solver = lym.Solver(case.ctx, lym.SOLVER_CG, precision=eps, max_steps=3000)
+    X = case.spinor()
+    info = solver.solve(case.op(lym.OP_WILSON, power=2), case.spinor(b), X)
+    x = X.download()
+    # Q^2 = (gamma5 D)^2 = D^dagger D for the gamma5-hermitian Wilson operator
+    res = np.linalg.norm((b - case.orc.apply(K_WILSON, x, power=2)).ravel()) / np.linalg.norm(b.ravel())
+    xo, oinfo, ok = case.orc.solve("cg", b, eps, kind=K_WILSON, power=2, max_steps=3000)
+    err = np.linalg.norm((x - xo).ravel()) / np.linalg.norm(xo.ravel())
+    print("8^4 Wilson CG on D^dagger D: %d iterations (oracle %d), residual %.2e, error vs oracle %.2e"
+          % (info.steps, oinfo.steps, res, err))
+    assert info.converged and ok
+    assert res <= 1e-10
+    assert err <= 1e-9
+    assert abs(int(info.steps) - int(oinfo.steps)) <= 2
+
+
+@pytest.fixture(scope="module")
+def clover16():
+    """The 16^3x32 lattice, a Gaussian source and the oracle's solution (one
+    oracle solve, about 10 s of CPU, shared by the cases below)."""
+    _need_gpu()
+    case = Case((16, 16, 16, 32), eps=0.5, kappa=0.13, csw=1.0, seed=2025)
+    b = case.gauss()
+    eps = (1e-11 * np.linalg.norm(b)) ** 2
+    print("oracle solve on 16^3x32", flush=True)
+    xo, oinfo, ok = case.orc.solve("precond_bicgstab", b, eps, kind=K_CLOVER, max_steps=3000)
+    assert ok
+    return case, b, eps, xo
+
+
+@pytest.mark.parametrize("kind,sloppy", [(lym.SOLVER_PRECOND_BICGSTAB, None), (lym.SOLVER_PRECOND_CG, None),
+                                         (lym.SOLVER_PRECOND_MIXED_CG, lym.PREC_HALF)],
+                         ids=["bicgstab", "cg", "mixed_cg_fp16"])
+def test_clover_16x16x16x32_even_odd_solve(clover16, kind, sloppy):
+    case, b, eps, xo = clover16
+    solver = lym.Solver(case.ctx, kind, precision=eps, max_steps=3000, sloppy=sloppy)
+    X = case.spinor()
+    info = solver.solve(case.op(lym.OP_CLOVER), case.spinor(b), X)
+    x = X.download()
+    res = np.linalg.norm((b - case.orc.apply(K_CLOVER, x, gamma5=False)).ravel()) / np.linalg.norm(b.ravel())
+    err = np.linalg.norm((x - xo).ravel()) / np.linalg.norm(xo.ravel())
+    print("16^3x32 clover, solver kind %d: %d iterations, residual %.2e, error vs oracle %.2e"
+          % (kind, info.steps, res, err))
+    assert info.converged
+    assert res <= 1e-10
+    assert err <= 1e-9
