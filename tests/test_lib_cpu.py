@@ -120,7 +120,8 @@ def test_argument_errors_are_codes_before_any_device_work():
     import ctypes
     L = lym.lib()
     out = ctypes.c_void_p()
-    for dims, what in (((4, 4, 4, 3), b"even"), ((4, 4, 4, 1), b">= 2")):
+    # above the largest lattice (2^28 sites: 32-bit buffer offsets per field component)
+    for dims, what in (((4, 4, 4, 3), b"even"), ((4, 4, 4, 1), b">= 2"), ((256, 256, 256, 64), b"2^28")):
         d = (ctypes.c_int * 4)(*dims)
         assert L.lym_ctx_create(d, lym.REP_SU3_FUND, -1, ctypes.byref(out)) == lym.ERR_INVALID
         assert what in L.lym_last_error()
