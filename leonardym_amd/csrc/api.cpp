@@ -363,9 +363,9 @@ static int set_op_data(lym::DslashParams& a, const lym_op* op, int prec, bool ba
 }
 
 // full lattice, nr right-hand sides, storage precision prec: out = Op in
-void apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* out_sc, const void* in,
-                   const float* in_sc, int gamma5, const int* skip, const int* lane_flags, int lane_skip_value,
-                   bool batch_links) {
+int apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* out_sc, const void* in,
+                  const float* in_sc, int gamma5, const int* skip, const int* lane_flags, int lane_skip_value,
+                  bool batch_links, double* norm_partials, int norm_max) {
     lym_ctx* c = op->ctx;
     const size_t po = 12 * (size_t)c->g.stride * nr * selem(prec);
     const size_t so = (size_t)c->g.stride * nr;
@@ -389,13 +389,16 @@ void apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* 
     a.skip = skip;
     a.lane_flags = lane_flags;
     a.lane_skip_value = lane_skip_value;
-    lym::launch_dslash(a, lk, op_has_clover(op) ? 2 : 1, false, c->stream);
+    a.norm_partials = norm_partials;
+    a.norm_max = norm_max;
+    return lym::launch_dslash(a, lk, op_has_clover(op) ? 2 : 1, false, c->stream);
 }
 
 // Schur complement on odd-half nr-fields, storage precision prec, explicit even intermediate
-void apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, float* out_sc, const void* in_o,
-                    const float* in_sc, void* tmp_e, float* tmp_sc, int gamma5, const int* skip,
-                    const int* lane_flags, int lane_skip_value, bool batch_links) {
+int apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, float* out_sc, const void* in_o,
+                   const float* in_sc, void* tmp_e, float* tmp_sc, int gamma5, const int* skip,
+                   const int* lane_flags, int lane_skip_value, bool batch_links, double* norm_partials,
+                   int norm_max) {
     lym_ctx* c = op->ctx;
     lym::DslashParams a = base_params(op);
     const int lk = set_op_data(a, op, prec, batch_links);
@@ -421,17 +424,21 @@ void apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, floa
     b.out_sc[1] = out_sc;
     b.hop_sign = -1.0;
     b.gamma5 = gamma5;
-    lym::launch_dslash(b, lk, op->csw != 0.0 ? 2 : 1, false, c->stream);
+    b.norm_partials = norm_partials;
+    b.norm_max = norm_max;
+    return lym::launch_dslash(b, lk, op->csw != 0.0 ? 2 : 1, false, c->stream);
 }
 
-void apply_full_sloppy(lym_op* op, int prec, void* out, float* out_sc, const void* in, const float* in_sc, int gamma5,
-                       const int* skip) {
-    apply_full_nr(op, prec, 1, 0, out, out_sc, in, in_sc, gamma5, skip, nullptr, 0, false);
+int apply_full_sloppy(lym_op* op, int prec, void* out, float* out_sc, const void* in, const float* in_sc, int gamma5,
+                      const int* skip, double* norm_partials, int norm_max) {
+    return apply_full_nr(op, prec, 1, 0, out, out_sc, in, in_sc, gamma5, skip, nullptr, 0, false, norm_partials,
+                         norm_max);
 }
 
-void apply_schur_sloppy(lym_op* op, int prec, void* out_o, float* out_sc, const void* in_o, const float* in_sc,
-                        void* tmp_e, float* tmp_sc, int gamma5, const int* skip) {
-    apply_schur_nr(op, prec, 1, 0, out_o, out_sc, in_o, in_sc, tmp_e, tmp_sc, gamma5, skip, nullptr, 0, false);
+int apply_schur_sloppy(lym_op* op, int prec, void* out_o, float* out_sc, const void* in_o, const float* in_sc,
+                       void* tmp_e, float* tmp_sc, int gamma5, const int* skip, double* norm_partials, int norm_max) {
+    return apply_schur_nr(op, prec, 1, 0, out_o, out_sc, in_o, in_sc, tmp_e, tmp_sc, gamma5, skip, nullptr, 0, false,
+                          norm_partials, norm_max);
 }
 
 // the EO driver's source and reconstruction steps on nr-fields (FP64)

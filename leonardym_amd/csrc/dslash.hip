@@ -215,8 +215,11 @@ __device__ __forceinline__ void hop_pair(const DslashParams& a, int p, int site,
     if constexpr (pin_end) pin(acc);
 }
 
-template <int PR, int LK, int DIAG, bool INV>
-__device__ __forceinline__ void dslash_site(const DslashParams& a, int p, int site, int r) {
+// NORM (low precision only): returns |v|^2 of the site's output as computed,
+// before rounding to the storage precision; else 0.  A template parameter, so
+// kernels without it compile as if the sum did not exist.
+template <int PR, int LK, int DIAG, bool INV, bool NORM = false>
+__device__ __forceinline__ double dslash_site(const DslashParams& a, int p, int site, int r) {
     typedef typename Prec<PR>::T T;
     typedef typename Prec<PR>::S S;
     const int ST = a.g.stride;
@@ -294,6 +297,29 @@ __device__ __forceinline__ void dslash_site(const DslashParams& a, int p, int si
         for (int k = 0; k < 12; ++k) v[k] = v[k] + al * w[k];
     }
     store_spinor<PR>(SpinorOut<PR>{reinterpret_cast<S*>(a.out[p]), a.out_sc[p]}, ST, a.nr, site, r, v);
+    if constexpr (NORM) {  // the site's 24 squares in FP32, the sum over sites in FP64
+        float n2 = 0.f;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) n2 += v[k].re * v[k].re + v[k].im * v[k].im;
+        return (double)n2;
+    }
+    return 0;
+}
+
+// Sum of v over the workgroup (blockDim.x a multiple of 64, at most 1024), in
+// one fixed order; the total is written by thread 0 to out[blockIdx.x].  Every
+// thread of the workgroup must call it.
+__device__ __forceinline__ void block_sum_store(double v, double* out) {
+    __shared__ double sh[16];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sh[w];
+        out[blockIdx.x] = t;
+    }
 }
 
 // Grid: gridDim.x workgroups dealt round-robin over the 8 XCDs.  XCD x owns a
@@ -364,18 +390,33 @@ __device__ __forceinline__ void block_of(const DslashParams& a, const BlockSweep
 #endif
 }
 
-template <int PR, int LK, int DIAG, bool INV>
+// NORM: also the per-workgroup output norms (a.norm_partials)
+template <int PR, int LK, int DIAG, bool INV, bool NORM = false>
 __global__ __launch_bounds__(LYM_DSLASH_THREADS) LYM_OCC void k_dslash(DslashParams a) {
+    static_assert(!NORM || PR >= 1, "output norms are the low-precision stencil's");
     if (a.skip && *a.skip) return;
     const BlockSweep b = block_sweep(a);
     const int ls = (int)threadIdx.x / a.nr, r = (int)threadIdx.x - ls * a.nr;
-    if (ls >= a.spb) return;
-    if (a.lane_flags && a.lane_flags[r] == a.lane_skip_value) return;
-    for (int lb = b.lo + b.k; lb < b.hi; lb += b.nphys) {
-        int p, bi;
-        block_of(a, b, lb, p, bi);
-        const int site = bi * a.spb + ls;
-        if (site < a.g.Vh) dslash_site<PR, LK, DIAG, INV>(a, p, site, r);
+    if constexpr (!NORM) {
+        if (ls >= a.spb) return;
+        if (a.lane_flags && a.lane_flags[r] == a.lane_skip_value) return;
+        for (int lb = b.lo + b.k; lb < b.hi; lb += b.nphys) {
+            int p, bi;
+            block_of(a, b, lb, p, bi);
+            const int site = bi * a.spb + ls;
+            if (site < a.g.Vh) dslash_site<PR, LK, DIAG, INV>(a, p, site, r);
+        }
+    } else {
+        // one RHS, no lane flags (the launcher's condition for output norms):
+        // every thread holds a site slot, and every thread reaches the sum
+        double n2 = 0;
+        for (int lb = b.lo + b.k; lb < b.hi; lb += b.nphys) {
+            int p, bi;
+            block_of(a, b, lb, p, bi);
+            const int site = bi * a.spb + (int)threadIdx.x;
+            if (site < a.g.Vh) n2 += dslash_site<PR, LK, DIAG, INV, true>(a, p, site, 0);
+        }
+        block_sum_store(n2, a.norm_partials);
     }
 }
 
@@ -444,7 +485,7 @@ __device__ __forceinline__ void epilogue_half_lp(const DslashParams& a, int p, i
 // blockDim.x = 2 * TH, TH a multiple of 64 (the launcher's thread count), so
 // every wave lies in one half.  Every thread reaches both barriers of every
 // block of the sweep (invalid lanes carry zeros and store nothing).
-template <int PR, int LK, int DIAG, bool INV>
+template <int PR, int LK, int DIAG, bool INV, bool NORM = false>
 __global__ __launch_bounds__(2 * LYM_DSLASH_THREADS) LYM_OCC void k_dslash_split(DslashParams a) {
     static_assert(PR >= 1, "the split form is the low-precision stencil's");
     typedef float T;
@@ -459,6 +500,7 @@ __global__ __launch_bounds__(2 * LYM_DSLASH_THREADS) LYM_OCC void k_dslash_split
     const int ls = tl / a.nr, r = tl - ls * a.nr;
     const bool lane_ok = ls < a.spb && !(a.lane_flags && a.lane_flags[r] == a.lane_skip_value);
     const size_t ks = (size_t)a.g.stride * a.nr;
+    [[maybe_unused]] double n2 = 0;
     for (int lb = b.lo + b.k; lb < b.hi; lb += b.nphys) {
         int p, bi;
         block_of(a, b, lb, p, bi);
@@ -499,6 +541,12 @@ __global__ __launch_bounds__(2 * LYM_DSLASH_THREADS) LYM_OCC void k_dslash_split
 #pragma unroll
         for (int k = 0; k < 6; ++k) v[k] = mk((T)0, (T)0);
         if (ok) epilogue_half_lp<PR, LK, DIAG, INV>(a, p, site, r, h, u, v);
+        if constexpr (NORM) {  // invalid lanes hold zeros
+            float h2 = 0.f;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) h2 += v[k].re * v[k].re + v[k].im * v[k].im;
+            n2 += (double)h2;
+        }
         const unsigned o = (unsigned)site * (unsigned)a.nr + (unsigned)r;
         S e[6];
         if constexpr (PR == 2) {
@@ -522,10 +570,23 @@ __global__ __launch_bounds__(2 * LYM_DSLASH_THREADS) LYM_OCC void k_dslash_split
             if (ok) store_half_lp<PR>(reinterpret_cast<S*>(a.out[p]), ks, o, h, e);
         }
     }
+    if constexpr (NORM) block_sum_store(n2, a.norm_partials);  // every thread reaches it
 }
 
 template <int PR, int LK, int DIAG, bool INV>
 static void launch_one(const DslashParams& a, int nblocks, hipStream_t st) {
+    // output norms: asked for only on launches with a diagonal and no R^-1
+    // (the last launch of a Schur apply, a full-lattice apply)
+    if constexpr (PR >= 1 && DIAG >= 1 && !INV) {
+        if (a.norm_partials) {
+            if (a.split)
+                hipLaunchKernelGGL((k_dslash_split<PR, LK, DIAG, INV, true>), dim3(nblocks), dim3(2 * a.split), 0, st,
+                                   a);
+            else
+                hipLaunchKernelGGL((k_dslash<PR, LK, DIAG, INV, true>), dim3(nblocks), dim3(a.spb * a.nr), 0, st, a);
+            return;
+        }
+    }
     if constexpr (PR >= 1) {
         if (a.split) {
             hipLaunchKernelGGL((k_dslash_split<PR, LK, DIAG, INV>), dim3(nblocks), dim3(2 * a.split), 0, st, a);
@@ -558,7 +619,7 @@ static void launch_pr(const DslashParams& a, int link_kind, int diag, bool inv, 
     }
 }
 
-void launch_dslash(DslashParams a, int link_kind, int diag, bool inv, hipStream_t st) {
+int launch_dslash(DslashParams a, int link_kind, int diag, bool inv, hipStream_t st) {
     if (a.nr < 1) a.nr = 1;
     const int threads = (a.threads > 0 && a.threads <= LYM_DSLASH_THREADS) ? a.threads : LYM_DSLASH_THREADS;
     a.spb = threads / a.nr;
@@ -577,11 +638,16 @@ void launch_dslash(DslashParams a, int link_kind, int diag, bool inv, hipStream_
     // workgroups in flight: LYM_GRID_PER_CU per CU of the 256 CUs, sweeping
     const int gpc = LYM_GRID_PER_CU >= 0 ? LYM_GRID_PER_CU : (nblocks <= LYM_SWEEP_MAX_BLOCKS ? 1 : 0);
     if (gpc > 0 && nblocks > 256 * gpc) nblocks = 256 * gpc;
+    // output norm partials: low precision only, one per workgroup, and only
+    // when the caller's buffer holds one per workgroup of this launch
+    if (a.prec == 0 || diag == 0 || inv || a.nr != 1 || a.lane_flags || nblocks > a.norm_max)
+        a.norm_partials = nullptr;
     switch (a.prec) {
         case 0: launch_pr<0>(a, link_kind, diag, inv, nblocks, st); break;
         case 1: launch_pr<1>(a, link_kind, diag, inv, nblocks, st); break;
         default: launch_pr<2>(a, link_kind, diag, inv, nblocks, st); break;
     }
+    return a.norm_partials ? nblocks : 0;
 }
 
 }  // namespace lym

@@ -43,10 +43,13 @@ struct DslashParams {
                            // is a whole number of blocks, else 0
     int allow_split;       // the caller allows the two-threads-per-site form (low precision)
     int split;             // set by the launcher: threads per half of that form, or 0
+    double* norm_partials; // optional (FP32/FP16 only): workgroup b writes the sum of |out|^2 over
+    int norm_max;          //   its sites to norm_partials[b], when the launch has at most norm_max workgroups
 };
 
-// link_kind: 0 complex 18, 1 complex 12, 2 real 9, 3 real 6; diag: 0 none, 1 identity, 2 clover
-void launch_dslash(DslashParams a, int link_kind, int diag, bool inv, hipStream_t st);
+// link_kind: 0 complex 18, 1 complex 12, 2 real 9, 3 real 6; diag: 0 none, 1 identity, 2 clover.
+// Returns the number of output-norm partials written (0: none were asked for or they did not fit).
+int launch_dslash(DslashParams a, int link_kind, int diag, bool inv, hipStream_t st);
 
 // ---------------------------------------------------------------- layout (layout.hip)
 void launch_spinor_to_eo(const Geom& g, const zd* aos, zd* eo, hipStream_t st);
@@ -137,12 +140,16 @@ void launch_link_row2_err(const void* src, int real_links, int stride, double* p
 // mixed-precision CG with reliable updates; vectors span nsites = nparity * stride sites
 void launch_mcg_init(int prec, MCGState* s, const zd* r, cplx<float>* rs, cplx<float>* xs, void* p, float* p_sc,
                      long nsites, int stride, double* partials, double eps, double delta, int max_steps, hipStream_t st);
-// <p, Ap> partials into partials[0, RED_BLOCKS); launch_mcg_update folds them into
-// alpha and needs partials[RED_BLOCKS, 2 RED_BLOCKS) for |r_s|^2
+// <p, Ap> partials into partials[0, RED_BLOCKS); launch_mcg_update folds ndot
+// partials of <p, Ap> (those, or the stencil's output-norm partials: for
+// A = Q^2 with Q hermitian, <p, Ap> = |Q p|^2) into alpha and writes |r_s|^2
+// partials to partials[RED_BLOCKS, 2 RED_BLOCKS)
+constexpr int NORM_FOLD_MAX = 2048;  // most partials the update folds
 void launch_mcg_alpha(int prec, MCGState* s, void* p, float* p_sc, void* Ap, float* Ap_sc, long nsites, int stride,
                       double* partials, hipStream_t st);
 void launch_mcg_update(int prec, MCGState* s, cplx<float>* xs, cplx<float>* rs, void* p, float* p_sc, void* Ap,
-                       float* Ap_sc, long nsites, int stride, double* partials, hipStream_t st);
+                       float* Ap_sc, long nsites, int stride, double* partials, const double* dot_partials, int ndot,
+                       hipStream_t st);
 void launch_rel_accum(MCGState* s, zd* x, cplx<float>* xs, long nsites, int stride, hipStream_t st);
 void launch_accum_xs(zd* x, const cplx<float>* xs, long nsites, int stride, hipStream_t st);
 void launch_rel_resid(MCGState* s, zd* r, cplx<float>* rs, const zd* b, const zd* Ax, long nsites, int stride,

@@ -93,6 +93,34 @@ __device__ __forceinline__ double m_fold_block(const double pl[PART_PER_THREAD])
     return t;
 }
 
+// The same for n <= NORM_FOLD_MAX partials (the stencil's per-workgroup
+// output norms: one per workgroup of its launch); slots past n count as zero,
+// so n = RED_BLOCKS gives m_fold_block's total.
+constexpr int NORM_PER_THREAD = NORM_FOLD_MAX / MR_THREADS;
+static_assert(NORM_FOLD_MAX % MR_THREADS == 0, "norm partials per thread");
+__device__ __forceinline__ void m_partials_load_n(const double* partials, int n, double pl[NORM_PER_THREAD]) {
+#pragma unroll
+    for (int i = 0; i < NORM_PER_THREAD; ++i) {
+        const int k = (int)threadIdx.x + i * MR_THREADS;
+        pl[i] = k < n ? partials[k] : 0.0;
+    }
+}
+__device__ __forceinline__ double m_fold_block_n(const double pl[NORM_PER_THREAD]) {
+    __shared__ double sh[MR_THREADS / 64];
+    double v = 0;
+#pragma unroll
+    for (int i = 0; i < NORM_PER_THREAD; ++i) v += pl[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0;
+#pragma unroll
+    for (int w = 0; w < MR_THREADS / 64; ++w) t += sh[w];
+    __syncthreads();
+    return t;
+}
+
 #define SITE_LOOP(j, n) for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < (n); j += (long)gridDim.x * blockDim.x)
 
 // A sloppy spinor vector over nsites = nparity * stride sites: site j lives in
@@ -385,15 +413,15 @@ __global__ __launch_bounds__(MR_THREADS) void k_mcg_dot(const MCGState* s, SVec<
 }
 
 // alpha = rr_s / <p, Ap>, then x_s += alpha p, r_s -= alpha Ap, partial |r_s|^2.
-// Every workgroup folds the <p, Ap> partials itself (in one fixed order, so
-// all get the same alpha) instead of a separate one-workgroup kernel.
+// Every workgroup folds the ndot <p, Ap> partials itself (in one fixed order,
+// so all get the same alpha) instead of a separate one-workgroup kernel.
 template <int PR>
 __global__ __launch_bounds__(MR_THREADS) void k_mcg_update(MCGState* s, cplx<float>* xs, cplx<float>* rs,
                                                            SVec<PR> p, SVec<PR> Ap, long nsites, int stride,
-                                                           const double* dot_partials, double* partials) {
+                                                           const double* dot_partials, int ndot, double* partials) {
     if (s->done) return;
-    double pl[PART_PER_THREAD];
-    m_partials_load(dot_partials, pl);
+    double pl[NORM_PER_THREAD];
+    m_partials_load_n(dot_partials, ndot, pl);
     // the first site's vectors load while the partials are folded
     long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long jstep = (long)gridDim.x * blockDim.x;
@@ -404,7 +432,7 @@ __global__ __launch_bounds__(MR_THREADS) void k_mcg_update(MCGState* s, cplx<flo
         sv_load<PR>(p, stride, j, a);
         sv_load<PR>(Ap, stride, j, b);
     }
-    const double alpha = s->rr_s / m_fold_block(pl);
+    const double alpha = s->rr_s / m_fold_block_n(pl);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         s->alpha = alpha;
         s->rr_prev = s->rr_s;  // the p update of this iteration divides by it
@@ -585,16 +613,16 @@ void launch_mcg_alpha(int prec, MCGState* s, void* p, float* p_sc, void* Ap, flo
 }
 
 void launch_mcg_update(int prec, MCGState* s, cplx<float>* xs, cplx<float>* rs, void* p, float* p_sc, void* Ap,
-                       float* Ap_sc, long nsites, int stride, double* partials, hipStream_t st) {
-    // <p, Ap> partials (launch_mcg_alpha) in the first RED_BLOCKS slots, |r_s|^2 in the next
+                       float* Ap_sc, long nsites, int stride, double* partials, const double* dot_partials, int ndot,
+                       hipStream_t st) {
+    // |r_s|^2 partials into the second RED_BLOCKS slots
     double* rpart = partials + RED_BLOCKS;
     if (prec == 1)
         hipLaunchKernelGGL(k_mcg_update<1>, dim3(RED_BLOCKS), dim3(MR_THREADS), 0, st, s, xs, rs, sv<1>(p, p_sc),
-                           sv<1>(Ap, Ap_sc), nsites, stride, partials, rpart);
+                           sv<1>(Ap, Ap_sc), nsites, stride, dot_partials, ndot, rpart);
     else
         hipLaunchKernelGGL(k_mcg_update<2>, dim3(RED_BLOCKS), dim3(MR_THREADS), 0, st, s, xs, rs, sv<2>(p, p_sc),
-                           sv<2>(Ap, Ap_sc), nsites, stride, partials, rpart);
-
+                           sv<2>(Ap, Ap_sc), nsites, stride, dot_partials, ndot, rpart);
 }
 
 void launch_rel_accum(MCGState* s, zd* x, cplx<float>* xs, long nsites, int stride, hipStream_t st) {

@@ -37,18 +37,24 @@ void apply_full(lym_op* op, zd* out, const zd* in, int gamma5, const int* skip);
 void apply_full_add(lym_op* op, zd* out, const zd* in, const zd* v2, zd alpha, int gamma5, const int* skip);
 int link_kind_of(const lym_gauge* g);
 int op_sloppy(lym_op* op, int prec, bool full_rows = false);
-void apply_full_sloppy(lym_op* op, int prec, void* out, float* out_sc, const void* in, const float* in_sc, int gamma5,
-                       const int* skip);
-void apply_schur_sloppy(lym_op* op, int prec, void* out_o, float* out_sc, const void* in_o, const float* in_sc,
-                        void* tmp_e, float* tmp_sc, int gamma5, const int* skip);
+// the sloppy applies can also write per-workgroup partial sums of |out|^2
+// (return value: how many; 0 when norm_partials is null or the launch has more
+// than norm_max workgroups)
+int apply_full_sloppy(lym_op* op, int prec, void* out, float* out_sc, const void* in, const float* in_sc, int gamma5,
+                      const int* skip, double* norm_partials = nullptr, int norm_max = 0);
+int apply_schur_sloppy(lym_op* op, int prec, void* out_o, float* out_sc, const void* in_o, const float* in_sc,
+                       void* tmp_e, float* tmp_sc, int gamma5, const int* skip, double* norm_partials = nullptr,
+                       int norm_max = 0);
 // batch_links: the batched solver's low-precision links (all rows where the
 // single-RHS copy keeps rows 0, 1)
-void apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* out_sc, const void* in,
-                   const float* in_sc, int gamma5, const int* skip, const int* lane_flags = nullptr,
-                   int lane_skip_value = 0, bool batch_links = false);
-void apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, float* out_sc, const void* in_o,
-                    const float* in_sc, void* tmp_e, float* tmp_sc, int gamma5, const int* skip,
-                    const int* lane_flags = nullptr, int lane_skip_value = 0, bool batch_links = false);
+int apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* out_sc, const void* in,
+                  const float* in_sc, int gamma5, const int* skip, const int* lane_flags = nullptr,
+                  int lane_skip_value = 0, bool batch_links = false, double* norm_partials = nullptr,
+                  int norm_max = 0);
+int apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, float* out_sc, const void* in_o,
+                   const float* in_sc, void* tmp_e, float* tmp_sc, int gamma5, const int* skip,
+                   const int* lane_flags = nullptr, int lane_skip_value = 0, bool batch_links = false,
+                   double* norm_partials = nullptr, int norm_max = 0);
 void eo_source_nr(lym_op* op, int nr, const zd* b_e, const zd* b_o, zd* te, zd* src_o);
 void eo_reconstruct_nr(lym_op* op, int nr, const zd* b_e, const zd* x_o, zd* x_e);
 }  // namespace lym_internal
@@ -413,17 +419,24 @@ struct SloppyOp {
     float* mid_sc;
     void* tmp_e;   // Schur intermediate
     float* tmp_sc;
-    void apply(void* out, float* out_sc, const void* in, const float* in_sc, const int* skip) const {
-        auto one = [&](void* o, float* osc, const void* i, const float* isc) {
-            if (half) lym_internal::apply_schur_sloppy(op, prec, o, osc, i, isc, tmp_e, tmp_sc, gamma5, skip);
-            else lym_internal::apply_full_sloppy(op, prec, o, osc, i, isc, gamma5, skip);
+    // out = Op in.  With power 2 and norm_partials set, the first apply also
+    // writes per-workgroup partials of |Q in|^2 = <in, Q^2 in> (Q hermitian:
+    // gamma5 set); returns how many (0: not asked for, or too many workgroups).
+    int apply(void* out, float* out_sc, const void* in, const float* in_sc, const int* skip,
+              double* norm_partials = nullptr) const {
+        auto one = [&](void* o, float* osc, const void* i, const float* isc, double* np) {
+            if (half)
+                return lym_internal::apply_schur_sloppy(op, prec, o, osc, i, isc, tmp_e, tmp_sc, gamma5, skip, np,
+                                                        lym::NORM_FOLD_MAX);
+            return lym_internal::apply_full_sloppy(op, prec, o, osc, i, isc, gamma5, skip, np, lym::NORM_FOLD_MAX);
         };
         if (power == 1) {
-            one(out, out_sc, in, in_sc);
-        } else {
-            one(mid, mid_sc, in, in_sc);
-            one(out, out_sc, mid, mid_sc);
+            one(out, out_sc, in, in_sc, nullptr);
+            return 0;
         }
+        const int n = one(mid, mid_sc, in, in_sc, gamma5 ? norm_partials : nullptr);
+        one(out, out_sc, mid, mid_sc, nullptr);
+        return n;
     }
 };
 
@@ -453,12 +466,21 @@ int run_mixed_cg(lym_solver* s, const LinOp& L, const SloppyOp& SL, const zd* b,
     Key key;
     key << 3 << s << nsites << b << prec << s->xs << s->rs << p << Ap << p_sc << Ap_sc;
     key_op(key, L.op);
+    // LYM_FUSED_NORM=0 keeps the separate <p, Ap> launch (for comparison)
+    const char* fn_env = std::getenv("LYM_FUSED_NORM");
+    const bool fused_norm = !(fn_env && fn_env[0] == '0');
     key << L.half << L.gamma5 << L.power << L.scratch << SL.half << SL.gamma5 << SL.power << SL.mid << SL.mid_sc
-        << SL.tmp_e << SL.tmp_sc;
+        << SL.tmp_e << SL.tmp_sc << fused_norm;
+    // <p, Ap> for A = Q^2: |Q p|^2 summed by the stencil launch that writes Q p
+    // (its partials in red_partials[4 RED_BLOCKS, 4 RED_BLOCKS + NORM_FOLD_MAX)),
+    // else a separate dot over p and Ap
+    static_assert(4 * lym::RED_BLOCKS + lym::NORM_FOLD_MAX <= 8 * lym::RED_BLOCKS, "partials buffer");
+    double* norm_part = c->red_partials + 4 * lym::RED_BLOCKS;
     auto body = [&](unsigned it) {
-        SL.apply(Ap, Ap_sc, p, p_sc, &S->done);
-        lym::launch_mcg_alpha(prec, S, p, p_sc, Ap, Ap_sc, nsites, c->g.stride, c->red_partials, st);
-        lym::launch_mcg_update(prec, S, s->xs, s->rs, p, p_sc, Ap, Ap_sc, nsites, c->g.stride, c->red_partials, st);
+        const int nn = SL.apply(Ap, Ap_sc, p, p_sc, &S->done, fused_norm ? norm_part : nullptr);
+        if (nn == 0) lym::launch_mcg_alpha(prec, S, p, p_sc, Ap, Ap_sc, nsites, c->g.stride, c->red_partials, st);
+        lym::launch_mcg_update(prec, S, s->xs, s->rs, p, p_sc, Ap, Ap_sc, nsites, c->g.stride, c->red_partials,
+                               nn ? norm_part : c->red_partials, nn ? nn : lym::RED_BLOCKS, st);
         lym::launch_mcg_p(prec, S, p, p_sc, s->rs, nsites, c->g.stride, c->red_partials, st);
         if (rel_slot(it)) {
             lym::launch_rel_accum(S, x, s->xs, nsites, c->g.stride, st);
