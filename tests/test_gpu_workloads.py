@@ -75,3 +75,26 @@ def test_clover_16x16x16x32_even_odd_solve(clover16, kind, sloppy):
     assert info.converged
     assert res <= 1e-10
     assert err <= 1e-9
+
+
+@pytest.mark.parametrize("sloppy", [lym.PREC_HALF, lym.PREC_SINGLE], ids=["fp16", "fp32"])
+def test_fused_output_norm_matches_separate_dot(clover16, sloppy, monkeypatch):
+    """Mixed CG takes <p, Q^2 p> as |Q p|^2 summed inside the stencil launch
+    that writes Q p; the separate <p, Ap> launch (LYM_FUSED_NORM=0) solves the
+    same system to the same stopping rule in about as many iterations."""
+    case, b, eps, xo = clover16
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("LYM_FUSED_NORM", fused)
+        solver = lym.Solver(case.ctx, lym.SOLVER_PRECOND_MIXED_CG, precision=eps, max_steps=3000, sloppy=sloppy)
+        X = case.spinor()
+        info = solver.solve(case.op(lym.OP_CLOVER), case.spinor(b), X)
+        assert info.converged
+        out[fused] = (info.steps, X.download())
+    (s1, x1), (s0, x0) = out["1"], out["0"]
+    print("fused norm: %d iterations, separate dot: %d" % (s1, s0))
+    assert abs(int(s1) - int(s0)) <= 2
+    for x in (x1, x0):
+        res = np.linalg.norm((b - case.orc.apply(K_CLOVER, x, gamma5=False)).ravel()) / np.linalg.norm(b.ravel())
+        assert res <= 1e-10
+    assert np.linalg.norm((x1 - x0).ravel()) / np.linalg.norm(x0.ravel()) <= 1e-9
