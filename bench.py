@@ -162,7 +162,7 @@ def gpu_bench(args):
             res["batch"] = batch_bench(lym)
         if "serving" in secs:
             if args.serve_full:
-                res["serving"] = serve_bench(lym, [(lam, 500) for lam in (1, 2, 5, 10, 20, 50)])
+                res["serving"] = serve_bench(lym, [(lam, args.serve_requests) for lam in (1, 2, 5, 10, 20, 50)])
             else:
                 # the low end of the 1-50 req/s sweep with fewer requests (about 20 s)
                 res["serving"] = serve_bench(lym, [(1, 20), (5, 60), (20, 60), (50, 60)])
@@ -393,7 +393,9 @@ def serve_bench(lym, rates, workers=4):
         used = {li: 0 for li in range(len(ids))}
         rids = []
         t0 = time.perf_counter()
-        for (li, k, tol, si), ta in zip(spec, arr):
+        for q, ((li, k, tol, si), ta) in enumerate(zip(spec, arr)):
+            if q and q % 50 == 0:
+                log("bench: serving lambda %g/s: %d of %d requests submitted" % (lam, q, nreq))
             dt = t0 + ta - time.perf_counter()
             if dt > 0:
                 time.sleep(dt)
@@ -453,8 +455,9 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="only the operator applies")
     ap.add_argument("--sections", default="solve,adjoint,batch,serving",
                     help="solver benchmarks to run (comma-separated)")
+    ap.add_argument("--serve-requests", type=int, default=500, help="requests per rate with --serve-full")
     ap.add_argument("--serve-full", action="store_true",
-                    help="serving at lambda 1,2,5,10,20,50 req/s with 500 requests each (long)")
+                    help="serving at lambda 1,2,5,10,20,50 req/s, --serve-requests each (long)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run 20 plain spinor copies (known bytes) to scale the HBM byte counters")
     args = ap.parse_args()
