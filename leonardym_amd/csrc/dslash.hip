@@ -640,7 +640,9 @@ int launch_dslash(DslashParams a, int link_kind, int diag, bool inv, hipStream_t
     if (gpc > 0 && nblocks > 256 * gpc) nblocks = 256 * gpc;
     // output norm partials: low precision only, one per workgroup, and only
     // when the caller's buffer holds one per workgroup of this launch
-    if (a.prec == 0 || diag == 0 || inv || a.nr != 1 || a.lane_flags || nblocks > a.norm_max)
+    // (the workgroup sum shuffles over whole 64-lane waves)
+    if (a.prec == 0 || diag == 0 || inv || a.nr != 1 || a.lane_flags || threads % 64 != 0 ||
+        nblocks > a.norm_max)
         a.norm_partials = nullptr;
     switch (a.prec) {
         case 0: launch_pr<0>(a, link_kind, diag, inv, nblocks, st); break;
