@@ -188,7 +188,10 @@ constexpr unsigned GRAPH_ITERS = 8;
 // goes on.  This removes 5-7 mostly-idle launches from three of four
 // iterations.  GRAPH_ITERS is a multiple of REL_EVERY, so captured chunks and
 // directly queued iterations place the slots identically.
-constexpr unsigned REL_EVERY = 4;
+#ifndef LYM_REL_EVERY
+#define LYM_REL_EVERY 4
+#endif
+constexpr unsigned REL_EVERY = LYM_REL_EVERY;
 static_assert(GRAPH_ITERS % REL_EVERY == 0, "slots must line up with captured chunks");
 inline bool rel_slot(unsigned it) { return it % REL_EVERY == REL_EVERY - 1; }
 
