@@ -162,10 +162,11 @@ def gpu_bench(args):
             res["batch"] = batch_bench(lym)
         if "serving" in secs:
             if args.serve_full:
-                res["serving"] = serve_bench(lym, [(lam, args.serve_requests) for lam in (1, 2, 5, 10, 20, 50)])
+                res["serving"] = serve_bench(lym, [(lam, args.serve_requests) for lam in (1, 2, 5, 10, 20, 50)],
+                                             args.serve_workers)
             else:
                 # the low end of the 1-50 req/s sweep with fewer requests (about 20 s)
-                res["serving"] = serve_bench(lym, [(1, 20), (5, 60), (20, 60), (50, 60)])
+                res["serving"] = serve_bench(lym, [(1, 20), (5, 60), (20, 60), (50, 60)], args.serve_workers)
     if args.calibrate:
         # k_copy over a full 16^3x32 spinor: 24 * stride * 16 B read and written
         g, op, src, out, eo = sets[0]
@@ -456,6 +457,7 @@ def main():
     ap.add_argument("--sections", default="solve,adjoint,batch,serving",
                     help="solver benchmarks to run (comma-separated)")
     ap.add_argument("--serve-requests", type=int, default=500, help="requests per rate with --serve-full")
+    ap.add_argument("--serve-workers", type=int, default=4, help="server worker threads (streams)")
     ap.add_argument("--serve-full", action="store_true",
                     help="serving at lambda 1,2,5,10,20,50 req/s, --serve-requests each (long)")
     ap.add_argument("--calibrate", action="store_true",
