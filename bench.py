@@ -38,6 +38,10 @@ import os
 import sys
 import time
 
+# the CPU baseline's OpenMP threads stay on neighbouring cores (read by the
+# OpenMP runtime when the oracle library loads)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -442,7 +446,19 @@ def cpu_baseline(budget_s=10.0):
         fld.apply(K_CLOVER, psi)
     per = (time.perf_counter() - t0) / n
     cores = int(oracle().orc_num_threads())
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": BYTES_PER_SITE_FULL * V / per / 1e9, "unit": "GB/s", "cores": cores, "kind": "port",
+            "cpu_model": model, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "build": "g++ -O2 -fopenmp -march=x86-64-v3 (the reference uses -march=native; the oracle is built "
+                     "once, where it is compiled, and must also run on this host)",
             "sample": "%d FP64 clover applies on 16^3x32 (%.1f ms each) with the oracle, OpenMP %d threads"
                       % (n, per * 1e3, cores)}
 

@@ -147,6 +147,13 @@ int lym_op_set_gamma5(lym_op* op, int gamma5);          /* setGamma5 (:127) */
 int lym_op_get_params(const lym_op* op, double* kappa, double* csw, int* gamma5, int* kind, unsigned* power);
 /* multiply (DiracOperator.h:90) */
 int lym_op_multiply(lym_op* op, lym_spinor* out, const lym_spinor* in);
+/* The same operator applied in a low storage precision (no counterpart in the
+ * reference; the sloppy operator of the mixed-precision solvers): the input is
+ * rounded to prec (LYM_PREC_SINGLE, or LYM_PREC_HALF with a per-site scale),
+ * links and clover blocks are the operator's low-precision copies, arithmetic
+ * is FP32, and the result is returned in FP64.  For the even-odd operator the
+ * even half is copied through as in lym_op_multiply. */
+int lym_op_multiply_sloppy(lym_op* op, int prec, lym_spinor* out, const lym_spinor* in);
 /* multiplyAdd (DiracOperator.h:99): out = Op v1 + alpha v2 */
 int lym_op_multiply_add(lym_op* op, lym_spinor* out, const lym_spinor* v1, const lym_spinor* v2, double are,
                         double aim);
@@ -232,6 +239,21 @@ int lym_server_wait(lym_server* s, uint64_t request_id, double timeout_seconds, 
 int lym_server_release(lym_server* s, uint64_t request_id);   /* forget a completed request */
 int lym_server_drain(lym_server* s);                           /* wait until nothing is queued or running */
 int lym_server_memory(lym_server* s, size_t* budget_bytes, size_t* in_use_bytes);
+
+/* ------------------------------------------------------------------ diagnostics
+ * The workgroup -> site-block mapping of one stencil launch (no counterpart in
+ * the reference), evaluated on the host by the device's own mapping functions
+ * (no device is touched).  parity 0 or 1: a half-lattice launch of the
+ * even-odd operator; -1: a full-lattice apply.  nr right-hand sides per site,
+ * threads per workgroup (0: the default), lowp = 1: a single-RHS FP32/FP16
+ * apply (two threads per site where the lattice allows).  Returns the number
+ * of workgroups, sites per workgroup, site blocks per parity and site blocks
+ * per x-slice (0 when an x-slice is not a whole number of blocks).  visits
+ * (2 * nbpp ints, or NULL) counts how often each (parity, block) is visited;
+ * branches (3 ints, or NULL) counts the blocks mapped by each renumbering:
+ * plain order, x-slice fastest, XCD pairs. */
+int lym_launch_plan(const int dims[4], int parity, int nr, int threads, int lowp, int* grid, int* spb, int* nbpp,
+                    int* xslice_blocks, int* visits, long visits_len, int* branches);
 
 /* ------------------------------------------------------------------ host-side input preparation
  * Seeded synthetic inputs (the reference seeds from time(NULL),

@@ -79,6 +79,7 @@ SIGNATURES = {
     "lym_op_get_params": (_I, [_P, _PD, _PD, _PI, _PI, ctypes.POINTER(_U)]),
     "lym_op_multiply": (_I, [_P, _P, _P]),
     "lym_op_multiply_add": (_I, [_P, _P, _P, _P, _D, _D]),
+    "lym_op_multiply_sloppy": (_I, [_P, _I, _P, _P]),
     "lym_solver_create": (_I, [_P, _I, ctypes.POINTER(_P)]),
     "lym_solver_destroy": (None, [_P]),
     "lym_solver_set_precision": (_I, [_P, _D]),
@@ -107,6 +108,7 @@ SIGNATURES = {
     "lym_apply_bc": (_I, [_P, _I, _I, _P]),
     "lym_gen_gaussian_spinor": (_I, [_L, _U64, _P]),
     "lym_point_source": (_I, [_P, _P, _I, _I, _P]),
+    "lym_launch_plan": (_I, [_P, _I, _I, _I, _I, _PI, _PI, _PI, _PI, _PI, _L, _PI]),
 }
 
 _lib = None
@@ -144,6 +146,22 @@ def device_count():
     n = ctypes.c_int()
     rc = lib().lym_device_count(ctypes.byref(n))
     return n.value if rc == 0 else 0
+
+
+def launch_plan(dims, parity=-1, nr=1, threads=0, lowp=False):
+    """The stencil's workgroup -> site-block mapping for one launch, computed
+    on the host by the library (no device needed): dict with grid, spb, nbpp,
+    xslice_blocks, visits [2, nbpp] and branches (plain, x-slice, XCD pairs)."""
+    grid, spb, nbpp, W = (ctypes.c_int() for _ in range(4))
+    _check(lib().lym_launch_plan(_arr(np.array(dims, np.int32)), parity, nr, threads, int(lowp), ctypes.byref(grid),
+                                 ctypes.byref(spb), ctypes.byref(nbpp), ctypes.byref(W), None, 0, None))
+    visits = np.zeros(2 * nbpp.value, np.int32)
+    branches = np.zeros(3, np.int32)
+    _check(lib().lym_launch_plan(_arr(np.array(dims, np.int32)), parity, nr, threads, int(lowp), ctypes.byref(grid),
+                                 ctypes.byref(spb), ctypes.byref(nbpp), ctypes.byref(W),
+                                 visits.ctypes.data_as(_PI), visits.size, branches.ctypes.data_as(_PI)))
+    return {"grid": grid.value, "spb": spb.value, "nbpp": nbpp.value, "xslice_blocks": W.value,
+            "visits": visits.reshape(2, nbpp.value), "branches": branches}
 
 
 # ---------------------------------------------------------------- host-side inputs
@@ -369,6 +387,10 @@ class Operator:
     def multiply_add(self, out, v1, v2, alpha):
         a = complex(alpha)
         _check(lib().lym_op_multiply_add(self.h, out.h, v1.h, v2.h, a.real, a.imag))
+
+    def multiply_sloppy(self, out, inp, prec):
+        """out = Op inp in storage precision prec (PREC_SINGLE or PREC_HALF)."""
+        _check(lib().lym_op_multiply_sloppy(self.h, prec, out.h, inp.h))
 
     def close(self):
         if self.h:

@@ -813,6 +813,10 @@ void lym_op_destroy(lym_op* op) {
     if (op->tmp) (void)hipFree(op->tmp);
     if (op->tmp2) (void)hipFree(op->tmp2);
     if (op->dev_kappa) (void)hipFree(op->dev_kappa);
+    for (int i = 0; i < 4; ++i) {
+        if (op->sp[i]) (void)hipFree(op->sp[i]);
+        if (op->sp_sc[i]) (void)hipFree(op->sp_sc[i]);
+    }
     delete op;
 }
 
@@ -877,6 +881,51 @@ int lym_op_multiply(lym_op* op, lym_spinor* out, const lym_spinor* in) {
         apply_power(op, par(o, c, 1), par(i, c, 1), nullptr);
     } else {
         apply_power(op, o, i, nullptr);
+    }
+    CHECK_LAUNCH();
+    return LYM_OK;
+}
+
+int lym_op_multiply_sloppy(lym_op* op, int prec, lym_spinor* out, const lym_spinor* in) {
+    int rc = check_op_ready(op);
+    if (rc) return rc;
+    if (!out || !in || out->ctx != op->ctx || in->ctx != op->ctx) return fail(LYM_ERR_INVALID, "bad spinors");
+    if (out == in) return fail(LYM_ERR_INVALID, "multiply needs distinct input and output");
+    if (prec != LYM_PREC_SINGLE && prec != LYM_PREC_HALF)
+        return fail(LYM_ERR_INVALID, "storage precision must be LYM_PREC_SINGLE or LYM_PREC_HALF");
+    rc = op_sloppy_data(op, prec, false);
+    if (rc) return rc;
+    lym_ctx* c = op->ctx;
+    const size_t S = c->g.stride;
+    for (int i = 0; i < 4; ++i) {
+        if (!op->sp[i] && (rc = dmalloc(c, &op->sp[i], 24 * S * sizeof(lym::cplx<float>)))) return rc;
+        if (!op->sp_sc[i] && (rc = dmalloc(c, (void**)&op->sp_sc[i], 2 * S * sizeof(float)))) return rc;
+    }
+    const size_t es = prec == LYM_PREC_SINGLE ? 8 : 4;  // bytes per complex component
+    auto po = [&](void* f, int p) { return (void*)((char*)f + p * 12 * S * es); };
+    auto ps = [&](float* f, int p) { return f + p * S; };
+    zd* o = reinterpret_cast<zd*>(out->d);
+    const zd* i = reinterpret_cast<const zd*>(in->d);
+    lym::launch_spinor_to_sloppy(prec, i, op->sp[0], op->sp_sc[0], 2 * (long)S, (int)S, c->stream);
+    if (op->kind == LYM_OP_EO_CLOVER) {
+        // even sites copied through (FP64), odd sites: the Schur complement on
+        // the odd half in the storage precision
+        void* mid = op->power == 2 ? op->sp[2] : op->sp[1];
+        float* mid_sc = op->power == 2 ? op->sp_sc[2] : op->sp_sc[1];
+        lym_internal::apply_schur_sloppy(op, prec, po(mid, 1), ps(mid_sc, 1), po(op->sp[0], 1), ps(op->sp_sc[0], 1), op->sp[3],
+                           op->sp_sc[3], op->gamma5, nullptr, nullptr, 0);
+        if (op->power == 2)
+            lym_internal::apply_schur_sloppy(op, prec, po(op->sp[1], 1), ps(op->sp_sc[1], 1), po(mid, 1), ps(mid_sc, 1),
+                               op->sp[3], op->sp_sc[3], op->gamma5, nullptr, nullptr, 0);
+        lym::launch_spinor_from_sloppy(prec, po(op->sp[1], 1), ps(op->sp_sc[1], 1), par(o, c, 1), (long)S, (int)S,
+                                       c->stream);
+        HIPCHECK(hipMemcpyAsync(o, i, sizeof(zd) * half_n(c), hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        void* mid = op->power == 2 ? op->sp[2] : op->sp[1];
+        float* mid_sc = op->power == 2 ? op->sp_sc[2] : op->sp_sc[1];
+        lym_internal::apply_full_sloppy(op, prec, mid, mid_sc, op->sp[0], op->sp_sc[0], op->gamma5, nullptr, nullptr, 0);
+        if (op->power == 2) lym_internal::apply_full_sloppy(op, prec, op->sp[1], op->sp_sc[1], mid, mid_sc, op->gamma5, nullptr, nullptr, 0);
+        lym::launch_spinor_from_sloppy(prec, op->sp[1], op->sp_sc[1], o, 2 * (long)S, (int)S, c->stream);
     }
     CHECK_LAUNCH();
     return LYM_OK;

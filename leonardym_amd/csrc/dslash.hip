@@ -328,17 +328,19 @@ __device__ __forceinline__ void block_sum_store(double v, double* out) {
 // in order, so at any time an XCD works on one compact slab and the
 // neighbour spinors and links of that slab are reused from its L2.  A
 // workgroup holds `spb` sites x `nr` right-hand sides, rhs fastest.
+// (The mapping functions take the workgroup index and grid size as arguments so
+// that the host can evaluate the same mapping: lym_launch_plan below.)
 struct BlockSweep {
     int lo, hi, k, nphys;
 };
-__device__ __forceinline__ BlockSweep block_sweep(const DslashParams& a) {
+__host__ __device__ __forceinline__ BlockSweep block_sweep_at(const DslashParams& a, int bx, int gdim) {
     const int nbt = a.nblocks_per_parity * (a.parity >= 0 ? 1 : 2);
     BlockSweep b;
 #if LYM_XCD_MAP
     const int nx = 8;
-    const int xcd = blockIdx.x % nx;
-    b.k = blockIdx.x / nx;
-    b.nphys = ((int)gridDim.x - xcd + nx - 1) / nx;  // workgroups on this XCD
+    const int xcd = bx % nx;
+    b.k = bx / nx;
+    b.nphys = (gdim - xcd + nx - 1) / nx;  // workgroups on this XCD
     const int q = nbt / nx, rr = nbt % nx;
     b.lo = xcd < rr ? xcd * (q + 1) : rr * (q + 1) + (xcd - rr) * q;
     b.hi = b.lo + (xcd < rr ? q + 1 : q);
@@ -346,16 +348,22 @@ __device__ __forceinline__ BlockSweep block_sweep(const DslashParams& a) {
     // plain order: consecutive workgroups (dealt round-robin over the XCDs) take
     // consecutive site blocks, so all XCDs advance one front through the lattice
     // and the neighbour slices they re-read are recent in the shared Infinity Cache
-    b.k = blockIdx.x;
-    b.nphys = (int)gridDim.x;
+    b.k = bx;
+    b.nphys = gdim;
     b.lo = 0;
     b.hi = nbt;
 #endif
     return b;
 }
 
-// logical block lb of the sweep -> parity p and site block bi of that parity
-__device__ __forceinline__ void block_of(const DslashParams& a, const BlockSweep& b, int lb, int& p, int& bi) {
+// Which renumbering block_of_at applied (reported by lym_launch_plan)
+enum { MAP_PLAIN = 0, MAP_XSLICE = 1, MAP_XCD_PAIRS = 2 };
+
+// logical block lb of the sweep of workgroup bx -> parity p and site block bi
+// of that parity; returns the MAP_* branch taken
+__host__ __device__ __forceinline__ int block_of_at(const DslashParams& a, const BlockSweep& b, int lb, int bx, int& p,
+                                                    int& bi) {
+    (void)bx;
     if (a.parity >= 0) {
         p = a.parity;
         bi = lb;
@@ -375,19 +383,29 @@ __device__ __forceinline__ void block_of(const DslashParams& a, const BlockSweep
         // half of every slice's blocks (a 2-D domain: shorter x halo)
         const int nbt = a.nblocks_per_parity * (a.parity >= 0 ? 1 : 2);
         if (W > 0 && whole && W % 2 == 0 && nbt % 16 == 0 && b0 % W == 0 && D % W == 0) {
-            const int x1 = (blockIdx.x % 8) & 1;
+            const int x1 = (bx % 8) & 1;
             const int b0p = b0 - x1 * D, nxs2 = 2 * D / W, Wh = W / 2;
             const int u = bi - b0, j = u / nxs2;
             bi = b0p + (u - j * nxs2) * W + x1 * Wh + j;
-        } else
+            return MAP_XCD_PAIRS;
+        }
 #endif
         if (W > 0 && whole && b0 % W == 0 && D % W == 0 && D > W) {
             const int u = bi - b0, nxs = D / W;
             const int j = u / nxs;
             bi = b0 + (u - j * nxs) * W + j;
+            return MAP_XSLICE;
         }
     }
 #endif
+    return MAP_PLAIN;
+}
+
+__device__ __forceinline__ BlockSweep block_sweep(const DslashParams& a) {
+    return block_sweep_at(a, (int)blockIdx.x, (int)gridDim.x);
+}
+__device__ __forceinline__ void block_of(const DslashParams& a, const BlockSweep& b, int lb, int& p, int& bi) {
+    (void)block_of_at(a, b, lb, (int)blockIdx.x, p, bi);
 }
 
 // NORM: also the per-workgroup output norms (a.norm_partials)
@@ -619,7 +637,9 @@ static void launch_pr(const DslashParams& a, int link_kind, int diag, bool inv, 
     }
 }
 
-int launch_dslash(DslashParams a, int link_kind, int diag, bool inv, hipStream_t st) {
+// Launch geometry: sites per workgroup, the split form, site blocks per parity
+// and per x-slice; returns the number of workgroups.
+static int plan_dslash(DslashParams& a) {
     if (a.nr < 1) a.nr = 1;
     const int threads = (a.threads > 0 && a.threads <= LYM_DSLASH_THREADS) ? a.threads : LYM_DSLASH_THREADS;
     a.spb = threads / a.nr;
@@ -638,6 +658,12 @@ int launch_dslash(DslashParams a, int link_kind, int diag, bool inv, hipStream_t
     // workgroups in flight: LYM_GRID_PER_CU per CU of the 256 CUs, sweeping
     const int gpc = LYM_GRID_PER_CU >= 0 ? LYM_GRID_PER_CU : (nblocks <= LYM_SWEEP_MAX_BLOCKS ? 1 : 0);
     if (gpc > 0 && nblocks > 256 * gpc) nblocks = 256 * gpc;
+    return nblocks;
+}
+
+int launch_dslash(DslashParams a, int link_kind, int diag, bool inv, hipStream_t st) {
+    const int threads = (a.threads > 0 && a.threads <= LYM_DSLASH_THREADS) ? a.threads : LYM_DSLASH_THREADS;
+    const int nblocks = plan_dslash(a);
     // output norm partials: low precision only, one per workgroup, and only
     // when the caller's buffer holds one per workgroup of this launch
     // (the workgroup sum shuffles over whole 64-lane waves)
@@ -653,3 +679,59 @@ int launch_dslash(DslashParams a, int link_kind, int diag, bool inv, hipStream_t
 }
 
 }  // namespace lym
+
+// ---------------------------------------------------------------- launch plan (host)
+#include <cstring>
+
+#include "../../include/lym.h"
+
+namespace lym_internal {
+int fail(int code, const char* fmt, ...);
+}
+
+extern "C" int lym_launch_plan(const int dims[4], int parity, int nr, int threads, int lowp, int* grid, int* spb,
+                               int* nbpp, int* xslice_blocks, int* visits, long visits_len, int* branches) {
+    using lym_internal::fail;
+    if (!dims || !grid || !spb || !nbpp || !xslice_blocks) return fail(LYM_ERR_INVALID, "null argument");
+    for (int i = 0; i < 4; ++i)
+        if (dims[i] < 2) return fail(LYM_ERR_INVALID, "lattice extent %d must be >= 2", dims[i]);
+    if (dims[3] % 2) return fail(LYM_ERR_INVALID, "Lt must be even");
+    if (parity < -1 || parity > 1 || nr < 1 || nr > 16 || threads < 0 || threads > LYM_DSLASH_THREADS)
+        return fail(LYM_ERR_INVALID, "bad launch arguments");
+    const long V = (long)dims[0] * dims[1] * dims[2] * dims[3];
+    if (V / 2 > (1L << 27) - 64) return fail(LYM_ERR_INVALID, "lattice too large");
+    lym::DslashParams a;
+    std::memset(&a, 0, sizeof(a));
+    for (int i = 0; i < 4; ++i) a.g.L[i] = dims[i];
+    a.g.Lth = dims[3] / 2;
+    a.g.Lxh = dims[0] / 2;
+    a.g.Vh = (int)(V / 2);
+    a.g.stride = (int)((V / 2 + 63) / 64 * 64);
+    a.g.bc_t = -1;
+    a.nr = nr;
+    a.threads = threads;
+    a.parity = parity;
+    a.prec = lowp ? 2 : 0;
+    a.allow_split = lowp ? 1 : 0;
+    const int gdim = lym::plan_dslash(a);
+    *grid = gdim;
+    *spb = a.spb;
+    *nbpp = a.nblocks_per_parity;
+    *xslice_blocks = a.xslice_blocks;
+    if (visits && visits_len < 2L * a.nblocks_per_parity) return fail(LYM_ERR_INVALID, "visits array too short");
+    if (visits) std::memset(visits, 0, sizeof(int) * 2 * (size_t)a.nblocks_per_parity);
+    if (branches) branches[0] = branches[1] = branches[2] = 0;
+    for (int bx = 0; bx < gdim; ++bx) {
+        const lym::BlockSweep b = lym::block_sweep_at(a, bx, gdim);
+        for (int lb = b.lo + b.k; lb < b.hi; lb += b.nphys) {
+            int p = 0, bi = 0;
+            const int br = lym::block_of_at(a, b, lb, bx, p, bi);
+            if (p < 0 || p > 1 || bi < 0 || bi >= a.nblocks_per_parity)
+                return fail(LYM_ERR_INVALID, "workgroup %d maps block %d to (%d, %d), outside the lattice", bx, lb, p,
+                            bi);
+            if (visits) ++visits[p * a.nblocks_per_parity + bi];
+            if (branches) ++branches[br];
+        }
+    }
+    return LYM_OK;
+}

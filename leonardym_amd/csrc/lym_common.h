@@ -195,4 +195,8 @@ struct lym_op {
     float* As_sc;      // [2][stride]
     void* Ainvs;       // [36][stride]
     float* Ainvs_sc;   // [stride]
+    // spinor fields of lym_op_multiply_sloppy (in, out, Q^2 intermediate, even
+    // Schur intermediate), sized for FP32 storage; allocated on first use
+    void* sp[4];
+    float* sp_sc[4];   // [2][stride] FP16 scales
 };

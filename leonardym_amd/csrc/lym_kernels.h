@@ -137,6 +137,12 @@ void launch_pack_links(int prec, const void* src, int nc, int real_links, void* 
 // per-workgroup maxima (RED_BLOCKS of them) of the deviation of row 2 of
 // 18-real / 9-real FP64 links from the row rebuilt out of rows 0 and 1
 void launch_link_row2_err(const void* src, int real_links, int stride, double* partials, hipStream_t st);
+// FP64 spinor field <-> sloppy field of storage precision prec (FP16: with its
+// per-site scales), nsites = nparity * stride sites
+void launch_spinor_to_sloppy(int prec, const zd* src, void* dst, float* dst_sc, long nsites, int stride,
+                             hipStream_t st);
+void launch_spinor_from_sloppy(int prec, const void* src, const float* src_sc, zd* dst, long nsites, int stride,
+                               hipStream_t st);
 // mixed-precision CG with reliable updates; vectors span nsites = nparity * stride sites
 void launch_mcg_init(int prec, MCGState* s, const zd* r, cplx<float>* rs, cplx<float>* xs, void* p, float* p_sc,
                      long nsites, int stride, double* partials, double eps, double delta, int max_steps, hipStream_t st);

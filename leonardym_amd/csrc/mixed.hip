@@ -352,6 +352,53 @@ void launch_convert_blocks(int prec, const zd* src, void* dst, float* sc, int np
         hipLaunchKernelGGL(k_convert_blocks<2>, dim3(site_blocks(n)), dim3(256), 0, st, src, dst, sc, nparity, stride);
 }
 
+// ---------------------------------------------------------------- spinor precision conversion
+// FP64 field -> sloppy field (rounded to the storage precision, FP16 with its
+// per-site scale) and back, over nsites = nparity * stride sites
+template <int PR>
+__global__ void k_spinor_to_sloppy(const zd* src, SVec<PR> dst, long nsites, int stride) {
+    SITE_LOOP(j, nsites) {
+        zd a[12];
+        cplx<float> f[12];
+        d_load(src, stride, j, a);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) f[k] = mk((float)a[k].re, (float)a[k].im);
+        sv_store<PR>(dst, stride, j, f);
+    }
+}
+template <int PR>
+__global__ void k_spinor_from_sloppy(SVec<PR> src, zd* dst, long nsites, int stride) {
+    SITE_LOOP(j, nsites) {
+        cplx<float> f[12];
+        zd a[12];
+        sv_load<PR>(src, stride, j, f);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) a[k] = mk((double)f[k].re, (double)f[k].im);
+        d_store(dst, stride, j, a);
+    }
+}
+
+void launch_spinor_to_sloppy(int prec, const zd* src, void* dst, float* dst_sc, long nsites, int stride,
+                             hipStream_t st) {
+    if (prec == 1)
+        hipLaunchKernelGGL(k_spinor_to_sloppy<1>, dim3(site_blocks(nsites)), dim3(256), 0, st, src,
+                           SVec<1>{reinterpret_cast<cplx<float>*>(dst), nullptr}, nsites, stride);
+    else
+        hipLaunchKernelGGL(k_spinor_to_sloppy<2>, dim3(site_blocks(nsites)), dim3(256), 0, st, src,
+                           SVec<2>{reinterpret_cast<h2_t*>(dst), dst_sc}, nsites, stride);
+}
+void launch_spinor_from_sloppy(int prec, const void* src, const float* src_sc, zd* dst, long nsites, int stride,
+                               hipStream_t st) {
+    if (prec == 1)
+        hipLaunchKernelGGL(k_spinor_from_sloppy<1>, dim3(site_blocks(nsites)), dim3(256), 0, st,
+                           SVec<1>{reinterpret_cast<cplx<float>*>(const_cast<void*>(src)), nullptr}, dst, nsites,
+                           stride);
+    else
+        hipLaunchKernelGGL(k_spinor_from_sloppy<2>, dim3(site_blocks(nsites)), dim3(256), 0, st,
+                           SVec<2>{reinterpret_cast<h2_t*>(const_cast<void*>(src)), const_cast<float*>(src_sc)},
+                           dst, nsites, stride);
+}
+
 // ---------------------------------------------------------------- mixed CG
 // r_s = r (FP32), p = r (sloppy), x_s = 0, partial |r|^2
 template <int PR>
@@ -390,8 +437,10 @@ __global__ __launch_bounds__(RED_BLOCKS) void k_mcg_init_final(MCGState* s, cons
         s->nrel = 0;
         s->rel = 0;
         s->skip_rel = 1;
-        s->converged = 0;
-        s->done = max_steps <= 0 ? 1 : 0;
+        // a zero initial residual (zero source, or a guess that solves exactly)
+        // is converged at step 0: the first alpha would be 0/0
+        s->converged = out[0] == 0.0 ? 1 : 0;
+        s->done = (max_steps <= 0 || out[0] == 0.0) ? 1 : 0;
         s->alpha = s->beta = 0;
     }
 }

@@ -6,7 +6,8 @@
 //
 // usage: dropin_check Lx Ly Lz Lt kappa csw links.bin out.bin
 //   links.bin: V x 4 x 9 complex<double> (column-major, BC folded)
-//   out.bin  : 12 solutions, then Q psi, D psi, Q psi + a psi2, Q^2 psi, CG solution
+//   out.bin  : 12 solutions, then Q psi, D psi, Q psi + a psi2, Q^2 psi, CG solution,
+//              getSquareRoot(Q^2) psi
 //              (each V x 12 complex<double>), psi and psi2 seeded in the program
 #include <cstdio>
 #include <cstdlib>
@@ -95,6 +96,11 @@ int main(int argc, char** argv) {
         }
         fprintf(stderr, "CG: %u steps, last error %g\n", cg.getLastSteps(), cg.getLastError());
         write(out, r);
+        // getSquareRoot of the squared operator: back to Q (DiracOperator.cpp:184-229)
+        DiracOperator* root = DiracOperator::getSquareRoot(sq);
+        root->multiply(r, psi);
+        write(out, r);
+        delete root;
         fclose(out);
         delete sq;
         delete solver;

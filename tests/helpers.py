@@ -101,7 +101,7 @@ def colmajor_to_links(a):
 class OracleField:
     """A gauge field prepared for the oracle (col-major links, F, R_ee^-1)."""
 
-    def __init__(self, dims, U, kappa=0.125, csw=0.0, need_clover=False):
+    def __init__(self, dims, U, kappa=0.125, csw=0.0, need_clover=False, need_inverse=True):
         self.dims = tuple(dims)
         self.V = int(np.prod(dims))
         self.U = U
@@ -111,12 +111,16 @@ class OracleField:
         self.F = None
         self.inv = None
         if need_clover:
-            self.build_clover()
+            self.build_clover(need_inverse)
 
-    def build_clover(self):
+    def build_clover(self, need_inverse=True):
+        """F for the clover term; the per-site 12x12 inverses R^-1 (needed by
+        the even-odd operator and driver only) unless need_inverse is False."""
         lib = oracle()
         self.F = np.zeros((self.V, 6, 3, 3), complex)
         lib.orc_field_strength(_p(np.array(self.dims, np.int32)), _p(self.Ucm), _p(self.F))
+        if not need_inverse:
+            return
         self.inv = np.zeros((self.V, 12, 12), complex)
         bad = lib.orc_clover_inverse(_p(np.array(self.dims, np.int32)), _p(self.F), self.kappa, self.csw,
                                      _p(self.inv))

@@ -32,7 +32,7 @@ def test_dropin_driver_matches_oracle(tmp_path):
     assert r.returncode == 0, r.stderr
     assert "expected error" in r.stderr
     res = np.fromfile(out, dtype=np.complex128).reshape(-1, V, 4, 3)
-    assert res.shape[0] == 17
+    assert res.shape[0] == 18
     orc = OracleField(dims, U, kappa=kappa, csw=csw, need_clover=True)
     # 12 propagator columns: D x = e_(alpha, c) at the origin
     for k in range(12):
@@ -48,3 +48,5 @@ def test_dropin_driver_matches_oracle(tmp_path):
     assert rel_err_per_site(res[15], orc.apply(K_CLOVER, psi, power=2)) < 1e-12
     Q2x = orc.apply(K_CLOVER, res[16], power=2)
     assert np.linalg.norm(Q2x - psi) / np.linalg.norm(psi) <= 1e-10
+    # getSquareRoot(getSquare(Q)) applies Q again
+    assert rel_err_per_site(res[17], orc.apply(K_CLOVER, psi)) < 1e-13

@@ -426,3 +426,25 @@ def test_shifted_bicgstab(case, kind, power, gamma5):
                                                              case.spinor())
     with pytest.raises(lym.LymError):
         s.solve(case.op(lym.OP_EO_CLOVER), case.spinor(b), case.spinor())
+
+
+@pytest.mark.parametrize("kind,op_kind,power", [(lym.SOLVER_MIXED_CG, lym.OP_CLOVER, 2),
+                                                (lym.SOLVER_PRECOND_MIXED_CG, lym.OP_CLOVER, 1)])
+@pytest.mark.parametrize("sloppy", [lym.PREC_SINGLE, lym.PREC_HALF])
+def test_mixed_cg_zero_source(case, kind, op_kind, power, sloppy):
+    """A zero source has the solution x = 0: the mixed-precision CG stops at
+    step 0 with a finite, all-zero solution, as the batched form does (the
+    first alpha would otherwise be 0/0)."""
+    z = np.zeros((case.V, 4, 3), complex)
+    solver = lym.Solver(case.ctx, kind, precision=1e-20, max_steps=200, sloppy=sloppy)
+    X = case.spinor(np.full((case.V, 4, 3), 7.0 + 1j))
+    info = solver.solve(case.op(op_kind, power=power), case.spinor(z), X)
+    x = X.download()
+    assert info.converged and info.steps == 0
+    assert np.all(np.isfinite(x)) and np.count_nonzero(x) == 0
+    # the solver is reusable afterwards on a real source
+    b = case.gauss()
+    eps = (1e-11 * np.linalg.norm(b)) ** 2
+    solver = lym.Solver(case.ctx, kind, precision=eps, max_steps=20000, sloppy=sloppy)
+    info = solver.solve(case.op(op_kind, power=power), case.spinor(b), X)
+    assert info.converged and info.steps > 0

@@ -85,6 +85,47 @@ def test_su2_adjoint_map():
     assert np.max(np.abs(U @ np.conj(np.swapaxes(U, 1, 2)) - np.eye(2))) < 1e-14
 
 
+def test_su2_adjoint_closed_form():
+    """ConvertLattice::toAdjoint (source/utils/ConvertLattice.h:24-34) with the
+    generators T_a = sigma_a / 2 in the order sigma_x, sigma_y, sigma_z
+    (source/utils/LieGenerators.h:18-25), checked entry by entry, signs
+    included, against the quaternion form of the rotation: for
+    U = u0 + i (u1 sigma_1 + u2 sigma_2 + u3 sigma_3),
+        R = (u0^2 - |u|^2) 1 + 2 u u^T - 2 u0 [u]_x,
+    with [u]_x the cross-product matrix.  A permutation of the generators or
+    a flipped sign changes off-diagonal entries and fails here (the group
+    properties alone do not see it).  The reference's element (0, 1) for
+    U = diag(e^{i th/2}, e^{-i th/2}) is +sin th; the form above agrees."""
+    U = lym.gen_su2_links((2, 2, 2, 4), 0.9, 17).reshape(-1, 2, 2)
+    R = lym.su2_to_adjoint(U)
+    u0, u3 = U[:, 0, 0].real, U[:, 0, 0].imag
+    u2, u1 = U[:, 0, 1].real, U[:, 0, 1].imag
+    # the parametrisation holds for these links (SU(2): U10 = -conj(U01), U11 = conj(U00))
+    assert np.max(np.abs(U[:, 1, 0] - (1j * u1 - u2))) < 1e-14
+    assert np.max(np.abs(U[:, 1, 1] - (u0 - 1j * u3))) < 1e-14
+    u = np.stack([u1, u2, u3], axis=1)
+    cross = np.zeros((len(U), 3, 3))
+    cross[:, 0, 1], cross[:, 0, 2], cross[:, 1, 2] = -u[:, 2], u[:, 1], -u[:, 0]
+    cross -= np.swapaxes(cross, 1, 2)
+    ref = ((u0 ** 2 - np.sum(u * u, axis=1))[:, None, None] * np.eye(3) + 2 * u[:, :, None] * u[:, None, :]
+           - 2 * u0[:, None, None] * cross)
+    assert np.max(np.abs(R - ref)) < 1e-14
+    # rotation by th about each axis k, with its sign: R_ij = cos th (i, j != k),
+    # R_{k+1, k+2} = +sin th (indices mod 3)
+    th = 0.61
+    c, s = np.cos(th), np.sin(th)
+    sig = [np.array([[0, 1], [1, 0]], complex), np.array([[0, -1j], [1j, 0]]), np.diag([1.0 + 0j, -1.0])]
+    for k in range(3):
+        uk = np.cos(th / 2) * np.eye(2) + 1j * np.sin(th / 2) * sig[k]
+        r = lym.su2_to_adjoint(uk[None])[0]
+        i, j = (k + 1) % 3, (k + 2) % 3
+        want = np.zeros((3, 3))
+        want[k, k] = 1
+        want[i, i] = want[j, j] = c
+        want[i, j], want[j, i] = s, -s
+        assert np.max(np.abs(r - want)) < 1e-15, (k, r)
+
+
 def test_bc_and_sources():
     dims = (2, 2, 4, 6)
     U = lym.gen_su3_links(dims, 0.5, 3)
@@ -136,3 +177,29 @@ def test_argument_errors_are_codes_before_any_device_work():
     assert L.lym_server_create(4, 0, 17, ctypes.byref(out)) == lym.ERR_INVALID
     assert L.lym_server_submit(None, 0, 0.1, 1e-8, None, None, None) == lym.ERR_INVALID
     assert L.lym_host_alloc(0, ctypes.byref(out)) == lym.OK and not out.value
+
+
+def test_stencil_block_mapping_is_a_bijection():
+    """Every launch shape of the stencil visits each (parity, site block) of
+    its lattice exactly once: the workgroup -> block mapping (XCD ranges,
+    sweeping grids, x-slice-fastest and XCD-pair renumberings) evaluated on the
+    host by the library's own device functions.  Covers single-RHS and batched
+    widths, workgroup sizes, full and half-lattice launches, and lattices that
+    take each renumbering."""
+    import itertools
+    seen = np.zeros(3, int)
+    lattices = [(4, 4, 4, 4), (6, 6, 6, 8), (8, 8, 8, 8), (4, 6, 2, 8), (12, 12, 12, 12), (16, 16, 16, 32),
+                (16, 8, 12, 16), (18, 4, 4, 32), (10, 4, 4, 32), (8, 4, 8, 32), (16, 4, 4, 12), (24, 24, 24, 48),
+                (24, 24, 24, 32), (32, 32, 32, 64), (5, 7, 3, 6)]
+    for dims, parity, (nr, threads) in itertools.product(
+            lattices, (-1, 0, 1), [(1, 0), (1, 64), (1, 128), (2, 0), (3, 192), (4, 128), (6, 0), (8, 0), (12, 0),
+                                   (16, 256)]):
+        for lowp in ((False, True) if nr == 1 else (False,)):
+            r = lym.launch_plan(dims, parity, nr=nr, threads=threads, lowp=lowp)
+            v = r["visits"]
+            want = np.zeros_like(v)
+            want[[0, 1] if parity < 0 else [parity]] = 1
+            assert np.array_equal(v, want), (dims, parity, nr, threads, lowp)
+            assert r["spb"] * r["nbpp"] >= int(np.prod(dims)) // 2 > r["spb"] * (r["nbpp"] - 1)
+            seen += r["branches"] > 0
+    assert np.all(seen > 0)  # every renumbering ran in at least one shape
