@@ -166,11 +166,12 @@ def gpu_bench(args):
             res["batch"] = batch_bench(lym)
         if "serving" in secs:
             if args.serve_full:
-                res["serving"] = serve_bench(lym, [(lam, args.serve_requests) for lam in (1, 2, 5, 10, 20, 50)],
-                                             args.serve_workers)
+                res["serving"] = serve_bench(lym, [(lam, args.serve_requests) for lam in (1, 2, 5, 10, 20, 50)]
+                                             + [(float("inf"), args.serve_requests)], args.serve_workers)
             else:
-                # the low end of the 1-50 req/s sweep with fewer requests (about 20 s)
-                res["serving"] = serve_bench(lym, [(1, 20), (5, 60), (20, 60), (50, 60)], args.serve_workers)
+                # the 1-50 req/s sweep with fewer requests at the low rates (about 50 s), then a burst
+                res["serving"] = serve_bench(lym, [(1, 15), (2, 20), (5, 40), (10, 60), (20, 80), (50, 120),
+                                                   (float("inf"), 120)], args.serve_workers)
     if args.calibrate:
         # k_copy over a full 16^3x32 spinor: 24 * stride * 16 B read and written
         g, op, src, out, eo = sets[0]
@@ -366,10 +367,11 @@ def batch_bench(lym):
 SERVE_LATTICES = [(8, 8, 8, 8), (12, 12, 12, 12), (16, 16, 16, 32), (24, 24, 24, 48)]
 
 
-def serve_bench(lym, rates, workers=4):
-    """rates: (arrivals per second, requests) pairs, run one after another."""
+def serve_bench(lym, rates, workers=4, max_batch=4):
+    """rates: (arrivals per second, requests) pairs, run one after another;
+    an infinite rate submits every request at once."""
     rng = np.random.default_rng(6001)
-    srv = lym.Server(workers=workers, max_batch=4)
+    srv = lym.Server(workers=workers, max_batch=max_batch)
     ids, srcs = [], []
     for i, d in enumerate(SERVE_LATTICES):
         ids.append(srv.add_lattice(d, lym.gen_su3_links(d, EPS, 4000 + i), csw=CSW))
@@ -387,7 +389,7 @@ def serve_bench(lym, rates, workers=4):
         srv.wait(r)
     out = []
     for lam, nreq in rates:
-        arr = np.cumsum(rng.exponential(1.0 / lam, nreq))
+        arr = np.zeros(nreq) if np.isinf(lam) else np.cumsum(rng.exponential(1.0 / lam, nreq))
         spec = [(int(rng.integers(len(ids))), float(rng.uniform(0.120, 0.134)),
                  float(rng.choice([1e-8, 1e-10, 1e-12])), int(rng.integers(3))) for _ in range(nreq)]
         need = {li: sum(1 for q in spec if q[0] == li) for li in range(len(ids))}
@@ -407,22 +409,27 @@ def serve_bench(lym, rates, workers=4):
             dst = outs[li][used[li]]
             used[li] += 1
             rids.append(srv.submit(ids[li], k, tol, srcs[li][si], dst))
-        lat, conv = [], 0
+        lat, conv, bsz = [], 0, []
         for r in rids:
             info, _ = srv.wait(r)
             lat.append(info.latency_seconds)
             conv += int(info.solve.converged)
+            bsz.append(int(info.batch_size))
         span = time.perf_counter() - t0
         lat = np.array(lat)
-        row = {"lambda_per_s": lam, "requests": nreq, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+        row = {"lambda_per_s": lam if np.isfinite(lam) else "burst", "requests": nreq, "p50_ms": float(np.percentile(lat, 50) * 1e3),
                "p99_ms": float(np.percentile(lat, 99) * 1e3), "mean_ms": float(lat.mean() * 1e3),
-               "solves_per_s": nreq / span, "converged": conv}
-        log("bench: serving lambda %g/s: p50 %.1f ms, p99 %.1f ms, %.2f solves/s" % (
-            lam, row["p50_ms"], row["p99_ms"], row["solves_per_s"]))
+               "solves_per_s": nreq / span, "converged": conv, "mean_batch_size": float(np.mean(bsz)),
+               "max_batch_size": int(np.max(bsz))}
+        log("bench: serving lambda %s/s: p50 %.1f ms, p99 %.1f ms, %.2f solves/s, mean batch %.2f" % (
+            lam, row["p50_ms"], row["p99_ms"], row["solves_per_s"], row["mean_batch_size"]))
         out.append(row)
     srv.close()
     return {"config": "SU(3) clover csw 1.0 eps 0.5 on %s, kappa ~ U[0.120,0.134], tol in {1e-8,1e-10,1e-12}, "
-                      "Poisson arrivals, %d workers" % (SERVE_LATTICES, workers), "rates": out}
+                      "Poisson arrivals, %d workers, up to %d requests per batch (continuous batching: "
+                      "requests join running batches between iteration chunks); lambda 'inf' is a burst of "
+                      "every request at once (saturation throughput)" % (SERVE_LATTICES, workers, max_batch),
+            "rates": out}
 
 
 def cpu_baseline(budget_s=10.0):

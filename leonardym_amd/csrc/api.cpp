@@ -531,6 +531,7 @@ int lym_ctx_create(const int dims[4], int rep, int bc_t, lym_ctx** out) {
     (void)hipGetDevice(&c->device);
     int rc = LYM_OK;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = fail(LYM_ERR_DEVICE, "stream");
+    c->owns_stream = 1;
     if (!rc) rc = dmalloc(c, (void**)&c->red_partials, sizeof(double) * 8 * lym::RED_BLOCKS);
     if (!rc) rc = dmalloc(c, (void**)&c->red_result, 256);
     if (!rc && hipHostMalloc((void**)&c->host_pinned, 4096, hipHostMallocDefault) != hipSuccess)
@@ -554,7 +555,7 @@ void lym_ctx_destroy(lym_ctx* c) {
     if (c->host_pinned) (void)hipHostFree(c->host_pinned);
     if (c->ev_start) (void)hipEventDestroy(c->ev_start);
     if (c->ev_stop) (void)hipEventDestroy(c->ev_stop);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream && c->owns_stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 

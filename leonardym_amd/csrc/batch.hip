@@ -136,14 +136,15 @@ __device__ __forceinline__ void bg_store(E* d, int stride, long j, int r, const 
 // ---------------------------------------------------------------- kernels
 // Sums: one thread per (site, RHS) pair, RHS fastest (coalesced like the
 // fields), chunk_store / final_sums.  Element-wise kernels: PAIR_LOOP.
+// mask (may be null: every lane): only the lanes with mask[q] != 0 start
 template <int PR, int NR>
 __global__ __launch_bounds__(BC * NR) void k_bcg_init(const zd* r, cplx<float>* rs, cplx<float>* xs,
                                                       typename Prec<PR>::S* p, float* p_sc, long nsites, int stride,
-                                                      double* partials, long nchunks) {
+                                                      double* partials, long nchunks, const int* mask) {
     long j;
     int q;
     double v = 0;
-    if (chunk_pair<NR>(nsites, j, q)) {
+    if (chunk_pair<NR>(nsites, j, q) && (!mask || mask[q])) {
         zd a[12];
         cplx<float> f[12], z[12];
         bg_load<zd, NR>(r, stride, j, q, a);
@@ -162,12 +163,16 @@ __global__ __launch_bounds__(BC * NR) void k_bcg_init(const zd* r, cplx<float>* 
 
 template <int NR>
 __global__ __launch_bounds__(FT) void k_bcg_init_final(BState* s, const double* partials, long nchunks,
-                                                       int max_steps) {
+                                                       int max_steps, const int* mask) {
     double out[NR];
     final_sums<NR>(partials, nchunks, out);
     if (threadIdx.x == 0) {
         int all = 1;
         for (int q = 0; q < NR; ++q) {
+            if (mask && !mask[q]) {  // a lane that keeps its state
+                all = all && s->done[q];
+                continue;
+            }
             s->rr_s[q] = s->rr_true[q] = s->maxrr[q] = out[q];
             s->step[q] = 0;
             s->nrel[q] = 0;
@@ -180,8 +185,11 @@ __global__ __launch_bounds__(FT) void k_bcg_init_final(BState* s, const double* 
         }
         s->max_steps = max_steps;
         s->all_done = all;
-        s->any_rel = 0;
-        s->skip_rel = 1;
+        // a lane that keeps its state may hold an update due at the next slot
+        int any = 0;
+        for (int q = 0; q < NR; ++q) any = any || (s->rel[q] && !s->done[q]);
+        s->any_rel = any;
+        s->skip_rel = any ? 0 : 1;
     }
 }
 
@@ -360,11 +368,11 @@ __global__ void k_bcg_p(const BState* s, typename Prec<PR>::S* p, float* p_sc, c
     }
 }
 
-// x += x_s for the RHS that stopped between reliable updates
+// x += x_s for the RHS that stopped between reliable updates (mask may be null: every lane)
 template <int NR>
-__global__ void k_baccum(const BState* s, zd* x, const cplx<float>* xs, long nsites, int stride) {
+__global__ void k_baccum(const BState* s, zd* x, const cplx<float>* xs, long nsites, int stride, const int* mask) {
     PAIR_LOOP(j, q, NR, nsites) {
-        if (s->conv[q]) continue;
+        if (s->conv[q] || (mask && !mask[q])) continue;
         zd a[12];
         cplx<float> f[12];
         bg_load<zd, NR>(x, stride, j, (int)q, a);
@@ -375,15 +383,46 @@ __global__ void k_baccum(const BState* s, zd* x, const cplx<float>* xs, long nsi
     }
 }
 
-// pack / unpack single full fields <-> an nr-field; also multi-RHS clover block apply
+// pack / unpack single full fields <-> an nr-field (a null pointer leaves
+// its lane alone); also multi-RHS clover block apply
 __global__ void k_pack(const zd* const* srcs, int nr, zd* multi, long n) {
     BSITE_LOOP(i, n) {
-        for (int r = 0; r < nr; ++r) multi[i * nr + r] = srcs[r][i];
+        for (int r = 0; r < nr; ++r)
+            if (srcs[r]) multi[i * nr + r] = srcs[r][i];
     }
 }
 __global__ void k_unpack(const zd* multi, int nr, zd* const* dsts, long n) {
     BSITE_LOOP(i, n) {
-        for (int r = 0; r < nr; ++r) dsts[r][i] = multi[i * nr + r];
+        for (int r = 0; r < nr; ++r)
+            if (dsts[r]) dsts[r][i] = multi[i * nr + r];
+    }
+}
+
+// ---------------------------------------------------------------- lanes of an nr-field
+// An nr-field is rows x nr elements, lane fastest.  These act on the lanes
+// with mask[q] != 0 only, or move every lane to another width.
+template <class E>
+__global__ void k_lane_copy(E* dst, const E* src, long rows, int nr, const int* mask) {
+    BSITE_LOOP(t, rows * nr) {
+        if (mask[t % nr]) dst[t] = src[t];
+    }
+}
+__global__ void k_lane_sub(zd* out, const zd* a, const zd* b, long rows, int nr, const int* mask) {
+    BSITE_LOOP(t, rows * nr) {
+        if (mask[t % nr]) out[t] = a[t] - b[t];
+    }
+}
+// out (width w_new) from in (width w_old): lane j keeps its index, lanes
+// j >= w_old are zero
+template <class E>
+__global__ void k_relayout(const E* in, E* out, long rows, int w_old, int w_new) {
+    BSITE_LOOP(t, rows * w_new) {
+        const long i = t / w_new;
+        const int j = (int)(t - i * w_new);
+        E e;
+        if (j < w_old) e = in[i * w_old + j];
+        else __builtin_memset(&e, 0, sizeof(E));
+        out[t] = e;
     }
 }
 
@@ -397,6 +436,22 @@ __global__ void k_clover_apply_nr(Geom g, const zd* __restrict__ blocks, const z
     for (int k = 0; k < 12; ++k) s[k] = in[((long)k * g.stride + idx) * nr + r];
     herm6(blocks, g.stride, idx, s, o);
     herm6(blocks + (size_t)18 * g.stride, g.stride, idx, s + 6, o + 6);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) out[((long)k * g.stride + idx) * nr + r] = o[k];
+}
+
+__global__ void k_clover_apply_lanes(Geom g, const zd* __restrict__ blocks, long lane_elems, const int* mask,
+                                     const zd* __restrict__ in, zd* __restrict__ out, int nr) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)g.Vh * nr) return;
+    const int idx = (int)(t / nr), r = (int)(t % nr);
+    if (!mask[r]) return;
+    const zd* B = blocks + r * lane_elems;
+    zd s[12], o[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) s[k] = in[((long)k * g.stride + idx) * nr + r];
+    herm6(B, g.stride, idx, s, o);
+    herm6(B + (size_t)18 * g.stride, g.stride, idx, s + 6, o + 6);
 #pragma unroll
     for (int k = 0; k < 12; ++k) out[((long)k * g.stride + idx) * nr + r] = o[k];
 }
@@ -423,8 +478,8 @@ static void bcg_step_impl(int which, BState* s, const BcgArgs& a, hipStream_t st
     switch (which) {
         case 0:
             hipLaunchKernelGGL((k_bcg_init<PR, NR>), cg, cb, 0, st, a.r, a.rs, a.xs, p, a.p_sc, a.nsites, a.stride,
-                               a.partials, nch);
-            hipLaunchKernelGGL((k_bcg_init_final<NR>), dim3(1), fb, 0, st, s, a.partials, nch, a.max_steps);
+                               a.partials, nch, a.mask);
+            hipLaunchKernelGGL((k_bcg_init_final<NR>), dim3(1), fb, 0, st, s, a.partials, nch, a.max_steps, a.mask);
             break;
         case 1:
             hipLaunchKernelGGL((k_bcg_dot<PR, NR>), cg, cb, 0, st, s, p, a.p_sc, Ap, a.Ap_sc, a.nsites, a.stride,
@@ -448,7 +503,7 @@ static void bcg_step_impl(int which, BState* s, const BcgArgs& a, hipStream_t st
             hipLaunchKernelGGL((k_bcg_p<PR, NR>), dim3(eb), dim3(256), 0, st, s, p, a.p_sc, a.rs, a.nsites, a.stride);
             break;
         default:
-            hipLaunchKernelGGL((k_baccum<NR>), dim3(eb), dim3(256), 0, st, s, a.x, a.xs, a.nsites, a.stride);
+            hipLaunchKernelGGL((k_baccum<NR>), dim3(eb), dim3(256), 0, st, s, a.x, a.xs, a.nsites, a.stride, a.mask);
             break;
     }
 }
@@ -479,6 +534,44 @@ void launch_pack(const zd* const* srcs_dev, int nr, zd* multi, long n, hipStream
 }
 void launch_unpack(const zd* multi, int nr, zd* const* dsts_dev, long n, hipStream_t st) {
     hipLaunchKernelGGL(k_unpack, dim3(bsite_blocks(n)), dim3(256), 0, st, multi, nr, dsts_dev, n);
+}
+
+void launch_lane_copy(int esize, void* dst, const void* src, long rows, int nr, const int* mask, hipStream_t st) {
+    const dim3 g(bsite_blocks(rows * nr)), b(256);
+    switch (esize) {
+        case 16: hipLaunchKernelGGL(k_lane_copy<zd>, g, b, 0, st, (zd*)dst, (const zd*)src, rows, nr, mask); break;
+        case 8:
+            hipLaunchKernelGGL(k_lane_copy<cplx<float>>, g, b, 0, st, (cplx<float>*)dst, (const cplx<float>*)src,
+                               rows, nr, mask);
+            break;
+        default:
+            hipLaunchKernelGGL(k_lane_copy<float>, g, b, 0, st, (float*)dst, (const float*)src, rows, nr, mask);
+            break;
+    }
+}
+void launch_lane_sub(zd* out, const zd* a, const zd* b, long rows, int nr, const int* mask, hipStream_t st) {
+    hipLaunchKernelGGL(k_lane_sub, dim3(bsite_blocks(rows * nr)), dim3(256), 0, st, out, a, b, rows, nr, mask);
+}
+void launch_relayout(int esize, const void* in, void* out, long rows, int w_old, int w_new, hipStream_t st) {
+    const dim3 g(bsite_blocks(rows * w_new)), b(256);
+    switch (esize) {
+        case 16:
+            hipLaunchKernelGGL(k_relayout<V16>, g, b, 0, st, (const V16*)in, (V16*)out, rows, w_old, w_new);
+            break;
+        case 8:
+            hipLaunchKernelGGL(k_relayout<cplx<float>>, g, b, 0, st, (const cplx<float>*)in, (cplx<float>*)out, rows,
+                               w_old, w_new);
+            break;
+        default:
+            hipLaunchKernelGGL(k_relayout<float>, g, b, 0, st, (const float*)in, (float*)out, rows, w_old, w_new);
+            break;
+    }
+}
+void launch_clover_apply_lanes(const Geom& g, const zd* blocks, long lane_elems, const int* mask, const zd* in,
+                               zd* out, int nr, hipStream_t st) {
+    long n = (long)g.Vh * nr;
+    hipLaunchKernelGGL(k_clover_apply_lanes, dim3((n + 255) / 256), dim3(256), 0, st, g, blocks, lane_elems, mask, in,
+                       out, nr);
 }
 void launch_clover_apply_nr(const Geom& g, const zd* blocks, const zd* in, zd* out, int nr, hipStream_t st) {
     long n = (long)g.Vh * nr;

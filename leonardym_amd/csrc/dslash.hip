@@ -217,8 +217,9 @@ __device__ __forceinline__ void hop_pair(const DslashParams& a, int p, int site,
 
 // NORM (low precision only): returns |v|^2 of the site's output as computed,
 // before rounding to the storage precision; else 0.  A template parameter, so
-// kernels without it compile as if the sum did not exist.
-template <int PR, int LK, int DIAG, bool INV, bool NORM = false>
+// kernels without it compile as if the sum did not exist.  LO: lane r has its
+// own kappa and clover blocks (a.lane_ops); also a template parameter.
+template <int PR, int LK, int DIAG, bool INV, bool NORM = false, bool LO = false>
 __device__ __forceinline__ double dslash_site(const DslashParams& a, int p, int site, int r) {
     typedef typename Prec<PR>::T T;
     typedef typename Prec<PR>::S S;
@@ -239,8 +240,21 @@ __device__ __forceinline__ double dslash_site(const DslashParams& a, int p, int 
         hop_pair<2, PR, LK>(a, p, site, r, c, acc);
         hop_pair<3, PR, LK>(a, p, site, r, c, acc);
     }
-    const double kap = a.kappa_dev ? *a.kappa_dev : a.kappa;
+    const double kap = LO ? a.kappa_dev[r] : (a.kappa_dev ? *a.kappa_dev : a.kappa);
     const T hk = (T)(-kap * a.hop_sign);
+    // this lane's clover blocks
+    const void* Ab_ = a.A;
+    const void* Ai_ = a.Ainv;
+    const float* Asc_ = a.A_sc;
+    const float* Aisc_ = a.Ainv_sc;
+    if constexpr (LO) {
+        Ab_ = reinterpret_cast<const char*>(a.A) + r * a.A_lane_bytes;
+        Ai_ = reinterpret_cast<const char*>(a.Ainv) + r * a.Ainv_lane_bytes;
+        if constexpr (PR == 2) {
+            Asc_ = a.A_sc + r * a.Asc_lane;
+            Aisc_ = a.Ainv_sc + r * a.Ainvsc_lane;
+        }
+    }
     cplx<T> v[12];
     if constexpr (DIAG == 0) {
 #pragma unroll
@@ -251,9 +265,9 @@ __device__ __forceinline__ double dslash_site(const DslashParams& a, int p, int 
         T scp;
         load_spinor_unscaled<PR>(SpinorRef<PR>{reinterpret_cast<const S*>(a.diag_in[p]), a.diag_sc[p]}, ST, a.nr,
                                  site, r, s, scp);
-        const T sc = PR == 2 ? (T)a.A_sc[(size_t)p * ST + site] * scp : (T)1;
-        herm6_packed<PR>(a.A, 4 * (size_t)ST, (size_t)(2 * p) * ST, site, sc, s, t);
-        herm6_packed<PR>(a.A, 4 * (size_t)ST, (size_t)(2 * p + 1) * ST, site, sc, s + 6, t + 6);
+        const T sc = PR == 2 ? (T)Asc_[(size_t)p * ST + site] * scp : (T)1;
+        herm6_packed<PR>(Ab_, 4 * (size_t)ST, (size_t)(2 * p) * ST, site, sc, s, t);
+        herm6_packed<PR>(Ab_, 4 * (size_t)ST, (size_t)(2 * p + 1) * ST, site, sc, s + 6, t + 6);
 #pragma unroll
         for (int k = 0; k < 12; ++k) v[k] = t[k] + hk * acc[k];
     } else {
@@ -261,7 +275,7 @@ __device__ __forceinline__ double dslash_site(const DslashParams& a, int p, int 
         load_spinor<PR>(SpinorRef<PR>{reinterpret_cast<const S*>(a.diag_in[p]), a.diag_sc[p]}, ST, a.nr, site, r, s);
         if constexpr (DIAG == 2) {
             cplx<T> t[12];
-            const S* Ab = reinterpret_cast<const S*>(a.A) + (size_t)p * 36 * ST;
+            const S* Ab = reinterpret_cast<const S*>(Ab_) + (size_t)p * 36 * ST;
             herm6<PR>(Ab, ST, site, (T)1, s, t);
             herm6<PR>(Ab + (size_t)18 * ST, ST, site, (T)1, s + 6, t + 6);
 #pragma unroll
@@ -274,13 +288,13 @@ __device__ __forceinline__ double dslash_site(const DslashParams& a, int p, int 
     if constexpr (INV) {
         cplx<T> t[12];
         if constexpr (PR == 0) {
-            const S* Ai = reinterpret_cast<const S*>(a.Ainv);
+            const S* Ai = reinterpret_cast<const S*>(Ai_);
             herm6<PR>(Ai, ST, site, (T)1, v, t);
             herm6<PR>(Ai + (size_t)18 * ST, ST, site, (T)1, v + 6, t + 6);
         } else {
-            const T sc = PR == 2 ? (T)a.Ainv_sc[site] : (T)1;
-            herm6_packed<PR>(a.Ainv, 2 * (size_t)ST, 0, site, sc, v, t);
-            herm6_packed<PR>(a.Ainv, 2 * (size_t)ST, (size_t)ST, site, sc, v + 6, t + 6);
+            const T sc = PR == 2 ? (T)Aisc_[site] : (T)1;
+            herm6_packed<PR>(Ai_, 2 * (size_t)ST, 0, site, sc, v, t);
+            herm6_packed<PR>(Ai_, 2 * (size_t)ST, (size_t)ST, site, sc, v + 6, t + 6);
         }
 #pragma unroll
         for (int k = 0; k < 12; ++k) v[k] = t[k];
@@ -408,8 +422,8 @@ __device__ __forceinline__ void block_of(const DslashParams& a, const BlockSweep
     (void)block_of_at(a, b, lb, (int)blockIdx.x, p, bi);
 }
 
-// NORM: also the per-workgroup output norms (a.norm_partials)
-template <int PR, int LK, int DIAG, bool INV, bool NORM = false>
+// NORM: also the per-workgroup output norms (a.norm_partials); LO: per-lane operators
+template <int PR, int LK, int DIAG, bool INV, bool NORM = false, bool LO = false>
 __global__ __launch_bounds__(LYM_DSLASH_THREADS) LYM_OCC void k_dslash(DslashParams a) {
     static_assert(!NORM || PR >= 1, "output norms are the low-precision stencil's");
     if (a.skip && *a.skip) return;
@@ -422,7 +436,7 @@ __global__ __launch_bounds__(LYM_DSLASH_THREADS) LYM_OCC void k_dslash(DslashPar
             int p, bi;
             block_of(a, b, lb, p, bi);
             const int site = bi * a.spb + ls;
-            if (site < a.g.Vh) dslash_site<PR, LK, DIAG, INV>(a, p, site, r);
+            if (site < a.g.Vh) dslash_site<PR, LK, DIAG, INV, false, LO>(a, p, site, r);
         }
     } else {
         // one RHS, no lane flags (the launcher's condition for output norms):
@@ -593,6 +607,10 @@ __global__ __launch_bounds__(2 * LYM_DSLASH_THREADS) LYM_OCC void k_dslash_split
 
 template <int PR, int LK, int DIAG, bool INV>
 static void launch_one(const DslashParams& a, int nblocks, hipStream_t st) {
+    if (a.lane_ops) {  // per-lane operators: the serving engine's batches (never split, no output norms)
+        hipLaunchKernelGGL((k_dslash<PR, LK, DIAG, INV, false, true>), dim3(nblocks), dim3(a.spb * a.nr), 0, st, a);
+        return;
+    }
     // output norms: asked for only on launches with a diagonal and no R^-1
     // (the last launch of a Schur apply, a full-lattice apply)
     if constexpr (PR >= 1 && DIAG >= 1 && !INV) {
@@ -645,7 +663,7 @@ static int plan_dslash(DslashParams& a) {
     a.spb = threads / a.nr;
     if (a.spb < 1) a.spb = 1;
     // two threads per site: low-precision single-RHS applies that allow it
-    a.split = (LYM_SPLIT_LOWP && a.allow_split && a.prec >= 1 && threads % 64 == 0 &&
+    a.split = (LYM_SPLIT_LOWP && a.allow_split && !a.lane_ops && a.prec >= 1 && threads % 64 == 0 &&
                (long)a.g.Vh * a.nr <= LYM_SPLIT_MAX_SITES)
                   ? threads
                   : 0;

@@ -149,6 +149,7 @@ struct lym_ctx {
     int rep;  // 0 SU(3) fundamental, 1 SU(2) adjoint
     int device;
     hipStream_t stream;
+    int owns_stream;        // 0: the stream belongs to the caller (serving workers share one per thread)
     void* staging;          // device staging for host-layout transfers
     size_t staging_bytes;
     double* red_partials;   // reduction partials

@@ -45,6 +45,11 @@ struct DslashParams {
     int split;             // set by the launcher: threads per half of that form, or 0
     double* norm_partials; // optional (FP32/FP16 only): workgroup b writes the sum of |out|^2 over
     int norm_max;          //   its sites to norm_partials[b], when the launch has at most norm_max workgroups
+    // Per-lane operators (multi-RHS launches of the serving engine): lane r uses
+    // kappa_dev[r] and the clover blocks at A + r * A_lane_bytes (scales at
+    // A_sc + r * Asc_lane), Ainv + r * Ainv_lane_bytes (Ainv_sc + r * Ainvsc_lane)
+    int lane_ops;
+    long A_lane_bytes, Ainv_lane_bytes, Asc_lane, Ainvsc_lane;
 };
 
 // link_kind: 0 complex 18, 1 complex 12, 2 real 9, 3 real 6; diag: 0 none, 1 identity, 2 clover.
@@ -189,13 +194,23 @@ struct BcgArgs {
     int stride;
     int max_steps;
     double* partials;  // BMAX * ceil(nsites / BATCH_CHUNK) doubles
+    const int* mask = nullptr;  // steps 0 (init) and 6 (final accumulate): only lanes with mask[q] != 0
 };
 // which: 0 init, 1 alpha, 2 update, 3 reliable accumulate, 4 reliable residual, 5 p update, 6 final accumulate
 bool launch_bcg_step(int which, int prec, int nr, BState* s, const BcgArgs& a, hipStream_t st);
 bool batch_nr_supported(int nr);
 void launch_pack(const zd* const* srcs_dev, int nr, zd* multi, long n, hipStream_t st);
 void launch_unpack(const zd* multi, int nr, zd* const* dsts_dev, long n, hipStream_t st);
+// lanes of nr-fields (rows x nr elements of esize bytes, lane fastest): copy or
+// subtract on the lanes with mask[q] != 0; relayout to another width (a lane
+// keeps its index; new lanes are zero)
+void launch_lane_copy(int esize, void* dst, const void* src, long rows, int nr, const int* mask, hipStream_t st);
+void launch_lane_sub(zd* out, const zd* a, const zd* b, long rows, int nr, const int* mask, hipStream_t st);
+void launch_relayout(int esize, const void* in, void* out, long rows, int w_old, int w_new, hipStream_t st);
 void launch_clover_apply_nr(const Geom& g, const zd* blocks, const zd* in, zd* out, int nr, hipStream_t st);
+// the same with lane r's blocks at blocks + r * lane_elems, only for lanes with mask[r] != 0
+void launch_clover_apply_lanes(const Geom& g, const zd* blocks, long lane_elems, const int* mask, const zd* in,
+                               zd* out, int nr, hipStream_t st);
 
 // x = uniform [0,1) complex on real sites, zero on padding (BiCGStab start for a zero source)
 void launch_fill_uniform(zd* x, int nparity, int stride, int Vh, uint64_t seed, hipStream_t st);

@@ -1,22 +1,26 @@
 // server.cpp -- solve-request serving loop over device-resident gauge fields.
 //
-// Callers register lattices (links uploaded once, kept in HBM) and submit
-// requests (lattice, kappa, tolerance, source, destination).  Worker threads,
-// each with its own HIP stream (default priority) per lattice, take requests
-// in arrival order; a worker batches the queued requests that share lattice
-// and kappa (same operator) into one multi-RHS solve.  Device state per
-// (worker, lattice) -- context, operator with its clover blocks, solver work
-// vectors and batch buffers, spinors -- is created on first use and charged
-// against an HBM budget; a worker that would exceed it first releases the
-// least recently used idle state (of any worker), and otherwise waits until
-// another worker's batch ends.
+// Callers register lattices (links uploaded once and kept in HBM, clover
+// field-strength blocks built once) and submit requests (lattice, kappa,
+// tolerance, source, destination).  Each request solves D x = b with the
+// even-odd mixed-precision CG driver (the reference's PreconditionedBiCGStab
+// entry point, source/inverters/PreconditionedBiCGStab.cpp:43-91) to
+// |b - D x| <= tol |b|, through the stopping rule |r|^2 < (0.1 tol |b|)^2 on
+// the even-odd system.
 //
-// Each request is solved by the even-odd mixed-precision CG driver (D x = b,
-// the reference's PreconditionedBiCGStab entry point,
-// source/inverters/PreconditionedBiCGStab.cpp:43-91) to |b - D x| <= tol |b|
-// through the stopping rule |r|^2 < (0.1 tol |b|)^2 on the even-odd system.
-// Results do not depend on what else is queued: a request gives the same
-// bits alone and in any batch (see batch.hip).
+// Continuous batching: a worker thread runs a session on one lattice, a
+// batched solve whose lanes are requests (engine.h).  Between chunks of
+// iterations the session returns the requests whose lanes have stopped and
+// admits queued requests for its lattice -- any kappa and tolerance -- into
+// the free lanes, widening the batch up to max_batch lanes.  A queued request
+// joins a running session on its lattice that has room; otherwise an idle
+// worker starts a new session for it.  Every worker has one HIP stream
+// (default priority) for all its lattices.  Device state per (worker,
+// lattice) -- context, engine with its lanes' operator data and fields -- is
+// created on first use and charged against an HBM budget; a worker that would
+// exceed it first releases the least recently used idle state (of any
+// worker), and otherwise waits until another worker's session ends.  A
+// request gives the same bits alone and in any mix (engine.h).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,11 +38,13 @@
 #include <vector>
 
 #include "../../include/lym.h"
+#include "engine.h"
 #include "lym_common.h"
 
 namespace lym_internal {
 int fail(int code, const char* fmt, ...);
-}
+int op_ready(lym_op* op, bool force_blocks);
+}  // namespace lym_internal
 using lym_internal::fail;
 
 namespace {
@@ -71,24 +77,16 @@ struct Lattice {
 // device state of one worker for one lattice
 struct WState {
     lym_ctx* ctx = nullptr;
-    lym_op* op = nullptr;
-    lym_solver* solver = nullptr;
-    std::vector<lym_spinor*> src, sol;
+    lym_internal::LaneEngine* eng = nullptr;
     size_t charge = 0;
     uint64_t last_use = 0;
     bool in_use = false;
 };
 
 void free_state(WState& s) {
-    for (auto* p : s.src) lym_spinor_destroy(p);
-    for (auto* p : s.sol) lym_spinor_destroy(p);
-    s.src.clear();
-    s.sol.clear();
-    if (s.solver) lym_solver_destroy(s.solver);
-    if (s.op) lym_op_destroy(s.op);
+    if (s.eng) lym_internal::engine_destroy(s.eng);
     if (s.ctx) lym_ctx_destroy(s.ctx);
-    s.solver = nullptr;
-    s.op = nullptr;
+    s.eng = nullptr;
     s.ctx = nullptr;
 }
 
@@ -104,6 +102,9 @@ struct lym_server {
     size_t budget = 0, used = 0;
     std::vector<std::unique_ptr<Lattice>> lattices;
     std::vector<std::map<int, WState>> states;  // per worker
+    std::vector<hipStream_t> streams;           // per worker
+    std::vector<int> sess_lattice;              // per worker: lattice of its running session, or -1
+    std::vector<int> sess_room;                 // per worker: requests its session can still take
     std::mutex m;
     std::condition_variable cv_work, cv_done, cv_mem;
     std::deque<Request*> queue;
@@ -116,66 +117,41 @@ struct lym_server {
 
 namespace {
 
-// device bytes of one worker's state for a lattice, for batches up to nr RHS:
-// operator (blocks, inverse, scratch, low-precision copies: links as rows 0, 1
-// and as all rows for batches, blocks), solver (8 FP64 work vectors, mixed and
-// batch buffers), nr source and solution spinors
-size_t state_bytes(const Lattice& L, unsigned nr) {
-    const double V = (double)L.dims[0] * L.dims[1] * L.dims[2] * L.dims[3];
-    const double full = V * 12 * 16;  // FP64 full spinor
-    const double op = V * (36 * 16 + 18 * 16) + 2 * full + 2 * V * 4 * 9 * 8 + V * 54 * 4;
-    const double solver = 8 * full + 6 * full / 2;
-    const double batch = nr * (2 * full + 8 * full / 2 + 4 * V * 12 * 4 + 2 * V * 6 * 8);
-    const double spinors = 2.0 * nr * full + full;  // + staging
-    return (size_t)((op + solver + batch + spinors) * 1.05) + (64u << 20);
-}
-
 int make_state(lym_server* S, int w, int li, WState& st) {
     const Lattice& L = *S->lattices[li];
-    int rc = lym_ctx_create(L.dims, L.rep, L.bc, &st.ctx);
-    if (!rc) rc = lym_op_create(st.ctx, LYM_OP_CLOVER, 1, 0.125, L.csw, 1, &st.op);
-    if (!rc) rc = lym_op_set_gauge(st.op, L.gauge);
-    if (!rc) rc = lym_solver_create(st.ctx, LYM_SOLVER_PRECOND_MIXED_CG, &st.solver);
-    if (!rc) rc = lym_solver_set_sloppy_precision(st.solver, S->sloppy);
-    if (!rc) rc = lym_solver_set_max_steps(st.solver, S->max_steps);
-    if (!rc) rc = lym_solver_set_rhs_block(st.solver, S->max_batch);
-    for (unsigned q = 0; !rc && q < S->max_batch; ++q) {
-        lym_spinor *a = nullptr, *b = nullptr;
-        rc = lym_spinor_create(st.ctx, &a);
-        if (!rc) st.src.push_back(a);
-        if (!rc) rc = lym_spinor_create(st.ctx, &b);
-        if (!rc) st.sol.push_back(b);
-    }
-    (void)w;
+    int rc = lym_internal::ctx_create_on_stream(L.dims, L.rep, L.bc, S->streams[w], &st.ctx);
+    if (!rc) rc = lym_internal::engine_create(st.ctx, L.gauge, L.csw, S->sloppy, S->max_steps, (int)S->max_batch,
+                                              &st.eng);
     if (rc) free_state(st);
     return rc;
 }
 
-// the next batch: the oldest request and the queued ones sharing its operator
-std::vector<Request*> take_batch(lym_server* S) {
+// the oldest queued request that no running session on its lattice will take
+// (sessions absorb queued requests of their lattice up to their room);
+// called with S->m held
+std::deque<Request*>::iterator startable(lym_server* S) {
+    std::map<int, int> room, ahead;
+    for (size_t w = 0; w < S->sess_lattice.size(); ++w)
+        if (S->sess_lattice[w] >= 0) room[S->sess_lattice[w]] += S->sess_room[w];
+    for (auto it = S->queue.begin(); it != S->queue.end(); ++it) {
+        const int li = (*it)->lattice;
+        if (ahead[li]++ >= room[li]) return it;
+    }
+    return S->queue.end();
+}
+
+// up to n queued requests of lattice li, oldest first; called with S->m held
+std::vector<Request*> take(lym_server* S, int li, int n) {
     std::vector<Request*> b;
-    Request* r0 = S->queue.front();
-    for (auto it = S->queue.begin(); it != S->queue.end() && b.size() < S->max_batch;) {
-        Request* r = *it;
-        if (r->lattice == r0->lattice && r->kappa == r0->kappa) {
-            b.push_back(r);
+    for (auto it = S->queue.begin(); it != S->queue.end() && (int)b.size() < n;) {
+        if ((*it)->lattice == li) {
+            b.push_back(*it);
             it = S->queue.erase(it);
         } else {
             ++it;
         }
     }
     return b;
-}
-
-void finish(lym_server* S, std::vector<Request*>& batch, int rc, const char* msg) {
-    auto now = clk::now();
-    std::lock_guard<std::mutex> lk(S->m);
-    for (Request* r : batch) {
-        r->t_done = now;
-        r->status = rc ? LYM_REQ_FAILED : LYM_REQ_DONE;
-        r->error = rc;
-        if (rc) r->msg = msg ? msg : "";
-    }
 }
 
 // device state for (w, li), created within the budget; called with S->m held.
@@ -189,7 +165,8 @@ WState* admit(lym_server* S, int w, int li, std::unique_lock<std::mutex>& lk, in
         it->second.in_use = true;
         return &it->second;
     }
-    const size_t need = state_bytes(*S->lattices[li], S->max_batch);
+    const Lattice& L = *S->lattices[li];
+    const size_t need = lym_internal::engine_bytes(L.dims, L.rep, L.storage, S->sloppy, (int)S->max_batch);
     if (need > S->budget) {  // never fits, whatever is released: fail now rather than wait
         *rc = fail(LYM_ERR_NOMEM, "lattice state of %zu bytes exceeds the HBM budget of %zu", need, S->budget);
         return nullptr;
@@ -237,54 +214,113 @@ WState* admit(lym_server* S, int w, int li, std::unique_lock<std::mutex>& lk, in
     return &mine[li];
 }
 
-void worker(lym_server* S, int w) {
-    (void)hipSetDevice(S->device);
-    std::unique_lock<std::mutex> lk(S->m);
+void finish_one(Request* r, int rc, const std::string& msg, const lym_solve_info* info, unsigned batch) {
+    r->t_done = clk::now();
+    r->status = rc ? LYM_REQ_FAILED : LYM_REQ_DONE;
+    r->error = rc;
+    if (rc) r->msg = msg;
+    if (info) r->info = *info;
+    r->info.seconds = seconds(r->t_start, r->t_done);
+    if (batch) r->batch = batch;
+}
+
+// One session: the batched solve on lattice li, fed with queued requests of
+// li between chunks until no lane is busy and none is queued.
+// Returns the device error that ended it, if any (its state is then unusable).
+int session(lym_server* S, int w, int li, WState* st, std::vector<Request*> batch, std::unique_lock<std::mutex>& lk) {
+    using lym_internal::LaneRequest;
+    using lym_internal::LaneResult;
+    lym_internal::LaneEngine* e = st->eng;
     for (;;) {
-        S->cv_work.wait(lk, [&] { return S->stop || !S->queue.empty(); });
-        if (S->stop && S->queue.empty()) break;
-        std::vector<Request*> batch = take_batch(S);
-        const int li = batch[0]->lattice;
-        const double kappa = batch[0]->kappa;
-        auto t0 = clk::now();
+        // admit (lock released while the device works)
+        const auto t0 = clk::now();
         for (Request* r : batch) {
             r->status = LYM_REQ_RUNNING;
             r->t_start = t0;
             r->worker = w;
-            r->batch = (unsigned)batch.size();
         }
+        lk.unlock();
+        std::vector<LaneRequest> lr;
+        for (Request* r : batch) lr.push_back(LaneRequest{r->src, r->out, r->kappa, r->tol, r});
+        std::vector<LaneResult> failed, done;
+        int rc = lym_internal::engine_admit(e, (int)lr.size(), lr.data(), &failed);
+        if (!rc) rc = lym_internal::engine_run_chunk(e);
+        if (!rc) rc = lym_internal::engine_harvest(e, &done);
+        std::string msg = rc ? lym_last_error() : "";
+        lk.lock();
+        for (auto& f : failed) finish_one((Request*)f.tag, f.error, "clover block not invertible at this kappa",
+                                          nullptr, 1);
+        for (auto& d : done) finish_one((Request*)d.tag, LYM_OK, "", &d.info, d.max_lanes);
+        if (rc) {
+            // a device error ends the session: every request it holds fails
+            for (Request* r : batch)
+                if (r->status == LYM_REQ_RUNNING) finish_one(r, rc, msg, nullptr, 0);
+            for (auto& kv : S->reqs)
+                if (kv.second->status == LYM_REQ_RUNNING && kv.second->worker == w)
+                    finish_one(kv.second.get(), rc, msg, nullptr, 0);
+            S->cv_done.notify_all();
+            S->sess_lattice[w] = -1;
+            S->sess_room[w] = 0;
+            S->cv_work.notify_all();
+            return rc;
+        }
+        if (!failed.empty() || !done.empty()) S->cv_done.notify_all();
+        const int room = lym_internal::engine_room(e);
+        batch = take(S, li, room);
+        if (batch.empty() && lym_internal::engine_busy(e) == 0) break;
+        S->sess_room[w] = room - (int)batch.size();
+    }
+    S->sess_lattice[w] = -1;
+    S->sess_room[w] = 0;
+    S->cv_work.notify_all();  // queued requests of li no longer have a session to join
+    return LYM_OK;
+}
+
+void worker(lym_server* S, int w) {
+    (void)hipSetDevice(S->device);
+    std::unique_lock<std::mutex> lk(S->m);
+    for (;;) {
+        std::deque<Request*>::iterator it;
+        S->cv_work.wait(lk, [&] { return (S->stop && S->queue.empty()) || (it = startable(S)) != S->queue.end(); });
+        if (S->stop && S->queue.empty()) break;
+        const int li = (*it)->lattice;
         ++S->busy;
         int rc = LYM_OK;
         WState* st = admit(S, w, li, lk, &rc);
-        lk.unlock();
-        std::string msg;
-        if (st) {
-            const unsigned n = (unsigned)batch.size();
-            std::vector<double> eps(n);
-            std::vector<lym_solve_info> infos(n);
-            rc = lym_op_set_kappa(st->op, kappa);
-            for (unsigned q = 0; !rc && q < n; ++q) {
-                double b2 = 0;
-                rc = lym_spinor_upload(st->src[q], batch[q]->src);
-                if (!rc) rc = lym_spinor_squared_norm(st->src[q], &b2);
-                const double t = 0.1 * batch[q]->tol;
-                eps[q] = t * t * b2;
+        if (!st) {
+            // the state cannot be created: the oldest request of li fails
+            std::vector<Request*> one = take(S, li, 1);
+            for (Request* r : one) {
+                r->t_start = clk::now();
+                r->worker = w;
+                finish_one(r, rc, lym_last_error(), nullptr, 0);
             }
-            if (!rc) rc = lym_solver_solve_batch(st->solver, st->op, n, st->src.data(), st->sol.data(), eps.data(),
-                                                 infos.data());
-            for (unsigned q = 0; !rc && q < n; ++q) rc = lym_spinor_download(st->sol[q], batch[q]->out);
-            if (!rc)
-                for (unsigned q = 0; q < n; ++q) batch[q]->info = infos[q];
-            if (rc) msg = lym_last_error();
-        } else {
-            msg = lym_last_error();
+            --S->busy;
+            S->cv_done.notify_all();
+            S->cv_mem.notify_all();
+            continue;
         }
-        finish(S, batch, rc, msg.c_str());
-        lk.lock();
-        if (st) st->in_use = false;
+        std::vector<Request*> batch = take(S, li, (int)S->max_batch);
+        S->sess_lattice[w] = li;
+        S->sess_room[w] = (int)S->max_batch - (int)batch.size();
+        const int src = batch.empty() ? LYM_OK : session(S, w, li, st, std::move(batch), lk);
+        S->sess_lattice[w] = -1;
+        S->sess_room[w] = 0;
+        if (src) {
+            // a device error left the engine's lanes undefined: release its state
+            WState dead = *st;
+            S->states[w].erase(li);
+            S->used -= dead.charge;
+            lk.unlock();
+            free_state(dead);
+            lk.lock();
+        } else {
+            st->in_use = false;
+        }
         --S->busy;
         S->cv_done.notify_all();
         S->cv_mem.notify_all();
+        S->cv_work.notify_all();
     }
 }
 
@@ -309,6 +345,17 @@ int lym_server_create(unsigned workers, size_t hbm_budget_bytes, unsigned max_ba
     // default budget: 80% of the free HBM
     S->budget = hbm_budget_bytes ? std::min(hbm_budget_bytes, freeb) : (size_t)(0.8 * (double)freeb);
     S->states.resize(workers);
+    S->sess_lattice.assign(workers, -1);
+    S->sess_room.assign(workers, 0);
+    for (unsigned w = 0; w < workers; ++w) {
+        hipStream_t st = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+            for (hipStream_t x : S->streams) (void)hipStreamDestroy(x);
+            delete S;
+            return fail(LYM_ERR_DEVICE, "hipStreamCreate");
+        }
+        S->streams.push_back(st);
+    }
     for (unsigned w = 0; w < workers; ++w) S->threads.emplace_back(worker, S, (int)w);
     *out = S;
     return LYM_OK;
@@ -324,6 +371,7 @@ void lym_server_destroy(lym_server* S) {
     for (auto& t : S->threads) t.join();
     for (auto& mine : S->states)
         for (auto& kv : mine) free_state(kv.second);
+    for (hipStream_t st : S->streams) (void)hipStreamDestroy(st);
     for (auto& L : S->lattices) {
         if (L->gauge) lym_gauge_destroy(L->gauge);
         if (L->base) lym_ctx_destroy(L->base);

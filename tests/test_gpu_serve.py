@@ -208,3 +208,71 @@ def test_serving_workload_at_full_size():
         assert info.status == lym.REQ_DONE
         assert np.array_equal(alone, results[q][1]), q
     srv.close()
+
+
+def _oracle_check(dims, U, kappa, x, b, tol):
+    orc = OracleField(dims, fold_bc(dims, U, -1), kappa=kappa, csw=CSW, need_clover=True, need_inverse=False)
+    Dx = orc.apply(K_CLOVER, x, gamma5=False)
+    res = np.linalg.norm((b - Dx).ravel()) / np.linalg.norm(b.ravel())
+    assert res <= tol, (kappa, res, tol)
+    return res
+
+
+def test_continuous_batching_mixed_kappa_and_refill():
+    """One worker, lanes for 3 requests: 9 requests of different kappa and
+    tolerance on one lattice, submitted at once.  They share batches (a lane
+    freed by a request that stops early takes the next queued one, at any
+    kappa), every result meets its tolerance against the oracle, and each
+    equals the same request solved alone, bit for bit."""
+    dims = (8, 8, 8, 8)
+    U = lym.gen_su3_links(dims, 0.5, 650)
+    V = int(np.prod(dims))
+    srv = lym.Server(workers=1, max_batch=3)
+    lid = srv.add_lattice(dims, U, csw=CSW)
+    spec = [(0.120 + 0.0015 * k, [1e-6, 1e-12, 1e-9][k % 3], lym.gaussian_spinor(V, 660 + k)) for k in range(9)]
+    rids = [srv.submit(lid, kappa, tol, b) for kappa, tol, b in spec]
+    mixed = [srv.wait(r) for r in rids]
+    sizes = [int(i.batch_size) for i, _ in mixed]
+    print("batch sizes:", sizes, "steps:", [int(i.solve.steps) for i, _ in mixed])
+    assert max(sizes) == 3 and min(sizes) >= 2
+    for (kappa, tol, b), (info, x) in zip(spec, mixed):
+        assert info.status == lym.REQ_DONE and info.solve.converged
+        _oracle_check(dims, U, kappa, x, b, tol)
+    for (kappa, tol, b), (info, x) in zip(spec, mixed):
+        ia, alone = srv.wait(srv.submit(lid, kappa, tol, b))
+        assert ia.batch_size == 1
+        assert np.array_equal(alone, x)
+    srv.close()
+
+
+def test_request_joins_a_running_batch():
+    """With one worker, a request that arrives while another one is being
+    solved on the same lattice joins its batch at the next chunk boundary
+    (it does not wait for it to finish) and both results are unchanged."""
+    import time
+    dims = (16, 16, 16, 32)
+    U = lym.gen_su3_links(dims, 0.5, 670)
+    V = int(np.prod(dims))
+    srv = lym.Server(workers=1, max_batch=4)
+    lid = srv.add_lattice(dims, U, csw=CSW)
+    b1, b2 = lym.gaussian_spinor(V, 671), lym.gaussian_spinor(V, 672)
+    srv.wait(srv.submit(lid, 0.125, 1e-8, b1))  # warm: state and captured chunks
+    r1 = srv.submit(lid, 0.133, 1e-12, b1)
+    time.sleep(0.004)
+    t2 = time.perf_counter()
+    r2 = srv.submit(lid, 0.121, 1e-10, b2)
+    i1, x1 = srv.wait(r1)
+    i2, x2 = srv.wait(r2)
+    print("first: %d steps, %.1f ms, batch %d; second: queued %.2f ms, batch %d"
+          % (i1.solve.steps, 1e3 * i1.latency_seconds, i1.batch_size, 1e3 * i2.queue_seconds, i2.batch_size))
+    assert i1.solve.converged and i2.solve.converged
+    # the second arrived while the first was in service: it joined, not queued behind it
+    assert i2.batch_size == 2 and i1.batch_size == 2
+    assert i2.queue_seconds < 0.5 * i1.service_seconds
+    _oracle_check(dims, U, 0.133, x1, b1, 1e-12)
+    _oracle_check(dims, U, 0.121, x2, b2, 1e-10)
+    _, a1 = srv.wait(srv.submit(lid, 0.133, 1e-12, b1))
+    _, a2 = srv.wait(srv.submit(lid, 0.121, 1e-10, b2))
+    assert np.array_equal(a1, x1) and np.array_equal(a2, x2)
+    assert t2 > 0
+    srv.close()
