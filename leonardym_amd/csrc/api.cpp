@@ -6,6 +6,7 @@
 // clover blocks; a solver holds its work vectors and a device state block.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -20,6 +21,8 @@
 using lym::zd;
 
 namespace lym_internal {
+
+void forget_op_captures(const lym_op* op);  // solvers.cpp
 
 thread_local std::string g_err;
 
@@ -777,6 +780,8 @@ int lym_op_create(lym_ctx* c, int kind, unsigned power, double kappa, double csw
     lym_op* op = new lym_op();
     std::memset(op, 0, sizeof(*op));
     op->ctx = c;
+    static std::atomic<uint64_t> next_uid{1};
+    op->uid = next_uid++;
     op->kind = kind;
     op->power = power;
     op->kappa = kappa;
@@ -808,6 +813,7 @@ int lym_op_create_by_name(lym_ctx* c, const char* name, unsigned power, double k
 void lym_op_destroy(lym_op* op) {
     if (!op) return;
     if (op->ctx && op->ctx->stream) (void)hipStreamSynchronize(op->ctx->stream);
+    lym_internal::forget_op_captures(op);
     free_sloppy(op);
     if (op->A) (void)hipFree(op->A);
     if (op->Ainv) (void)hipFree(op->Ainv);

@@ -175,6 +175,7 @@ struct lym_spinor {
 
 struct lym_op {
     lym_ctx* ctx;
+    uint64_t uid;      // unique for the process's lifetime (keys of captured solver iterations)
     int kind;          // LYM_OP_*
     unsigned power;
     double kappa, csw;

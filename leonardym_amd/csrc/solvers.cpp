@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <set>
 #include <string>
 
 #include "../../include/lym.h"
@@ -95,10 +96,12 @@ struct lym_solver {
     zd* bzero;              // zero source / dummy destination for padded lanes
     // plain BiCGStab on A + shift (the reference's shifted overload)
     double shift_re, shift_im;
-    // captured iteration chunks (see queue_iterations)
-    uint64_t graph_key[4];
-    hipGraphExec_t graph_exec[4];
-    uint64_t graph_used[4], graph_tick;
+    // captured iteration chunks (see queue_iterations), with the id of the
+    // operator each one launches
+    uint64_t graph_key[16];
+    hipGraphExec_t graph_exec[16];
+    uint64_t graph_used[16], graph_tick;
+    uint64_t graph_op[16];
 };
 
 namespace {
@@ -209,9 +212,11 @@ struct Key {
 };
 
 // what the operator's launches capture (kappa is read from device memory, so a
-// kappa change keeps the captures valid; csw enters only through the blocks)
+// kappa change keeps the captures valid; csw enters only through the blocks).
+// The operator's id, never reused, keeps a capture from matching another
+// operator that happens to be allocated at the same addresses.
 void key_op(Key& k, const lym_op* op) {
-    k << op << op->ctx << op->kind << op->power << op->dev_kappa << (op->csw != 0.0) << op->gamma5 << op->gauge << op->gauge->U
+    k << op->uid << op << op->ctx << op->kind << op->power << op->dev_kappa << (op->csw != 0.0) << op->gamma5 << op->gauge << op->gauge->U
       << op->gauge->compression << op->A << op->Ainv << op->tmp << op->tmp2 << op->sloppy_prec << op->Us << op->Usf << op->As
       << op->As_sc << op->Ainvs << op->Ainvs_sc << op->ctx->g.stride << op->ctx->g.Vh;
 }
@@ -221,11 +226,27 @@ bool graphs_enabled() {
     return !(e && e[0] == '0');
 }
 
+constexpr int GRAPH_SLOTS = 16;
+
+// Live solvers, so that destroying an operator can drop every capture that
+// launches it (no capture outlives the buffers it reads); the mutex also
+// guards each solver's capture cache.
+std::mutex& graph_mutex() {
+    static std::mutex m;
+    return m;
+}
+std::set<lym_solver*>& live_solvers() {
+    static std::set<lym_solver*> s;
+    return s;
+}
+
 void drop_graphs(lym_solver* s) {
-    for (int i = 0; i < 4; ++i) {
+    std::lock_guard<std::mutex> lk(graph_mutex());
+    for (int i = 0; i < GRAPH_SLOTS; ++i) {
         if (s->graph_exec[i]) (void)hipGraphExecDestroy(s->graph_exec[i]);
         s->graph_exec[i] = nullptr;
         s->graph_key[i] = 0;
+        s->graph_op[i] = 0;
     }
 }
 
@@ -234,20 +255,27 @@ void drop_graphs(lym_solver* s) {
 // device state).  When replaying, the queued count is a multiple of
 // GRAPH_ITERS, so the index modulo GRAPH_ITERS is the same in both paths.
 template <class F>
-int queue_iterations(lym_solver* s, uint64_t key, unsigned k, F&& body, unsigned* queued) {
+int queue_iterations(lym_solver* s, uint64_t key, uint64_t op_uid, unsigned k, F&& body, unsigned* queued) {
     hipStream_t st = s->ctx->stream;
     if (!graphs_enabled()) {
         for (unsigned i = 0; i < k; ++i) body(*queued + i);
         *queued += k;
         return LYM_OK;
     }
+    std::unique_lock<std::mutex> lk(graph_mutex());
     int slot = -1;
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < GRAPH_SLOTS; ++i)
         if (s->graph_exec[i] && s->graph_key[i] == key) slot = i;
     if (slot < 0) {
-        slot = 0;
-        for (int i = 1; i < 4; ++i)
-            if (s->graph_used[i] < s->graph_used[slot]) slot = i;
+        // a free slot, else the least recently used one (its last replay
+        // ended before the host read the state that led here)
+        for (int i = 0; i < GRAPH_SLOTS && slot < 0; ++i)
+            if (!s->graph_exec[i]) slot = i;
+        if (slot < 0) {
+            slot = 0;
+            for (int i = 1; i < GRAPH_SLOTS; ++i)
+                if (s->graph_used[i] < s->graph_used[slot]) slot = i;
+        }
         if (s->graph_exec[slot]) (void)hipGraphExecDestroy(s->graph_exec[slot]);
         s->graph_exec[slot] = nullptr;
         s->graph_key[slot] = 0;
@@ -263,10 +291,13 @@ int queue_iterations(lym_solver* s, uint64_t key, unsigned k, F&& body, unsigned
             return fail(LYM_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(e));
         }
         s->graph_key[slot] = key;
+        s->graph_op[slot] = op_uid;
     }
     s->graph_used[slot] = ++s->graph_tick;
+    hipGraphExec_t exec = s->graph_exec[slot];
+    lk.unlock();
     const unsigned n = (k + GRAPH_ITERS - 1) / GRAPH_ITERS;
-    for (unsigned i = 0; i < n; ++i) HIPCHECK(hipGraphLaunch(s->graph_exec[slot], st));
+    for (unsigned i = 0; i < n; ++i) HIPCHECK(hipGraphLaunch(exec, st));
     *queued += n * GRAPH_ITERS;
     return LYM_OK;
 }
@@ -308,7 +339,7 @@ int run_cg(lym_solver* s, const LinOp& L, const zd* b, zd* xout, const zd* x0, l
     };
     while (!H->done && queued <= s->max_steps) {
         unsigned k = next_chunk(r0, H->rr, (unsigned)H->step, s->precision, s->max_steps + 1 - queued);
-        int rc = queue_iterations(s, key.h, k, body, &queued);
+        int rc = queue_iterations(s, key.h, L.op->uid, k, body, &queued);
         if (rc) return rc;
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(H, S, sizeof(lym::CGState), hipMemcpyDeviceToHost, st));
@@ -364,7 +395,7 @@ int run_bicgstab(lym_solver* s, const LinOp& L, const zd* b, zd* xout, const zd*
     };
     while (!H->done && queued <= s->max_steps) {
         unsigned k = next_chunk(r0, rnow, (unsigned)H->step, s->precision, s->max_steps + 1 - queued);
-        int rc = queue_iterations(s, key.h, k, body, &queued);
+        int rc = queue_iterations(s, key.h, L.op->uid, k, body, &queued);
         if (rc) return rc;
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(H, S, sizeof(lym::BiCGState), hipMemcpyDeviceToHost, st));
@@ -493,7 +524,7 @@ int run_mixed_cg(lym_solver* s, const LinOp& L, const SloppyOp& SL, const zd* b,
     };
     while (!H->done && queued <= s->max_steps) {
         unsigned k = next_chunk(r0, H->rr_s, (unsigned)H->step, s->precision, s->max_steps + 1 - queued);
-        int rc = queue_iterations(s, key.h, k, body, &queued);
+        int rc = queue_iterations(s, key.h, L.op->uid, k, body, &queued);
         if (rc) return rc;
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(H, S, sizeof(lym::MCGState), hipMemcpyDeviceToHost, st));
@@ -802,7 +833,7 @@ int run_batch(lym_solver* s, lym_op* op, bool eo, bool half, int nr, int thr, in
     while (!Sh->all_done && queued <= s->max_steps) {
         unsigned k = queued == 0 ? 8 : 16;
         if (k > s->max_steps + 1 - queued) k = s->max_steps + 1 - queued;
-        int rc = queue_iterations(s, key.h, k, body, &queued);
+        int rc = queue_iterations(s, key.h, op->uid, k, body, &queued);
         if (rc) return rc;
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(Sh, Sdev, sizeof(lym::BState), hipMemcpyDeviceToHost, st));
@@ -837,6 +868,23 @@ int run_batch(lym_solver* s, lym_op* op, bool eo, bool half, int nr, int thr, in
 }
 
 }  // namespace
+
+namespace lym_internal {
+// Drops every captured iteration chunk that launches this operator (called
+// when it is destroyed, after its stream has drained): no capture outlives
+// the buffers it reads.
+void forget_op_captures(const lym_op* op) {
+    std::lock_guard<std::mutex> lk(graph_mutex());
+    for (lym_solver* s : live_solvers())
+        for (int i = 0; i < GRAPH_SLOTS; ++i)
+            if (s->graph_exec[i] && s->graph_op[i] == op->uid) {
+                (void)hipGraphExecDestroy(s->graph_exec[i]);
+                s->graph_exec[i] = nullptr;
+                s->graph_key[i] = 0;
+                s->graph_op[i] = 0;
+            }
+}
+}  // namespace lym_internal
 
 extern "C" {
 
@@ -912,6 +960,10 @@ int lym_solver_create(lym_ctx* c, int kind, lym_solver** out) {
     s->even_odd = 1;
     s->sloppy_prec = LYM_PREC_HALF;
     s->delta = 0.05;
+    {
+        std::lock_guard<std::mutex> lk(graph_mutex());
+        live_solvers().insert(s);
+    }
     *out = s;
     return LYM_OK;
 }
@@ -931,6 +983,10 @@ void lym_solver_destroy(lym_solver* s) {
     if (s->host_state) (void)hipHostFree(s->host_state);
     free_batch(s);
     drop_graphs(s);
+    {
+        std::lock_guard<std::mutex> lk(graph_mutex());
+        live_solvers().erase(s);
+    }
     if (s->bpartials) (void)hipFree(s->bpartials);
     if (s->bstate) (void)hipFree(s->bstate);
     if (s->bhost) (void)hipHostFree(s->bhost);

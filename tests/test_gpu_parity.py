@@ -448,3 +448,31 @@ def test_mixed_cg_zero_source(case, kind, op_kind, power, sloppy):
     solver = lym.Solver(case.ctx, kind, precision=eps, max_steps=20000, sloppy=sloppy)
     info = solver.solve(case.op(op_kind, power=power), case.spinor(b), X)
     assert info.converged and info.steps > 0
+
+
+def test_graph_captures_follow_operator_lifetimes(case, monkeypatch):
+    """One solver, a sequence of operators created and destroyed in turn (so a
+    later one may reuse an earlier one's addresses): captured iterations are
+    dropped with their operator and never replayed for another, and every
+    solve equals the same solve with the kernels queued one by one."""
+    b = case.gauss()
+    eps = (1e-11 * np.linalg.norm(b)) ** 2
+    kappas = [case.kappa * f for f in (1.0, 0.95, 0.9, 0.95, 1.0, 0.85)]
+
+    def run():
+        s = lym.Solver(case.ctx, lym.SOLVER_PRECOND_MIXED_CG, precision=eps, max_steps=5000, sloppy=lym.PREC_HALF)
+        out = []
+        for k in kappas:
+            op = case.op(lym.OP_CLOVER, kappa=k)
+            X = case.spinor()
+            info = s.solve(op, case.spinor(b), X)
+            out.append((X.download(), int(info.steps)))
+            op.close()
+        return out
+
+    monkeypatch.setenv("LYM_GRAPHS", "0")
+    eager = run()
+    monkeypatch.delenv("LYM_GRAPHS")
+    graph = run()
+    for (xe, se), (xg, sg) in zip(eager, graph):
+        assert se == sg and np.array_equal(xe, xg)
