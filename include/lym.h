@@ -240,6 +240,23 @@ int lym_server_release(lym_server* s, uint64_t request_id);   /* forget a comple
 int lym_server_drain(lym_server* s);                           /* wait until nothing is queued or running */
 int lym_server_memory(lym_server* s, size_t* budget_bytes, size_t* in_use_bytes);
 
+/* ------------------------------------------------------------------ measurements
+ * Connected meson correlators from point-source propagators:
+ * MesonCorrelator::execute (source/correlators/MesonCorrelator.cpp:22-200).
+ * The 12 spin-colour unit sources at source_site (StochasticEstimator::
+ * generateSource, source/fermion_measurements/StochasticEstimator.cpp:149-161,
+ * there at the origin) are solved for D x = e with the given solver and
+ * operator (gamma5 is switched off for the solves, as MesonCorrelator.cpp:71
+ * does, and restored); a mixed-precision CG solver solves all 12 in one
+ * batched call.  out holds 4 x Lt doubles, each indexed by the sink time
+ * relative to the source timeslice (the reference's source at t = 0 of links
+ * shifted by t_source_origin, :54-66): pion_exact, scalar_exact, ps_exact
+ * (normalised by 4 kappa^2, :162) and pcac_mass (:189-190).  infos (NULL or
+ * 12) receives the solves, column c * 4 + alpha for source spin alpha, colour c.
+ * The stochastic disconnected part (:201-) is not computed. */
+int lym_meson_correlators(lym_solver* solver, lym_op* op, const int source_site[4], double* out,
+                          lym_solve_info* infos);
+
 /* ------------------------------------------------------------------ diagnostics
  * The workgroup -> site-block mapping of one stencil launch (no counterpart in
  * the reference), evaluated on the host by the device's own mapping functions

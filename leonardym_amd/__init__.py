@@ -109,6 +109,7 @@ SIGNATURES = {
     "lym_gen_gaussian_spinor": (_I, [_L, _U64, _P]),
     "lym_point_source": (_I, [_P, _P, _I, _I, _P]),
     "lym_launch_plan": (_I, [_P, _I, _I, _I, _I, _PI, _PI, _PI, _PI, _PI, _L, _PI]),
+    "lym_meson_correlators": (_I, [_P, _P, _P, _PD, ctypes.POINTER(SolveInfo)]),
 }
 
 _lib = None
@@ -461,6 +462,19 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+
+def meson_correlators(solver, op, source_site):
+    """MesonCorrelator::execute on the device: dict of pion, scalar, ps (4 kappa^2
+    normalised) and pcac arrays over the sink time relative to the source
+    timeslice, and the 12 SolveInfo of the propagator columns."""
+    T = op.ctx.dims[3]
+    out = np.zeros(4 * T)
+    infos = (SolveInfo * 12)()
+    _check(lib().lym_meson_correlators(solver.h, op.h, _arr(np.array(source_site, np.int32)),
+                                       out.ctypes.data_as(_PD), infos))
+    return {"pion": out[:T], "scalar": out[T:2 * T], "ps": out[2 * T:3 * T], "pcac": out[3 * T:],
+            "infos": [infos[i] for i in range(12)]}
 
 
 class Server:

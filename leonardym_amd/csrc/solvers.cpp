@@ -870,6 +870,9 @@ int run_batch(lym_solver* s, lym_op* op, bool eo, bool half, int nr, int thr, in
 }  // namespace
 
 namespace lym_internal {
+int solver_kind(const lym_solver* s) { return s->kind; }
+int solver_even_odd(const lym_solver* s) { return s->even_odd; }
+
 // Drops every captured iteration chunk that launches this operator (called
 // when it is destroyed, after its stream has drained): no capture outlives
 // the buffers it reads.

@@ -862,4 +862,80 @@ int orc_precond_bicgstab(const orc_op_desc* d, const double* bsrc, double* xout,
     return ok;
 }
 
+// MesonCorrelator::execute, connected part (MesonCorrelator.cpp:22-199).  The
+// 12 unit sources at `site` (StochasticEstimator.cpp:149-161, there the
+// origin) are solved with PreconditionedBiCGStab on D (gamma5 off, :71): the
+// even-odd driver for the clover operator (kind 2), plain BiCGStab otherwise
+// (PreconditionedBiCGStab.cpp:43 falls back to it); the propagator column of
+// source (alpha, c) is c * 4 + alpha (:94-98).  The pion (:104-118), scalar
+// (:121-139) and pseudoscalar-axial (:142-160) contractions are summed per
+// timeslice in long double, normalised by 4 kappa^2 (:162), with the PCAC
+// ratio of the raw sums (:189-190).  Every output is indexed by the sink time
+// relative to the source's, as the reference's source at t = 0 on links
+// shifted by t_source_origin (:54-66).  out: 4 x Lt (pion, scalar, ps, pcac).
+// Returns 1 when all 12 solves converged.
+int orc_meson_correlators(const orc_op_desc* d, const int* site, double precision, unsigned max_steps, double* out) {
+    const Geo& g = geo(d->dims);
+    const long V = g.V;
+    const int T = g.L[3];
+    // gamma5 gamma_3 and gamma_3 (Gamma.cpp:8-9, rows listed)
+    const cd G53[4][4] = {{0, 0, 1, 0}, {0, 0, 0, 1}, {-1, 0, 0, 0}, {0, -1, 0, 0}};
+    const cd G3[4][4] = {{0, 0, 1, 0}, {0, 0, 0, 1}, {1, 0, 0, 0}, {0, 1, 0, 0}};
+    std::vector<std::vector<cd>> P(12, std::vector<cd>(V * 12));
+    const long idx = (((long)site[0] * g.L[1] + site[1]) * g.L[2] + site[2]) * T + site[3];
+    orc_op_desc dd = *d;
+    dd.power = 1;
+    dd.gamma5 = 0;
+    int all_ok = 1;
+    for (int alpha = 0; alpha < 4; ++alpha)
+        for (int c = 0; c < 3; ++c) {
+            std::vector<cd> src(V * 12, cd(0, 0));
+            src[idx * 12 + 3 * alpha + c] = 1;
+            orc_solve_info info;
+            double* x = reinterpret_cast<double*>(P[c * 4 + alpha].data());
+            const double* b = reinterpret_cast<const double*>(src.data());
+            int ok = dd.kind == K_CLOVER ? orc_precond_bicgstab(&dd, b, x, nullptr, precision, max_steps, &info)
+                                         : orc_bicgstab(&dd, b, x, nullptr, precision, max_steps, &info);
+            all_ok = all_ok && ok;
+        }
+    auto S = [&](int col, long s, int mu, int dcol) { return P[col][s * 12 + 3 * mu + dcol]; };
+    std::vector<long double> pion(T, 0.0L), scalar(T, 0.0L), ps(T, 0.0L);
+    for (int alpha = 0; alpha < 4; ++alpha)
+        for (int c = 0; c < 3; ++c)
+            for (long s = 0; s < V; ++s)
+                for (int mu = 0; mu < 4; ++mu)
+                    for (int dc = 0; dc < 3; ++dc) pion[s % T] += std::norm(S(c * 4 + alpha, s, mu, dc));
+    for (int alpha = 0; alpha < 4; ++alpha)
+        for (int beta = 0; beta < 4; ++beta)
+            for (int c = 0; c < 3; ++c)
+                for (long s = 0; s < V; ++s)
+                    for (int mu = 0; mu < 4; ++mu)
+                        for (int nu = 0; nu < 4; ++nu)
+                            for (int dc = 0; dc < 3; ++dc)
+                                scalar[s % T] += std::real(G53[mu][nu] * G53[alpha][beta] *
+                                                           std::conj(S(c * 4 + alpha, s, mu, dc)) *
+                                                           S(c * 4 + beta, s, nu, dc));
+    for (int alpha = 0; alpha < 4; ++alpha)
+        for (int beta = 0; beta < 4; ++beta)
+            for (int c = 0; c < 3; ++c)
+                for (long s = 0; s < V; ++s)
+                    for (int mu = 0; mu < 4; ++mu)
+                        for (int dc = 0; dc < 3; ++dc) {
+                            cd v = -G3[alpha][beta] * std::conj(S(c * 4 + alpha, s, mu, dc)) * S(c * 4 + beta, s, mu, dc);
+                            v += G3[alpha][beta] * std::conj(S(c * 4 + mu, s, alpha, dc)) * S(c * 4 + mu, s, beta, dc);
+                            ps[s % T] += std::real(v);
+                        }
+    const long double factor = 4.0L * d->kappa * d->kappa;
+    const int t0 = site[3];
+    for (int t = 0; t < T; ++t) {
+        const int a = (t0 + t) % T;
+        const int am = (t0 + (t == 0 ? T - 1 : t - 1)) % T, ap = (t0 + (t == T - 1 ? 0 : t + 1)) % T;
+        out[t] = (double)(factor * pion[a]);
+        out[T + t] = (double)(factor * scalar[a]);
+        out[2 * T + t] = (double)(factor * ps[a]);
+        out[3 * T + t] = (double)((ps[am] - ps[ap]) / (4.0L * pion[a]));
+    }
+    return all_ok;
+}
+
 }  // extern "C"

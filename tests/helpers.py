@@ -77,6 +77,8 @@ def oracle():
                                        ctypes.c_double, ctypes.c_double, ctypes.POINTER(SolveInfo)]
     lib.orc_bicgstab_shift.restype = ctypes.c_int
     lib.orc_num_threads.restype = ctypes.c_int
+    lib.orc_meson_correlators.argtypes = [ctypes.POINTER(OpDesc), P, ctypes.c_double, ctypes.c_uint, P]
+    lib.orc_meson_correlators.restype = ctypes.c_int
     _oracle = lib
     return lib
 
@@ -147,6 +149,16 @@ class OracleField:
         oracle().orc_multiply(ctypes.byref(d), _p(out), _p(psi), _p(v2c), float(np.real(alpha)),
                               float(np.imag(alpha)))
         return out
+
+    def meson_correlators(self, site, eps, max_steps=5000, kind=K_CLOVER):
+        """MesonCorrelator::execute (connected part): dict of pion, scalar, ps,
+        pcac over the sink time relative to the source's, and whether every
+        solve converged."""
+        d = self.desc(kind, 1, False)
+        T = self.dims[3]
+        out = np.zeros(4 * T)
+        ok = oracle().orc_meson_correlators(ctypes.byref(d), _p(np.array(site, np.int32)), eps, max_steps, _p(out))
+        return {"pion": out[:T], "scalar": out[T:2 * T], "ps": out[2 * T:3 * T], "pcac": out[3 * T:], "ok": bool(ok)}
 
     def solve(self, which, b, eps, max_steps=5000, kind=K_CLOVER, power=1, gamma5=True, x0=None, shift=None):
         d = self.desc(kind, power, gamma5)

@@ -474,3 +474,134 @@ def test_solver_iterates_after_fixed_steps(dense_case, nsteps):
     sol, info, ok = fld.solve("cg", b, 1e-300, kind=K_CLOVER, power=2, max_steps=nsteps)
     xn, _ = _np_cg(Q @ Q, b.ravel(), b.ravel().copy(), 1e-300, nsteps)
     assert not ok and _relerr(sol.ravel(), xn) < 1e-12
+
+
+# ---------------------------------------------------------------- meson correlators
+# MesonCorrelator::execute (source/correlators/MesonCorrelator.cpp:22-199),
+# restated in the oracle with the reference's index loops.  Pinned here by the
+# free-field propagator in closed form with the contractions written as spin
+# traces, by gauge invariance, and by the reference's own time shift.
+G53 = GAMMA5 @ GAMMA[3]
+
+
+def free_propagator(dims, kappa):
+    """S(x) = D^-1 delta_{x,0} for U = 1, time-antiperiodic: (1/V) sum_p e^{ipx}
+    D(p)^-1 with p_t = (2n+1) pi / T, as [Lx, Ly, Lz, Lt, sink spin, source spin]."""
+    n = [np.arange(L) for L in dims]
+    p = np.meshgrid(*[2 * np.pi * n[m] / dims[m] for m in range(3)], (2 * n[3] + 1) * np.pi / dims[3],
+                    indexing="ij")
+    D = (1 - 2 * kappa * sum(np.cos(q) for q in p))[..., None, None] * np.eye(4)
+    for mu in range(4):
+        D = D + 2j * kappa * np.sin(p[mu])[..., None, None] * GAMMA[mu]
+    Dinv = np.linalg.inv(D)
+    S = np.fft.ifftn(Dinv, axes=(0, 1, 2, 3))  # (1/V) sum_n e^{2 pi i n x / L}
+    t = np.arange(dims[3])
+    return S * np.exp(1j * np.pi * t / dims[3])[None, None, None, :, None, None]
+
+
+def trace_correlators(S, kappa):
+    """Per timeslice, 4 kappa^2 x: pion = tr S^+ S, scalar = Re tr(G^T S^+ G S),
+    ps = Re[tr(g S S^+) - tr(g^T S^+ S)] with G = gamma5 gamma_3 (x) 1,
+    g = gamma_3 (x) 1, for the propagator as matrices S[..., (sink spin, colour),
+    (source spin, colour)] over [Lx, Ly, Lz, Lt]; and the PCAC ratio."""
+    G = np.kron(G53, np.eye(3))
+    g = np.kron(GAMMA[3], np.eye(3))
+    Sd = np.conj(np.swapaxes(S, -1, -2))
+    pion = np.einsum("...ab,...ba->...", Sd, S).real
+    scalar = np.einsum("ab,...ab->...", G, Sd @ G @ S).real  # tr(A^T B) = sum_ab A_ab B_ab
+    ps = (np.einsum("ab,...ba->...", g, S @ Sd) - np.einsum("ab,...ab->...", g, Sd @ S)).real
+    f = 4 * kappa ** 2
+    raw = {k: v.sum(axis=(0, 1, 2)) for k, v in (("pion", pion), ("scalar", scalar), ("ps", ps))}
+    out = {k: f * v for k, v in raw.items()}
+    out["pcac"] = (np.roll(raw["ps"], 1) - np.roll(raw["ps"], -1)) / (4 * raw["pion"])
+    return out
+
+
+@pytest.mark.parametrize("kind", [K_WILSON, K_CLOVER])
+def test_meson_correlators_free_field(kind):
+    """U = 1: the oracle's 12 solves and contractions equal the closed-form
+    propagator's spin traces (BiCGStab on D for Wilson, the even-odd driver
+    for clover, whose term vanishes at U = 1)."""
+    dims = (4, 4, 4, 8)
+    kappa = 0.11
+    V = int(np.prod(dims))
+    U = np.broadcast_to(np.eye(3, dtype=complex), (V, 4, 3, 3)).copy()
+    fld = OracleField(dims, fold_bc(dims, U), kappa=kappa, csw=1.0, need_clover=kind == K_CLOVER)
+    got = fld.meson_correlators((0, 0, 0, 0), 1e-26, kind=kind)
+    assert got["ok"]
+    want = trace_correlators(np.kron(free_propagator(dims, kappa), np.eye(3)), kappa)
+    for k in ("pion", "scalar"):
+        scale = np.max(np.abs(want[k]))
+        assert np.max(np.abs(got[k] - want[k])) <= 1e-10 * scale, (k, got[k], want[k])
+    assert np.all(got["pion"] > 0)
+    # for S = a + b.gamma the PS combination tr(gamma_3 [S, S^+]) vanishes identically
+    assert np.max(np.abs(got["ps"])) <= 1e-12 * np.max(got["pion"])
+    assert np.max(np.abs(want["ps"])) <= 1e-12 * np.max(want["pion"])
+
+
+def test_meson_correlators_random_links_trace_form():
+    """Random links, where every channel is nonzero: the oracle's contractions
+    (the reference's index loops) equal the spin-colour trace forms applied to
+    the propagator columns solved separately with the oracle's even-odd driver."""
+    rng = np.random.default_rng(37)
+    dims = (4, 4, 4, 6)
+    kappa = 0.12
+    V = int(np.prod(dims))
+    U = fold_bc(dims, random_links(dims, rng, eps=0.5))
+    fld = OracleField(dims, U, kappa=kappa, csw=1.0, need_clover=True)
+    site = (1, 0, 2, 1)
+    got = fld.meson_correlators(site, 1e-26)
+    idx = ((site[0] * dims[1] + site[1]) * dims[2] + site[2]) * dims[3] + site[3]
+    S = np.zeros((V, 12, 12), complex)
+    for alpha in range(4):
+        for c in range(3):
+            b = np.zeros((V, 4, 3), complex)
+            b[idx, alpha, c] = 1
+            x, info, ok = fld.solve("precond_bicgstab", b, 1e-26)
+            assert ok
+            S[:, :, 3 * alpha + c] = x.reshape(V, 12)
+    want = trace_correlators(S.reshape(dims + (12, 12)), kappa)
+    for k in ("pion", "scalar", "ps", "pcac"):
+        want[k] = np.roll(want[k], -site[3])  # relative to the source timeslice
+        assert np.max(np.abs(got[k] - want[k])) <= 1e-9 * np.max(np.abs(want[k])), (k, got[k], want[k])
+    assert np.max(np.abs(want["ps"])) > 1e-6 * np.max(want["pion"])  # far above rounding: the channel is not empty
+
+
+def random_gauge_transform(dims, U, rng):
+    """U'_mu(x) = g(x) U_mu(x) g(x + mu)^+ with Haar-random g."""
+    from helpers import random_su3
+    V = int(np.prod(dims))
+    g = random_su3(V, rng)
+    c = coords(dims)
+    out = np.empty_like(U)
+    for mu in range(4):
+        cu = c.copy()
+        cu[:, mu] = (cu[:, mu] + 1) % dims[mu]
+        iu = ((cu[:, 0] * dims[1] + cu[:, 1]) * dims[2] + cu[:, 2]) * dims[3] + cu[:, 3]
+        out[:, mu] = g @ U[:, mu] @ np.conj(np.swapaxes(g[iu], -1, -2))
+    return out
+
+
+def test_meson_correlators_gauge_invariant_and_time_shift():
+    """On random links: (i) a gauge transformation leaves every correlator
+    unchanged (the colour indices are traced correctly); (ii) the source at
+    time t0 gives, relative to t0, what the source at t = 0 gives on links
+    shifted by t0 in time (the reference's t_source_origin, MesonCorrelator.cpp:54-66)."""
+    rng = np.random.default_rng(31)
+    dims = (4, 4, 4, 6)
+    kappa = 0.12
+    U = random_links(dims, rng, eps=0.5)
+    base = OracleField(dims, fold_bc(dims, U), kappa=kappa, csw=1.0, need_clover=True)
+    ref = base.meson_correlators((1, 2, 3, 2), 1e-26)
+    gt = OracleField(dims, fold_bc(dims, random_gauge_transform(dims, U, rng)), kappa=kappa, csw=1.0,
+                     need_clover=True).meson_correlators((1, 2, 3, 2), 1e-26)
+    for k in ("pion", "scalar", "ps", "pcac"):
+        assert np.max(np.abs(gt[k] - ref[k])) <= 1e-9 * np.max(np.abs(ref[k])), k
+    # the reference's shift: lattice'[x, t] = lattice[x, t + t0] on the links
+    # with the boundary sign folded in, source at time 0
+    t0 = 2
+    Uf = fold_bc(dims, U).reshape(dims + (4, 3, 3))
+    shifted = np.roll(Uf, -t0, axis=3).reshape(-1, 4, 3, 3)
+    sh = OracleField(dims, shifted, kappa=kappa, csw=1.0, need_clover=True).meson_correlators((1, 2, 3, 0), 1e-26)
+    for k in ("pion", "scalar", "ps", "pcac"):
+        assert np.max(np.abs(sh[k] - ref[k])) <= 1e-9 * np.max(np.abs(ref[k])), k
