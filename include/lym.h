@@ -257,6 +257,71 @@ int lym_server_memory(lym_server* s, size_t* budget_bytes, size_t* in_use_bytes)
 int lym_meson_correlators(lym_solver* solver, lym_op* op, const int source_site[4], double* out,
                           lym_solve_info* infos);
 
+/* ------------------------------------------------------------------ gauge-configuration files
+ * Readers and writers of the reference's configuration formats on host arrays
+ * of fundamental gauge links (gaugeLinkConfiguration raw data): V x [4 mu]
+ * [N x N column-major] complex double, N = ncolors (2 or 3), sites in the
+ * t-fastest lexicographic order above.  The fermion links of the operators
+ * are made from them as Environment::synchronize does (source/Environment.cpp:
+ * 21-79): lym_su2_to_adjoint for adjoint fermions, then the boundary sign.
+ *   LYM_FORMAT_LEONARD : "leonard_format" (ReadStartGaugeConfiguration.cpp:51-130,
+ *     OutputSweep.cpp:32-84): <dir><name>_<number>.descriptor.txt and one XDR
+ *     file per process <dir><name>_<number>_<proc>.txt of (x, y, z, t, links
+ *     as doubles); written as one process, read for any number of them
+ *   LYM_FORMAT_MUENSTER : "muenster_format", XDR floats with a trailer;
+ *     SU(2) <dir>0:<number>:<name> (ReadStartGaugeConfiguration.cpp:132-297,
+ *     OutputSweep.cpp:87-283), SU(3) <dir><name>.config-<number>
+ *     (ReadStartGaugeConfiguration.cpp:299-480, OutputSweep.cpp:284-440;
+ *     read links are re-unitarised from their first two columns)
+ * Errors (unreadable, short or malformed files, a lattice size different from
+ * dims) return LYM_ERR_INVALID; the reference prints and returns false. */
+#define LYM_FORMAT_LEONARD 0
+#define LYM_FORMAT_MUENSTER 1
+typedef struct {
+    double plaquette;  /* temporal plaquette: of the links when writing, the file's value when reading */
+    double beta;       /* muenster trailer (written as a float, as the reference does) */
+    double kappa;      /* muenster SU(2) trailer (the last, positive kappa) */
+    double c1;         /* muenster SU(3) trailer (-1/12 when written) */
+    int sweep;         /* muenster SU(2) trailer: sweep number */
+} lym_config_info;
+/* info: beta, kappa, sweep are inputs; plaquette (and c1) are set on return */
+int lym_config_write(int format, const char* directory, const char* name, int number, const int dims[4],
+                     int ncolors, const double* links, lym_config_info* info);
+/* info may be NULL */
+int lym_config_read(int format, const char* directory, const char* name, int number, const int dims[4],
+                    int ncolors, double* links, lym_config_info* info);
+/* Plaquette::temporalPlaquette (source/wilson_loops/Plaquette.cpp:49-66):
+ * sum over sites and mu < 3 of Re tr[U_mu(x) U_t(x+mu) U_mu(x+t)^+ U_t(x)^+] / (3 N V) */
+int lym_temporal_plaquette(const int dims[4], int ncolors, const double* links, double* out);
+
+/* ------------------------------------------------------------------ HMC fermion force
+ * TwoFlavorFermionAction::force of every link (source/actions/TwoFlavorFermionAction.cpp:17-20):
+ * minus FermionForce::force (source/hmc_forces/FermionForce.cpp:33-58) of
+ * DiracWilsonFermionForce::derivative (source/hmc_forces/DiracWilsonFermionForce.cpp:10-54)
+ * from the solutions X = Q^-2 phi and Y = Q^-1 phi, for a Wilson operator
+ * ("DiracWilson" / "BasicDiracWilson"; its kappa and gauge field).  force:
+ * host V x [4 mu][N x N column-major] complex double, N = 3 (SU(3)) or 2
+ * (SU(2) with adjoint fermions), traceless anti-hermitian.  As in the
+ * reference the derivative is taken with respect to the fermion link and
+ * projected with the link without the boundary sign, so on the time links
+ * that carry the antiperiodic sign it is the negative of the derivative with
+ * respect to the gauge link. */
+int lym_fermion_force(lym_op* op, const lym_spinor* X, const lym_spinor* Y, double* force);
+/* TwoFlavorFermionAction::updateForce (:21-43): Y = Q^-1 phi and X = Q^-1 Y
+ * with the solver (LYM_SOLVER_BICGSTAB or LYM_SOLVER_PRECOND_BICGSTAB, its
+ * precision and step limit; the reference uses a BiConjugateGradient), then
+ * lym_fermion_force.  infos: NULL or 2 entries (the two solves). */
+int lym_two_flavor_force(lym_solver* s, lym_op* op, const lym_spinor* phi, double* force, lym_solve_info* infos);
+
+/* Stout smearing of host fundamental gauge links (layout as lym_config_*) on
+ * the device, `levels` times with parameter rho: StoutSmearing::smearing
+ * (source/utils/StoutSmearing.cpp:12-20, smearLink :55-62) as
+ * Environment::synchronize applies it before building the fermion links
+ * (source/Environment.cpp:22-47).  in and out may be the same array.  An
+ * exponent that is exactly zero gives the identity (the reference divides
+ * 0 by 0 there). */
+int lym_stout_smear(const int dims[4], int ncolors, double rho, int levels, const double* in, double* out);
+
 /* ------------------------------------------------------------------ diagnostics
  * The workgroup -> site-block mapping of one stencil launch (no counterpart in
  * the reference), evaluated on the host by the device's own mapping functions

@@ -21,6 +21,7 @@ SOLVER_MIXED_CG, SOLVER_PRECOND_CG, SOLVER_PRECOND_MIXED_CG = 3, 4, 5
 PREC_DOUBLE, PREC_SINGLE, PREC_HALF = 0, 1, 2
 REQ_PENDING, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 LINKS_FULL, LINKS_RECONSTRUCT = 0, 1
+FORMAT_LEONARD, FORMAT_MUENSTER = 0, 1
 
 # C ABI: name -> (restype, argtypes)
 _P = ctypes.c_void_p
@@ -42,6 +43,11 @@ class RequestInfo(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("error", ctypes.c_int), ("batch_size", ctypes.c_uint),
                 ("worker", ctypes.c_int), ("queue_seconds", ctypes.c_double), ("service_seconds", ctypes.c_double),
                 ("latency_seconds", ctypes.c_double), ("solve", SolveInfo)]
+
+
+class ConfigInfo(ctypes.Structure):
+    _fields_ = [("plaquette", ctypes.c_double), ("beta", ctypes.c_double), ("kappa", ctypes.c_double),
+                ("c1", ctypes.c_double), ("sweep", ctypes.c_int)]
 
 
 SIGNATURES = {
@@ -110,6 +116,12 @@ SIGNATURES = {
     "lym_point_source": (_I, [_P, _P, _I, _I, _P]),
     "lym_launch_plan": (_I, [_P, _I, _I, _I, _I, _PI, _PI, _PI, _PI, _PI, _L, _PI]),
     "lym_meson_correlators": (_I, [_P, _P, _P, _PD, ctypes.POINTER(SolveInfo)]),
+    "lym_config_write": (_I, [_I, ctypes.c_char_p, ctypes.c_char_p, _I, _PI, _I, _P, ctypes.POINTER(ConfigInfo)]),
+    "lym_config_read": (_I, [_I, ctypes.c_char_p, ctypes.c_char_p, _I, _PI, _I, _P, ctypes.POINTER(ConfigInfo)]),
+    "lym_temporal_plaquette": (_I, [_PI, _I, _P, _PD]),
+    "lym_stout_smear": (_I, [_PI, _I, _D, _I, _P, _P]),
+    "lym_fermion_force": (_I, [_P, _P, _P, _P]),
+    "lym_two_flavor_force": (_I, [_P, _P, _P, _P, ctypes.POINTER(SolveInfo)]),
 }
 
 _lib = None
@@ -166,6 +178,50 @@ def launch_plan(dims, parity=-1, nr=1, threads=0, lowp=False):
 
 
 # ---------------------------------------------------------------- host-side inputs
+def _dims_arr(dims):
+    return np.ascontiguousarray(np.array(dims, np.int32))
+
+
+def config_write(fmt, directory, name, number, dims, U, beta=0.0, kappa=0.0, sweep=0):
+    """Write gauge links U [V, 4, N, N] (element (i, j) at [..., i, j]) in the
+    reference's leonard or muenster format; returns the ConfigInfo (plaquette)."""
+    N = U.shape[-1]
+    cm = np.ascontiguousarray(np.swapaxes(np.asarray(U, np.complex128), -1, -2))
+    d = _dims_arr(dims)
+    info = ConfigInfo(0.0, float(beta), float(kappa), 0.0, int(sweep))
+    _check(lib().lym_config_write(fmt, directory.encode(), name.encode(), int(number),
+                                  d.ctypes.data_as(_PI), N, _arr(cm), ctypes.byref(info)))
+    return info
+
+
+def config_read(fmt, directory, name, number, dims, ncolors):
+    """Read gauge links [V, 4, N, N] and the ConfigInfo of a configuration file."""
+    V = int(np.prod(dims))
+    cm = np.empty((V, 4, ncolors, ncolors), np.complex128)
+    d = _dims_arr(dims)
+    info = ConfigInfo()
+    _check(lib().lym_config_read(fmt, directory.encode(), name.encode(), int(number), d.ctypes.data_as(_PI),
+                                 ncolors, _arr(cm), ctypes.byref(info)))
+    return np.ascontiguousarray(np.swapaxes(cm, -1, -2)), info
+
+
+def temporal_plaquette(dims, U):
+    cm = np.ascontiguousarray(np.swapaxes(np.asarray(U, np.complex128), -1, -2))
+    d = _dims_arr(dims)
+    out = ctypes.c_double()
+    _check(lib().lym_temporal_plaquette(d.ctypes.data_as(_PI), U.shape[-1], _arr(cm), ctypes.byref(out)))
+    return out.value
+
+
+def stout_smear(dims, U, rho, levels=1):
+    """Stout-smeared copy of fundamental gauge links U [V, 4, N, N] (on the device)."""
+    cm = np.ascontiguousarray(np.swapaxes(np.asarray(U, np.complex128), -1, -2))
+    out = np.empty_like(cm)
+    d = _dims_arr(dims)
+    _check(lib().lym_stout_smear(d.ctypes.data_as(_PI), U.shape[-1], float(rho), int(levels), _arr(cm), _arr(out)))
+    return np.ascontiguousarray(np.swapaxes(out, -1, -2))
+
+
 def gen_su3_links(dims, eps, seed):
     V = int(np.prod(dims))
     out = np.empty((V, 4, 3, 3), np.complex128)  # memory: column-major per link
@@ -389,6 +445,16 @@ class Operator:
         a = complex(alpha)
         _check(lib().lym_op_multiply_add(self.h, out.h, v1.h, v2.h, a.real, a.imag))
 
+    def _force_array(self):
+        N = 3 if self.ctx.rep == REP_SU3_FUND else 2
+        return np.empty((self.ctx.V, 4, N, N), np.complex128)
+
+    def fermion_force(self, X, Y):
+        """TwoFlavorFermionAction::force of every link, [V, 4, N, N] (i, j) at [..., i, j]."""
+        out = self._force_array()
+        _check(lib().lym_fermion_force(self.h, X.h, Y.h, _arr(out)))
+        return np.ascontiguousarray(np.swapaxes(out, -1, -2))
+
     def multiply_sloppy(self, out, inp, prec):
         """out = Op inp in storage precision prec (PREC_SINGLE or PREC_HALF)."""
         _check(lib().lym_op_multiply_sloppy(self.h, prec, out.h, inp.h))
@@ -429,6 +495,15 @@ class Solver:
         _check(lib().lym_solver_solve(self.h, op.h, source.h, solution.h, guess.h if guess is not None else None,
                                       ctypes.byref(info)))
         return info
+
+    def two_flavor_force(self, op, phi):
+        """TwoFlavorFermionAction::updateForce: the two solves and the force
+        [V, 4, N, N]; returns (force, (info_Y, info_X))."""
+        N = 3 if op.ctx.rep == REP_SU3_FUND else 2
+        out = np.empty((op.ctx.V, 4, N, N), np.complex128)
+        infos = (SolveInfo * 2)()
+        _check(lib().lym_two_flavor_force(self.h, op.h, phi.h, _arr(out), infos))
+        return np.ascontiguousarray(np.swapaxes(out, -1, -2)), (infos[0], infos[1])
 
     def solve_batch(self, op, sources, solutions, precisions=None):
         """Solve op x_q = b_q for every q, batched; returns one SolveInfo per RHS."""

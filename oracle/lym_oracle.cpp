@@ -939,3 +939,282 @@ int orc_meson_correlators(const orc_op_desc* d, const int* site, double precisio
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- stout smearing
+// StoutSmearing::smearing (source/utils/StoutSmearing.cpp:12-20) with smearLink
+// (:55-62), WilsonGaugeAction::staple (source/actions/WilsonGaugeAction.cpp:9-20)
+// and ExponentialMap::exp (source/utils/ExpMap.h:100-140 for SU(3), :142-155 for
+// SU(2)), on fundamental gauge links V x [4][N x N column-major], N = 2 or 3.
+// The reference divides by zero when the exponent is exactly zero (cold links,
+// rho = 0); the restatement returns the identity there, the limit.
+
+namespace {
+struct MN {
+    int n;
+    cd a[9];
+    cd& operator()(int i, int j) { return a[i + n * j]; }
+    const cd& operator()(int i, int j) const { return a[i + n * j]; }
+};
+MN mn_zero(int n) {
+    MN m;
+    m.n = n;
+    for (int k = 0; k < 9; ++k) m.a[k] = 0;
+    return m;
+}
+MN mn_load(int n, const cd* p) {
+    MN m = mn_zero(n);
+    for (int k = 0; k < n * n; ++k) m.a[k] = p[k];
+    return m;
+}
+MN mn_mul(const MN& A, const MN& B) {
+    MN C = mn_zero(A.n);
+    for (int i = 0; i < A.n; ++i)
+        for (int j = 0; j < A.n; ++j) {
+            cd s = 0;
+            for (int k = 0; k < A.n; ++k) s += A(i, k) * B(k, j);
+            C(i, j) = s;
+        }
+    return C;
+}
+MN mn_dag(const MN& A) {
+    MN C = mn_zero(A.n);
+    for (int i = 0; i < A.n; ++i)
+        for (int j = 0; j < A.n; ++j) C(i, j) = std::conj(A(j, i));
+    return C;
+}
+MN mn_axpy(cd a, const MN& X, const MN& Y) {  // a X + Y
+    MN C = mn_zero(X.n);
+    for (int k = 0; k < X.n * X.n; ++k) C.a[k] = a * X.a[k] + Y.a[k];
+    return C;
+}
+
+// ExponentialMap::exp of an anti-hermitian traceless X
+MN expmap(const MN& X) {
+    const int n = X.n;
+    MN R = mn_zero(n);
+    if (n == 2) {
+        const double a = X(0, 0).imag(), b = X(0, 1).real(), c = X(0, 1).imag();
+        const double rho = std::sqrt(a * a + b * b + c * c);
+        if (rho == 0.0) {
+            R(0, 0) = R(1, 1) = 1.0;
+            return R;
+        }
+        const cd phi(std::cos(rho), std::sin(rho));
+        R(0, 0) = (-(a - rho) * std::conj(phi) + (a + rho) * phi) / (2. * rho);
+        R(0, 1) = (cd(-c, b) * std::conj(phi) + cd(c, -b) * phi) / (2. * rho);
+        R(1, 0) = -std::conj(R(0, 1));
+        R(1, 1) = std::conj(R(0, 0));
+        return R;
+    }
+    // Q = -i X hermitian traceless; c0 = det Q, c1 = tr(Q^2)/2 (Cayley-Hamilton)
+    MN Q = mn_zero(3);
+    for (int k = 0; k < 9; ++k) Q.a[k] = cd(X.a[k].imag(), -X.a[k].real());
+    const cd det = Q(0, 0) * (Q(1, 1) * Q(2, 2) - Q(1, 2) * Q(2, 1)) - Q(0, 1) * (Q(1, 0) * Q(2, 2) - Q(1, 2) * Q(2, 0)) +
+                   Q(0, 2) * (Q(1, 0) * Q(2, 1) - Q(1, 1) * Q(2, 0));
+    const MN X2 = mn_mul(X, X);
+    double c0 = det.real();
+    const double c1 = -0.5 * (X2(0, 0) + X2(1, 1) + X2(2, 2)).real();
+    if (c1 == 0.0) {
+        for (int i = 0; i < 3; ++i) R(i, i) = 1.0;
+        return R;
+    }
+    bool inv = false;
+    if (c0 < 0) {
+        c0 = -c0;
+        inv = true;
+    }
+    const double c0max = (2. * c1 / 3.) * std::sqrt(c1 / 3.);
+    const double theta = std::acos(std::min(1.0, c0 / c0max));
+    const double u = std::sqrt(c1 / 3.) * std::cos(theta / 3.);
+    const double w = std::sqrt(c1) * std::sin(theta / 3.);
+    const cd e2(std::cos(2. * u), std::sin(2. * u));
+    const cd e1(std::cos(u), -std::sin(u));
+    const double xi = std::fabs(w) < 0.05 ? 1. - (1. / 6.) * w * w * (1. - (1. / 20.) * w * w * (1. - (1. / 42.) * w * w))
+                                          : std::sin(w) / w;
+    const cd ii(0., 1.);
+    const cd h0 = (u * u - w * w) * e2 + e1 * (8. * u * u * std::cos(w) + 2. * u * ii * (3. * u * u + w * w) * xi);
+    const cd h1 = 2. * u * e2 - e1 * (2. * u * std::cos(w) - ii * (3. * u * u - w * w) * xi);
+    const cd h2 = e2 - e1 * (std::cos(w) + 3. * u * ii * xi);
+    const double den = 9. * u * u - w * w;
+    cd f0 = h0 / den, f1 = h1 / den, f2 = h2 / den;
+    if (inv) {
+        f0 = std::conj(f0);
+        f1 = -std::conj(f1);
+        f2 = std::conj(f2);
+    }
+    // f0 + f1 Q + f2 Q^2 with Q = -i X, Q^2 = -X^2
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) R(i, j) = -f2 * X2(i, j) + f1 * Q(i, j) + (i == j ? f0 : cd(0, 0));
+    return R;
+}
+
+MN gauge_link(const double* U, int n, long s, int mu) {
+    return mn_load(n, reinterpret_cast<const cd*>(U) + (s * 4 + mu) * n * n);
+}
+}  // namespace
+
+extern "C" {
+
+void orc_expmap(int n, const double* X, double* out) {
+    MN R = expmap(mn_load(n, reinterpret_cast<const cd*>(X)));
+    std::memcpy(out, R.a, sizeof(cd) * n * n);
+}
+
+// one level of stout smearing; staple_out (or NULL) receives the staples
+void orc_stout_smear(const int* dims, int n, double rho, const double* in, double* out, double* staple_out) {
+    const Geo& g = geo(dims);
+#pragma omp parallel for
+    for (long s = 0; s < g.V; ++s)
+        for (int mu = 0; mu < 4; ++mu) {
+            MN st = mn_zero(n);
+            const long smu = g.up[s * 4 + mu];
+            for (int nu = 0; nu < 4; ++nu) {
+                if (nu == mu) continue;
+                const long snu = g.up[s * 4 + nu], sdn = g.dn[s * 4 + nu];
+                const long smudn = g.up[sdn * 4 + mu];
+                // U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
+                MN a = mn_mul(mn_mul(gauge_link(in, n, smu, nu), mn_dag(gauge_link(in, n, snu, mu))),
+                              mn_dag(gauge_link(in, n, s, nu)));
+                // U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
+                MN b = mn_mul(mn_mul(mn_dag(gauge_link(in, n, smudn, nu)), mn_dag(gauge_link(in, n, sdn, mu))),
+                              gauge_link(in, n, sdn, nu));
+                st = mn_axpy(1.0, a, mn_axpy(1.0, b, st));
+            }
+            if (staple_out) std::memcpy(staple_out + (s * 4 + mu) * 2 * n * n, st.a, sizeof(cd) * n * n);
+            const MN Umu = gauge_link(in, n, s, mu);
+            // Omega = rho C U^+, C = staple^+; iStout = (Omega^+ - Omega)/2; X = -(iStout - tr/n)
+            const MN Om = mn_mul(mn_dag(st), mn_dag(Umu));
+            MN X = mn_zero(n);
+            cd tr = 0;
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j) {
+                    X(i, j) = -0.5 * rho * (std::conj(Om(j, i)) - Om(i, j));
+                    if (i == j) tr += X(i, j);
+                }
+            for (int i = 0; i < n; ++i) X(i, i) -= tr / (double)n;
+            const MN E = mn_mul(expmap(X), Umu);
+            std::memcpy(out + (s * 4 + mu) * 2 * n * n, E.a, sizeof(cd) * n * n);
+        }
+}
+
+
+// ---------------------------------------------------------------- fermion force
+// TwoFlavorFermionAction::force (source/actions/TwoFlavorFermionAction.cpp:17-20)
+// = -FermionForce::force (source/hmc_forces/FermionForce.cpp:33-58) of
+// DiracWilsonFermionForce::derivative (source/hmc_forces/DiracWilsonFermionForce.cpp:10-54)
+// for every link.  flinks: the operator's fermion links (boundary sign folded
+// in, V x 4 x 3x3 complex column-major; real adjoint links given as complex);
+// glinks: the gauge links V x 4 x N x N (no boundary sign); X, Y: spinors.
+// rep 0: SU(3) fundamental fermions (N = 3); rep 1: SU(2) with adjoint
+// fermions (N = 2; the adjoint link of the projection is toAdjoint of the
+// gauge link, ConvertLattice.h:24-34).  Generators: lambda_a / 2 for SU(3),
+// sigma_a / 2 for SU(2) and (T^adj_a)_{bc} = -i eps_{abc} for the adjoint
+// (source/utils/LieGenerators.h).
+void orc_wilson_fermion_force(const int* dims, int rep, double kappa, const double* flinks_, const double* glinks_,
+                              const double* X_, const double* Y_, double* force_) {
+    const Geo& g = geo(dims);
+    const cd* W = reinterpret_cast<const cd*>(flinks_);
+    const cd* G = reinterpret_cast<const cd*>(glinks_);
+    const cd* X = reinterpret_cast<const cd*>(X_);
+    const cd* Y = reinterpret_cast<const cd*>(Y_);
+    cd* F = reinterpret_cast<cd*>(force_);
+    const int N = rep == 0 ? 3 : 2;
+    const int NG = N * N - 1;
+    // gamma5 (1 -+ gamma_mu) from the chiral gamma matrices
+    cd g5m[4][4][4], g5p[4][4][4];
+    for (int mu = 0; mu < 4; ++mu)
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) {
+                const double s5 = a < 2 ? 1.0 : -1.0;
+                const cd id = a == b ? cd(1, 0) : cd(0, 0);
+                g5m[mu][a][b] = s5 * (id - GAMMA[mu][a][b]);
+                g5p[mu][a][b] = s5 * (id + GAMMA[mu][a][b]);
+            }
+    // generators of the gauge group (fundamental) and of the fermion representation
+    std::vector<MN> Tg(NG), Tf(NG);
+    if (N == 3) {
+        for (int a = 0; a < 8; ++a) Tg[a] = mn_zero(3);
+        const double r3 = 1.0 / std::sqrt(3.0);
+        Tg[0](0, 1) = Tg[0](1, 0) = 0.5;
+        Tg[1](0, 1) = cd(0, -0.5); Tg[1](1, 0) = cd(0, 0.5);
+        Tg[2](0, 0) = 0.5; Tg[2](1, 1) = -0.5;
+        Tg[3](0, 2) = Tg[3](2, 0) = 0.5;
+        Tg[4](0, 2) = cd(0, -0.5); Tg[4](2, 0) = cd(0, 0.5);
+        Tg[5](1, 2) = Tg[5](2, 1) = 0.5;
+        Tg[6](1, 2) = cd(0, -0.5); Tg[6](2, 1) = cd(0, 0.5);
+        Tg[7](0, 0) = Tg[7](1, 1) = 0.5 * r3; Tg[7](2, 2) = -r3;
+        Tf = Tg;
+    } else {
+        for (int a = 0; a < 3; ++a) {
+            Tg[a] = mn_zero(2);
+            Tf[a] = mn_zero(3);
+        }
+        Tg[0](0, 1) = Tg[0](1, 0) = 0.5;
+        Tg[1](0, 1) = cd(0, -0.5); Tg[1](1, 0) = cd(0, 0.5);
+        Tg[2](0, 0) = 0.5; Tg[2](1, 1) = -0.5;
+        const int eps[3][3][3] = {{{0, 0, 0}, {0, 0, 1}, {0, -1, 0}}, {{0, 0, -1}, {0, 0, 0}, {1, 0, 0}},
+                                  {{0, 1, 0}, {-1, 0, 0}, {0, 0, 0}}};
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                for (int c = 0; c < 3; ++c) Tf[a](b, c) = cd(0, -(double)eps[a][b][c]);
+    }
+#pragma omp parallel for
+    for (long s = 0; s < g.V; ++s)
+        for (int mu = 0; mu < 4; ++mu) {
+            const long sp = g.up[s * 4 + mu];
+            const cd* Wl = W + (s * 4 + mu) * 9;
+            MN D = mn_zero(3);
+            // two orderings: (A, B) = (X, Y) and (Y, X)
+            for (int ord = 0; ord < 2; ++ord) {
+                const cd* A = ord == 0 ? X : Y;
+                const cd* B = ord == 0 ? Y : X;
+                for (int al = 0; al < 4; ++al) {
+                    // from A(x)^+ g5 (1 - g_mu) W B(x+mu): tmp = -kappa sum_b g5m B(x+mu)_b
+                    cd tmp[3] = {0, 0, 0}, tmq[3] = {0, 0, 0};
+                    for (int be = 0; be < 4; ++be) {
+                        for (int c = 0; c < 3; ++c) {
+                            tmp[c] += -kappa * g5m[mu][al][be] * B[sp * 12 + 3 * be + c];
+                            // from A(x+mu)^+ g5 (1 + g_mu) W^+ B(x): tmq = -kappa sum_b g5p W^+ B(x)_b
+                            cd wb = 0;
+                            for (int k = 0; k < 3; ++k) wb += std::conj(Wl[c * 3 + k]) * B[s * 12 + 3 * be + k];
+                            tmq[c] += -kappa * g5p[mu][al][be] * wb;
+                        }
+                    }
+                    // W A(x+mu)_alpha
+                    cd wa[3];
+                    for (int i = 0; i < 3; ++i) {
+                        wa[i] = 0;
+                        for (int k = 0; k < 3; ++k) wa[i] += Wl[i + 3 * k] * A[sp * 12 + 3 * al + k];
+                    }
+                    // tensor(x, y)(i, j) = y_i conj(x_j)
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j)
+                            D(i, j) += tmp[i] * std::conj(A[s * 12 + 3 * al + j]) - tmq[i] * std::conj(wa[j]);
+                }
+            }
+            // projection: result = -i sum_a Im tr(D T^f_a L) T^g_a with L the fermion-representation
+            // link of the gauge link (no boundary sign); the action's force is minus it
+            MN L = mn_zero(3);
+            const cd* Gl = G + (s * 4 + mu) * N * N;
+            if (N == 3) {
+                for (int k = 0; k < 9; ++k) L.a[k] = Gl[k];
+            } else {
+                const MN U2 = mn_load(2, Gl);
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) {
+                        const MN P = mn_mul(mn_mul(mn_mul(Tg[a], U2), Tg[b]), mn_dag(U2));
+                        L(a, b) = 2.0 * (P(0, 0) + P(1, 1)).real();
+                    }
+            }
+            MN R = mn_zero(N);
+            for (int a = 0; a < NG; ++a) {
+                const MN M = mn_mul(mn_mul(D, Tf[a]), L);
+                cd tr = 0;
+                for (int i = 0; i < 3; ++i) tr += M(i, i);
+                R = mn_axpy(cd(0, tr.imag()), Tg[a], R);  // -(-i Im tr) T_a
+            }
+            std::memcpy(F + (s * 4 + mu) * N * N, R.a, sizeof(cd) * N * N);
+        }
+}
+
+}  // extern "C"

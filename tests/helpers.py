@@ -79,6 +79,9 @@ def oracle():
     lib.orc_num_threads.restype = ctypes.c_int
     lib.orc_meson_correlators.argtypes = [ctypes.POINTER(OpDesc), P, ctypes.c_double, ctypes.c_uint, P]
     lib.orc_meson_correlators.restype = ctypes.c_int
+    lib.orc_expmap.argtypes = [ctypes.c_int, P, P]
+    lib.orc_stout_smear.argtypes = [P, ctypes.c_int, ctypes.c_double, P, P, P]
+    lib.orc_wilson_fermion_force.argtypes = [P, ctypes.c_int, ctypes.c_double, P, P, P, P, P]
     _oracle = lib
     return lib
 
@@ -89,6 +92,46 @@ def _p(a):
 
 def _dims(dims):
     return (ctypes.c_int * 4)(*dims)
+
+
+def oracle_expmap(X):
+    """ExponentialMap::exp of one anti-hermitian traceless N x N matrix (oracle)."""
+    n = X.shape[-1]
+    cm = np.ascontiguousarray(X.T, np.complex128)
+    out = np.empty_like(cm)
+    oracle().orc_expmap(n, cm.ctypes.data, out.ctypes.data)
+    return np.ascontiguousarray(out.T)
+
+
+def oracle_stout(dims, U, rho, levels=1, staples=False):
+    """Stout smearing of gauge links [V, 4, N, N] by the oracle; with
+    staples=True also the first level's staple sums [V, 4, N, N]."""
+    n = U.shape[-1]
+    cur = np.ascontiguousarray(np.swapaxes(U, -1, -2), np.complex128)
+    st = np.empty_like(cur) if staples else None
+    for lev in range(levels):
+        out = np.empty_like(cur)
+        oracle().orc_stout_smear(_dims(dims), n, float(rho), cur.ctypes.data, out.ctypes.data,
+                                 st.ctypes.data if (staples and lev == 0) else None)
+        cur = out
+    res = np.ascontiguousarray(np.swapaxes(cur, -1, -2))
+    if staples:
+        return res, np.ascontiguousarray(np.swapaxes(st, -1, -2))
+    return res
+
+
+def oracle_fermion_force(dims, rep, kappa, Wf, Ug, X, Y):
+    """TwoFlavorFermionAction::force of every link by the oracle: fermion links
+    Wf [V, 4, 3, 3] (boundary sign in), gauge links Ug [V, 4, N, N], spinors
+    X, Y [V, 4, 3]; rep 0 SU(3), 1 SU(2) adjoint.  Returns [V, 4, N, N]."""
+    N = Ug.shape[-1]
+    wf = np.ascontiguousarray(np.swapaxes(np.asarray(Wf, complex), -1, -2))
+    ug = np.ascontiguousarray(np.swapaxes(np.asarray(Ug, complex), -1, -2))
+    out = np.empty((Ug.shape[0], 4, N, N), complex)
+    oracle().orc_wilson_fermion_force(_dims(dims), rep, float(kappa), wf.ctypes.data, ug.ctypes.data,
+                                      np.ascontiguousarray(X).ctypes.data, np.ascontiguousarray(Y).ctypes.data,
+                                      out.ctypes.data)
+    return np.ascontiguousarray(np.swapaxes(out, -1, -2))
 
 
 def links_to_colmajor(U):
