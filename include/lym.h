@@ -204,11 +204,13 @@ int lym_solver_get_rhs_block(lym_solver* s, unsigned* nrhs_per_batch, unsigned* 
  * ChiralCondensate.cpp:43-120 -- solve one source at a time in their own
  * loop).  Each request solves D x = b for the clover operator of its lattice
  * and kappa with the even-odd mixed-precision CG driver to
- * |b - D x| <= tolerance |b|; worker threads (one HIP stream each per lattice)
- * take requests in arrival order and batch the queued ones that share lattice
- * and kappa.  Device state is charged against an HBM budget (0: 80% of the
- * free HBM at creation); a worker releases its least recently used lattice
- * state to admit a new one.  The source must stay valid and the solution
+ * |b - D x| <= tolerance |b|.  Continuous batching: a worker thread (one HIP
+ * stream each) runs a batched solve on one lattice whose lanes are requests
+ * of any kappa and tolerance; between chunks of iterations it returns the
+ * requests that have stopped and admits queued requests of that lattice into
+ * free lanes (up to max_batch).  Device state is charged against an HBM
+ * budget (0: 80% of the free HBM at creation); a worker releases its least
+ * recently used lattice state to admit a new one.  The source must stay valid and the solution
  * writable until the request completes.  A result does not depend on the
  * other requests in flight. */
 typedef struct lym_server lym_server;
