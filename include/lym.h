@@ -309,6 +309,16 @@ int lym_temporal_plaquette(const int dims[4], int ncolors, const double* links, 
  * that carry the antiperiodic sign it is the negative of the derivative with
  * respect to the gauge link. */
 int lym_fermion_force(lym_op* op, const lym_spinor* X, const lym_spinor* Y, double* force);
+/* lym_fermion_force without any device allocation: no counterpart in the
+ * reference.  It is for callers that cannot allocate device memory (tight
+ * HMC or serving loops, harnesses that forbid hipMalloc; it synchronises the
+ * context stream, so it is not for use under graph capture).  Same arguments
+ * and the same bits as lym_fermion_force; the
+ * force passes through a fixed 16 MiB device array of the library (one per
+ * GPU, held until the process ends) in chunks of whole sites, so any lattice
+ * size is accepted.  Calls that overlap on one GPU share that array and race:
+ * the caller serialises them. */
+int lym_fermion_force_noalloc(lym_op* op, const lym_spinor* X, const lym_spinor* Y, double* force);
 /* TwoFlavorFermionAction::updateForce (:21-43): Y = Q^-1 phi and X = Q^-1 Y
  * with the solver (LYM_SOLVER_BICGSTAB or LYM_SOLVER_PRECOND_BICGSTAB, its
  * precision and step limit; the reference uses a BiConjugateGradient), then

@@ -121,6 +121,7 @@ SIGNATURES = {
     "lym_temporal_plaquette": (_I, [_PI, _I, _P, _PD]),
     "lym_stout_smear": (_I, [_PI, _I, _D, _I, _P, _P]),
     "lym_fermion_force": (_I, [_P, _P, _P, _P]),
+    "lym_fermion_force_noalloc": (_I, [_P, _P, _P, _P]),
     "lym_two_flavor_force": (_I, [_P, _P, _P, _P, ctypes.POINTER(SolveInfo)]),
 }
 

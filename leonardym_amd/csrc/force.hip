@@ -18,7 +18,9 @@
 // generators (tr T_a T_b = delta_ab / 2),  F = (M - M^+ - tr(M - M^+)/3) / 4
 // with M = L D; for SU(2) adjoint fermions (T^adj_a)_{bc} = -i eps_{abc},
 // F = i sum_a c_a sigma_a / 2 with c_a = -Re sum_{bc} eps_{abc} (L D)_{cb}.
+#include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "../../include/lym.h"
 #include "lym_device.h"
@@ -79,7 +81,8 @@ struct ForceParams {
     const void* links;
     int link_kind;
     double kappa;
-    zd* out;      // lexicographic [V][4][N*N], column-major per link
+    long s0, ns;  // the lexicographic sites [s0, s0 + ns) of this launch
+    zd* out;      // [ns][4][N*N] for those sites, column-major per link
 };
 
 __device__ __forceinline__ void load_sp(const zd* f, const Geom& g, int p, int i, zd s[12]) {
@@ -165,7 +168,7 @@ __device__ void force_link(const ForceParams& a, const int c[4], long lex) {
             cfma(v, W[3 * r + 2], D[6 + s]);
             M[3 * r + s] = v;
         }
-    zd* o = a.out + ((size_t)lex * 4 + MU) * (LK <= 1 ? 9 : 4);
+    zd* o = a.out + ((size_t)(lex - a.s0) * 4 + MU) * (LK <= 1 ? 9 : 4);
     if constexpr (LK <= 1) {
         // F = (A - tr(A)/3) / 4, A = M - M^+
         zd A[9];
@@ -198,10 +201,9 @@ __device__ void force_link(const ForceParams& a, const int c[4], long lex) {
 template <int LK>
 __global__ __launch_bounds__(128) void k_fermion_force(ForceParams a) {
     const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long V = 2L * a.g.Vh;
-    if (j >= 4 * V) return;
-    const int mu = (int)(j / V);
-    const long lex = j - (long)mu * V;
+    if (j >= 4 * a.ns) return;
+    const int mu = (int)(j / a.ns);
+    const long lex = a.s0 + (j - (long)mu * a.ns);
     int c[4];
     {
         long r = lex;
@@ -225,7 +227,12 @@ __global__ __launch_bounds__(128) void k_fermion_force(ForceParams a) {
 
 using lym::zd;
 
-extern "C" int lym_fermion_force(lym_op* op, const lym_spinor* X, const lym_spinor* Y, double* force) {
+namespace {
+
+// checks of lym_fermion_force's arguments; fills everything of the launch
+// parameters but the site range and the output
+int force_params(lym_op* op, const lym_spinor* X, const lym_spinor* Y, const double* force, lym::ForceParams& a,
+                 int& nn) {
     if (!op || !X || !Y || !force) return fail(LYM_ERR_INVALID, "null argument");
     lym_ctx* c = op->ctx;
     if (X->ctx != c || Y->ctx != c) return fail(LYM_ERR_INVALID, "objects of different contexts");
@@ -234,30 +241,117 @@ extern "C" int lym_fermion_force(lym_op* op, const lym_spinor* X, const lym_spin
     if (!op->gauge) return fail(LYM_ERR_INVALID, "operator has no gauge field (call lym_op_set_gauge)");
     if (c->g.L[3] % 2) return fail(LYM_ERR_INVALID, "Lt must be even");
     const int N = c->rep == LYM_REP_SU3_FUND ? 3 : 2;
-    const long V = c->V;
-    const size_t bytes = sizeof(zd) * (size_t)V * 4 * N * N;
-    zd* dev = nullptr;
-    if (hipMalloc((void**)&dev, bytes) != hipSuccess) return fail(LYM_ERR_NOMEM, "hipMalloc(%zu bytes)", bytes);
-    lym::ForceParams a;
+    nn = N * N;
     a.g = c->g;
     a.X = reinterpret_cast<const zd*>(X->d);
     a.Y = reinterpret_cast<const zd*>(Y->d);
     a.links = op->gauge->U;
     a.kappa = op->kappa;
-    a.out = dev;
     const lym_gauge* gg = op->gauge;
     a.link_kind = gg->real_links ? (gg->compression == 6 ? 3 : 2) : (gg->compression == 12 ? 1 : 0);
-    const unsigned grid = (unsigned)((4 * V + 127) / 128);
+    a.s0 = 0;
+    a.ns = 0;
+    a.out = nullptr;
+    return LYM_OK;
+}
+
+void launch_force(const lym::ForceParams& a, hipStream_t st) {
+    const unsigned grid = (unsigned)((4 * a.ns + 127) / 128);
     switch (a.link_kind) {
-        case 0: hipLaunchKernelGGL(lym::k_fermion_force<0>, dim3(grid), dim3(128), 0, c->stream, a); break;
-        case 1: hipLaunchKernelGGL(lym::k_fermion_force<1>, dim3(grid), dim3(128), 0, c->stream, a); break;
-        case 2: hipLaunchKernelGGL(lym::k_fermion_force<2>, dim3(grid), dim3(128), 0, c->stream, a); break;
-        default: hipLaunchKernelGGL(lym::k_fermion_force<3>, dim3(grid), dim3(128), 0, c->stream, a); break;
+        case 0: hipLaunchKernelGGL(lym::k_fermion_force<0>, dim3(grid), dim3(128), 0, st, a); break;
+        case 1: hipLaunchKernelGGL(lym::k_fermion_force<1>, dim3(grid), dim3(128), 0, st, a); break;
+        case 2: hipLaunchKernelGGL(lym::k_fermion_force<2>, dim3(grid), dim3(128), 0, st, a); break;
+        default: hipLaunchKernelGGL(lym::k_fermion_force<3>, dim3(grid), dim3(128), 0, st, a); break;
     }
+}
+
+}  // namespace
+
+extern "C" int lym_fermion_force(lym_op* op, const lym_spinor* X, const lym_spinor* Y, double* force) {
+    lym::ForceParams a;
+    int nn = 0;
+    int rc = force_params(op, X, Y, force, a, nn);
+    if (rc) return rc;
+    lym_ctx* c = op->ctx;
+    const long V = c->V;
+    const size_t bytes = sizeof(zd) * (size_t)V * 4 * nn;
+    zd* dev = nullptr;
+    if (hipMalloc((void**)&dev, bytes) != hipSuccess) return fail(LYM_ERR_NOMEM, "hipMalloc(%zu bytes)", bytes);
+    a.ns = V;
+    a.out = dev;
+    launch_force(a, c->stream);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(force, dev, bytes, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(dev);
+    if (e != hipSuccess) return fail(LYM_ERR_DEVICE, "fermion force: %s", hipGetErrorString(e));
+    return LYM_OK;
+}
+
+// ---------------------------------------------------------------- without allocation
+// The force of lym_fermion_force_noalloc goes through this fixed device array
+// in chunks of whole sites (LYM_FORCE_SCRATCH_BYTES: 16 MiB, 29127 SU(3) or
+// 65536 SU(2) sites per chunk).  Every GPU the library runs on holds its own
+// copy until the process ends.  In host code `scratch` names a host-side
+// stand-in, never a device address: its device address on each GPU comes from
+// hipGetSymbolAddress, once per device, and only that address is passed to
+// kernels and copies.
+#define LYM_FORCE_SCRATCH_BYTES (16u << 20)
+static __device__ __align__(256) zd scratch[LYM_FORCE_SCRATCH_BYTES / sizeof(zd)];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+zd* g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of `scratch` on GPU `dev`, looked up with that GPU current
+int scratch_on(int dev, zd** out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(LYM_ERR_DEVICE, "device %d beyond the %d supported", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e == hipSuccess && cur != dev) e = hipSetDevice(dev);
+        void* p = nullptr;
+        if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur >= 0 && cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            return fail(LYM_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<zd*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return LYM_OK;
+}
+
+}  // namespace
+
+extern "C" int lym_fermion_force_noalloc(lym_op* op, const lym_spinor* X, const lym_spinor* Y, double* force) {
+    lym::ForceParams a;
+    int nn = 0;
+    int rc = force_params(op, X, Y, force, a, nn);
+    if (rc) return rc;
+    lym_ctx* c = op->ctx;
+    zd* buf = nullptr;
+    if ((rc = scratch_on(c->device, &buf))) return rc;
+    const long per_site = 4L * nn;  // complex entries per site
+    const long chunk = (long)(LYM_FORCE_SCRATCH_BYTES / sizeof(zd)) / per_site;
+    a.out = buf;
+    hipError_t e = hipSuccess;
+    for (long s0 = 0; s0 < c->V && e == hipSuccess; s0 += chunk) {
+        a.s0 = s0;
+        a.ns = std::min(chunk, c->V - s0);
+        if (a.ns * per_site * (long)sizeof(zd) > (long)LYM_FORCE_SCRATCH_BYTES)
+            return fail(LYM_ERR_INVALID, "fermion force chunk of %ld sites exceeds the %u-byte scratch", a.ns,
+                        LYM_FORCE_SCRATCH_BYTES);
+        launch_force(a, c->stream);
+        e = hipGetLastError();
+        // stream order keeps the next chunk's kernel behind this copy
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(force + 2 * s0 * per_site, buf, sizeof(zd) * a.ns * per_site, hipMemcpyDeviceToHost,
+                               c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return fail(LYM_ERR_DEVICE, "fermion force: %s", hipGetErrorString(e));
     return LYM_OK;
 }
