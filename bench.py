@@ -156,6 +156,7 @@ def gpu_bench(args):
     log("bench: 12-real links clover apply %.4f ms  %.1f GB/s (18-real effective %.1f)" % (
         ms, res["reconstruct12"]["gbs"], res["reconstruct12"]["gbs_18_real_effective"]))
     del sets12
+    res["full_32x32x32x64"] = big_apply_bench(lym, args)
 
     if not args.no_solve:
         if "solve" in secs:
@@ -164,6 +165,8 @@ def gpu_bench(args):
             res["adjoint"] = adjoint_bench(lym)
         if "batch" in secs:
             res["batch"] = batch_bench(lym)
+        if "batch_apply" in secs:
+            res["batch_apply"] = batch_apply_bench(lym)
         if "serving" in secs:
             if args.serve_full:
                 res["serving"] = serve_bench(lym, [(lam, args.serve_requests) for lam in (1, 2, 5, 10, 20, 50)]
@@ -180,6 +183,31 @@ def gpu_bench(args):
         ctx.synchronize()
         log("calibrate: 20 copies of %d bytes each" % (24 * (V // 2) * 16))
     return res
+
+
+def big_apply_bench(lym, args):
+    """The same FP64 clover full apply on 32^3x64 (2.1 M sites): one set of
+    links, clover blocks and spinors is 3.2 GB, past the Infinity Cache without
+    rotation."""
+    dims = (32, 32, 32, 64)
+    V = int(np.prod(dims))
+    ctx = lym.Context(dims, lym.REP_SU3_FUND, -1)
+    g = lym.Gauge(ctx, fold_bc(dims, lym.gen_su3_links(dims, EPS, GAUGE_SEED)), folded=True,
+                  storage=lym.LINKS_FULL)
+    op = lym.Operator(ctx, lym.OP_CLOVER, 1, KAPPA, CSW, True, gauge=g)
+    src = lym.Spinor(ctx, lym.gaussian_spinor(V, SOURCE_SEED))
+    out = lym.Spinor(ctx)
+    for _ in range(max(3, args.warmup // 4)):
+        op.multiply(out, src)
+    ctx.synchronize()
+    n = max(5, args.steps // 10)
+    ctx.timer_start()
+    for _ in range(n):
+        op.multiply(out, src)
+    ms = ctx.timer_stop() / n
+    gbs = BYTES_PER_SITE_FULL * V / (ms * 1e-3) / 1e9
+    log("bench: 32^3x64 full clover apply %.4f ms  %.1f GB/s (%.3f of 8 TB/s)" % (ms, gbs, gbs / HBM_PEAK_GBS))
+    return {"ms": ms, "gbs": gbs, "frac": gbs / HBM_PEAK_GBS, "steps": n}
 
 
 def true_residual(lym, ctx, g, x, b):
@@ -304,8 +332,9 @@ def adjoint_bench(lym):
 
 
 def batch_bench(lym):
-    """36 RHS on 32^3x64: batched mixed CG (autotuned RHS per batch) against
-    FP64 even-odd CG one RHS at a time (3 of the 36, one per kappa, timed)."""
+    """36 RHS on 32^3x64: batched mixed CG (autotuned RHS per batch; vector
+    stencil, then with the matrix-core form allowed) against FP64 even-odd CG
+    one RHS at a time (all 36 timed)."""
     dims = (32, 32, 32, 64)
     V = int(np.prod(dims))
     kappas = (0.130, 0.132, 0.134)
@@ -318,50 +347,86 @@ def batch_bench(lym):
     sols = [lym.Spinor(ctx) for _ in range(12)]
     Dx = lym.Spinor(ctx)
     eps = 1e-11 ** 2  # |b| = 1 for a point source
-    # FP64, one RHS at a time
-    fp64_t, fp64_it = 0.0, []
+    ops = {k: lym.Operator(ctx, lym.OP_CLOVER, 1, k, CSW, True, gauge=g) for k in kappas}
+    Ds = {k: lym.Operator(ctx, lym.OP_CLOVER, 1, k, CSW, False, gauge=g) for k in kappas}
+    # FP64, one RHS at a time, every one of the 36
     s64 = lym.Solver(ctx, lym.SOLVER_PRECOND_CG, precision=eps, max_steps=20000)
-    for i, k in enumerate(kappas):
-        op = lym.Operator(ctx, lym.OP_CLOVER, 1, k, CSW, True, gauge=g)
-        if i == 0:
-            s64.solve(op, srcs[0], sols[0])  # warm
-        t0 = time.perf_counter()
-        info = s64.solve(op, srcs[4 * i], sols[0])
-        fp64_t += time.perf_counter() - t0
-        fp64_it.append(int(info.steps))
-    fp64_rate = len(kappas) / fp64_t
-    log("bench: 32^3x64 FP64 EO CG one RHS at a time: %.3f s/solve, iterations %s" % (1 / fp64_rate, fp64_it))
-    # batched mixed precision, 12 RHS per kappa
-    sb = lym.Solver(ctx, lym.SOLVER_PRECOND_MIXED_CG, precision=eps, max_steps=4000, sloppy=lym.PREC_HALF)
-    op = lym.Operator(ctx, lym.OP_CLOVER, 1, kappas[0], CSW, True, gauge=g)
-    sb.solve_batch(op, srcs, sols)  # warm: autotune over up to 12 RHS, allocate
-    t0 = time.perf_counter()
-    its, worst, nconv = [], 0.0, 0
+    s64.solve(ops[kappas[0]], srcs[0], sols[0])  # warm
+    fp64_t, fp64_it = 0.0, []
     for k in kappas:
-        op = lym.Operator(ctx, lym.OP_CLOVER, 1, k, CSW, True, gauge=g)
-        D = lym.Operator(ctx, lym.OP_CLOVER, 1, k, CSW, False, gauge=g)
-        infos = sb.solve_batch(op, srcs, sols)
-        ctx.synchronize()
-        log("bench: 32^3x64 batched kappa %.3f: iterations %s, converged %d of 12" % (
-            k, [int(i.steps) for i in infos], sum(int(i.converged) for i in infos)))
-        t_k = time.perf_counter()
         for q in range(12):
-            D.multiply(Dx, sols[q])
-            worst = max(worst, float(np.sqrt(Dx.difference_norm(srcs[q]))))
-        t0 += time.perf_counter() - t_k  # residual checks are not part of the timed work
-        its += [int(i.steps) for i in infos]
-        nconv += sum(int(i.converged) for i in infos)
-    bt = time.perf_counter() - t0
-    nr, thr = sb.rhs_block()
-    rate = 36 / bt
-    log("bench: 32^3x64 batched mixed CG: 36 RHS in %.3f s (%.2f solves/s, %.2fx FP64), RHS/batch %d, "
-        "threads %d, max true residual %.2e" % (bt, rate, rate / fp64_rate, nr, thr, worst))
+            t0 = time.perf_counter()
+            info = s64.solve(ops[k], srcs[q], sols[q])
+            fp64_t += time.perf_counter() - t0
+            fp64_it.append(int(info.steps))
+        log("bench: 32^3x64 FP64 EO CG kappa %.3f, 12 RHS one at a time: %.2f s so far" % (k, fp64_t))
+    fp64_rate = 36 / fp64_t
+    log("bench: 32^3x64 FP64 EO CG one RHS at a time: %.3f s/solve, iterations %d-%d" % (
+        1 / fp64_rate, min(fp64_it), max(fp64_it)))
+
+    def batched(matrix_cores):
+        sb = lym.Solver(ctx, lym.SOLVER_PRECOND_MIXED_CG, precision=eps, max_steps=4000, sloppy=lym.PREC_HALF)
+        sb.set_matrix_cores(matrix_cores)
+        sb.solve_batch(ops[kappas[0]], srcs, sols)  # warm: autotune over up to 12 RHS, allocate
+        t0 = time.perf_counter()
+        its, worst, nconv = [], 0.0, 0
+        for k in kappas:
+            infos = sb.solve_batch(ops[k], srcs, sols)
+            ctx.synchronize()
+            t_k = time.perf_counter()
+            for q in range(12):
+                Ds[k].multiply(Dx, sols[q])
+                worst = max(worst, float(np.sqrt(Dx.difference_norm(srcs[q]))))
+            t0 += time.perf_counter() - t_k  # residual checks are not part of the timed work
+            its += [int(i.steps) for i in infos]
+            nconv += sum(int(i.converged) for i in infos)
+        bt = time.perf_counter() - t0
+        nr, thr = sb.rhs_block()
+        used = sb.matrix_cores()[1]
+        rate = 36 / bt
+        log("bench: 32^3x64 batched mixed CG%s: 36 RHS in %.3f s (%.2f solves/s, %.2fx FP64), RHS/batch %d, "
+            "threads %d, matrix cores used %s, iterations %d-%d, max true residual %.2e" % (
+                " (matrix cores allowed)" if matrix_cores else "", bt, rate, rate / fp64_rate, nr, thr, used,
+                min(its), max(its), worst))
+        return {"solves_per_s": rate, "seconds_36": bt, "rhs_per_batch": nr, "threads": thr,
+                "matrix_cores": bool(used), "iterations_min_max": [min(its), max(its)], "converged": nconv,
+                "max_true_residual": worst, "speedup_vs_fp64": rate / fp64_rate}
+
+    vec = batched(False)
+    mfma = batched(True)
+    best = max(vec["solves_per_s"], mfma["solves_per_s"])
     return {"config": "SU(3) clover 32^3x64, csw 1.0, eps 0.5, 12 point sources at site %s x kappa %s" % (site, kappas),
-            "fp64_one_at_a_time": {"solves_per_s": fp64_rate, "iterations": fp64_it, "sampled_rhs": len(kappas)},
-            "batched_mixed": {"solves_per_s": rate, "seconds_36": bt, "rhs_per_batch": nr, "threads": thr,
-                              "iterations_min_max": [min(its), max(its)], "converged": nconv,
-                              "max_true_residual": worst},
-            "speedup": rate / fp64_rate}
+            "fp64_one_at_a_time": {"solves_per_s": fp64_rate, "seconds_36": fp64_t,
+                                   "iterations_min_max": [min(fp64_it), max(fp64_it)], "timed_rhs": 36},
+            "batched_mixed": vec, "batched_mixed_matrix_cores": mfma,
+            "speedup": best / fp64_rate}
+
+
+def batch_apply_bench(lym, reps=10):
+    """The sloppy FP16 operator of the batched solver on 32^3x64 (even-odd
+    Schur complement, 8 Gaussian RHS per multi-RHS field) in its vector form
+    and its matrix-core form, reps applies each: host time per call (packing
+    and FP16 conversion included; the stencil launches are what a kernel
+    trace of this section separates)."""
+    dims = (32, 32, 32, 64)
+    V = int(np.prod(dims))
+    ctx = lym.Context(dims, lym.REP_SU3_FUND, -1)
+    g = lym.Gauge(ctx, fold_bc(dims, lym.gen_su3_links(dims, EPS, 1003)), folded=True, storage=lym.LINKS_FULL)
+    op = lym.Operator(ctx, lym.OP_EO_CLOVER, 1, 0.13, CSW, True, gauge=g)
+    ins = [lym.Spinor(ctx, lym.gaussian_spinor(V, 3000 + q)) for q in range(8)]
+    outs = [lym.Spinor(ctx) for _ in range(8)]
+    res = {}
+    for name, mc in (("vector", False), ("matrix_cores", True)):
+        op.multiply_sloppy_batch(outs, ins, lym.PREC_HALF, matrix_cores=mc)  # warm
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            op.multiply_sloppy_batch(outs, ins, lym.PREC_HALF, matrix_cores=mc)
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        res[name] = {"ms_per_call_host": ms}
+        log("bench: 32^3x64 batched FP16 Schur apply, 8 RHS, %s form: %.3f ms per call (host)" % (name, ms))
+    return res
 
 
 SERVE_LATTICES = [(8, 8, 8, 8), (12, 12, 12, 12), (16, 16, 16, 32), (24, 24, 24, 48)]
@@ -513,6 +578,7 @@ def main():
                      "frac_of_measured_stream": round(res["gbs"] / res["stream"]["copy_gbs"], 4) if res["stream"] else None},
         "eo_schur": {k: round(v, 4) for k, v in res["eo_schur"].items()},
         "reconstruct12": {k: round(v, 4) for k, v in res["reconstruct12"].items()},
+        "full_32x32x32x64": {k: round(v, 4) for k, v in res["full_32x32x32x64"].items()},
     }
     for k in ("solves", "adjoint", "batch", "serving"):
         if k in res:

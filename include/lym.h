@@ -154,6 +154,13 @@ int lym_op_multiply(lym_op* op, lym_spinor* out, const lym_spinor* in);
  * is FP32, and the result is returned in FP64.  For the even-odd operator the
  * even half is copied through as in lym_op_multiply. */
 int lym_op_multiply_sloppy(lym_op* op, int prec, lym_spinor* out, const lym_spinor* in);
+/* lym_op_multiply_sloppy for nrhs spinors at once in one multi-RHS field
+ * (no counterpart in the reference; the sloppy operator of the batched
+ * solver): nrhs one of 1, 2, 3, 4, 6, 8, 12, 16.  matrix_cores = 1 runs the
+ * FP16 SU(3) stencil's link products on the MFMA units (nrhs 2, 4 or 8;
+ * LYM_ERR_UNSUPPORTED otherwise). */
+int lym_op_multiply_sloppy_batch(lym_op* op, int prec, int matrix_cores, unsigned nrhs, lym_spinor* const* outs,
+                                 const lym_spinor* const* ins);
 /* multiplyAdd (DiracOperator.h:99): out = Op v1 + alpha v2 */
 int lym_op_multiply_add(lym_op* op, lym_spinor* out, const lym_spinor* v1, const lym_spinor* v2, double are,
                         double aim);
@@ -197,6 +204,15 @@ int lym_solver_solve_batch(lym_solver* s, lym_op* op, unsigned nrhs, const lym_s
 int lym_solver_set_rhs_block(lym_solver* s, unsigned nrhs_per_batch);
 /* the batch size and workgroup size in use (threads 0: default) */
 int lym_solver_get_rhs_block(lym_solver* s, unsigned* nrhs_per_batch, unsigned* threads);
+/* The matrix-core form of the batched FP16 stencil (no counterpart in the
+ * reference): with enable = 1, FP16 batches of SU(3) links may run the link
+ * product of every hop on the MFMA units for batch widths 2, 4 and 8 (the
+ * autotuner times it beside the vector form and keeps the faster; with a fixed
+ * RHS block those widths use it).  Default 0.  Results differ from the vector
+ * form by FP32 rounding, so an RHS gives the same bits in batches of those
+ * widths, not across forms.  used: the form the last batched call chose. */
+int lym_solver_set_matrix_cores(lym_solver* s, int enable);
+int lym_solver_get_matrix_cores(const lym_solver* s, int* enabled, int* used);
 
 /* ------------------------------------------------------------------ serving
  * A request loop over device-resident gauge fields (no counterpart in the

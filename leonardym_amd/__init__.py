@@ -98,6 +98,9 @@ SIGNATURES = {
     "lym_solver_solve_batch": (_I, [_P, _P, _U, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_D),
                                     ctypes.POINTER(SolveInfo)]),
     "lym_solver_set_rhs_block": (_I, [_P, _U]),
+    "lym_solver_set_matrix_cores": (_I, [_P, _I]),
+    "lym_solver_get_matrix_cores": (_I, [_P, _PI, _PI]),
+    "lym_op_multiply_sloppy_batch": (_I, [_P, _I, _I, _U, _P, _P]),
     "lym_server_create": (_I, [_U, ctypes.c_size_t, _U, ctypes.POINTER(_P)]),
     "lym_server_destroy": (None, [_P]),
     "lym_server_set_solver": (_I, [_P, _I, _U]),
@@ -460,6 +463,15 @@ class Operator:
         """out = Op inp in storage precision prec (PREC_SINGLE or PREC_HALF)."""
         _check(lib().lym_op_multiply_sloppy(self.h, prec, out.h, inp.h))
 
+    def multiply_sloppy_batch(self, outs, ins, prec, matrix_cores=False):
+        """outs[q] = Op ins[q] for every q in one multi-RHS apply at storage precision prec."""
+        n = len(ins)
+        if len(outs) != n:
+            raise ValueError("outs and ins differ in length")
+        o = (_P * n)(*[x.h for x in outs])
+        i = (_P * n)(*[x.h for x in ins])
+        _check(lib().lym_op_multiply_sloppy_batch(self.h, prec, int(bool(matrix_cores)), n, o, i))
+
     def close(self):
         if self.h:
             lib().lym_op_destroy(self.h)
@@ -527,6 +539,15 @@ class Solver:
         a, b = _U(), _U()
         _check(lib().lym_solver_get_rhs_block(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def set_matrix_cores(self, enable):
+        _check(lib().lym_solver_set_matrix_cores(self.h, int(bool(enable))))
+
+    def matrix_cores(self):
+        """(enabled, used by the last batched call)"""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _check(lib().lym_solver_get_matrix_cores(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return bool(a.value), bool(b.value)
 
     def close(self):
         if self.h:

@@ -368,7 +368,7 @@ static int set_op_data(lym::DslashParams& a, const lym_op* op, int prec, bool ba
 // full lattice, nr right-hand sides, storage precision prec: out = Op in
 int apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* out_sc, const void* in,
                   const float* in_sc, int gamma5, const int* skip, const int* lane_flags, int lane_skip_value,
-                  bool batch_links, double* norm_partials, int norm_max) {
+                  bool batch_links, double* norm_partials, int norm_max, int mfma) {
     lym_ctx* c = op->ctx;
     const size_t po = 12 * (size_t)c->g.stride * nr * selem(prec);
     const size_t so = (size_t)c->g.stride * nr;
@@ -394,6 +394,7 @@ int apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* o
     a.lane_skip_value = lane_skip_value;
     a.norm_partials = norm_partials;
     a.norm_max = norm_max;
+    a.mfma = mfma;
     return lym::launch_dslash(a, lk, op_has_clover(op) ? 2 : 1, false, c->stream);
 }
 
@@ -401,13 +402,14 @@ int apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* o
 int apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, float* out_sc, const void* in_o,
                    const float* in_sc, void* tmp_e, float* tmp_sc, int gamma5, const int* skip,
                    const int* lane_flags, int lane_skip_value, bool batch_links, double* norm_partials,
-                   int norm_max) {
+                   int norm_max, int mfma) {
     lym_ctx* c = op->ctx;
     lym::DslashParams a = base_params(op);
     const int lk = set_op_data(a, op, prec, batch_links);
     a.allow_split = !batch_links;  // batches keep one form for every width (same bits)
     a.nr = nr;
     a.threads = threads;
+    a.mfma = mfma;
     a.skip = skip;
     a.lane_flags = lane_flags;
     a.lane_skip_value = lane_skip_value;
@@ -435,13 +437,13 @@ int apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, float
 int apply_full_sloppy(lym_op* op, int prec, void* out, float* out_sc, const void* in, const float* in_sc, int gamma5,
                       const int* skip, double* norm_partials, int norm_max) {
     return apply_full_nr(op, prec, 1, 0, out, out_sc, in, in_sc, gamma5, skip, nullptr, 0, false, norm_partials,
-                         norm_max);
+                         norm_max, 0);
 }
 
 int apply_schur_sloppy(lym_op* op, int prec, void* out_o, float* out_sc, const void* in_o, const float* in_sc,
                        void* tmp_e, float* tmp_sc, int gamma5, const int* skip, double* norm_partials, int norm_max) {
     return apply_schur_nr(op, prec, 1, 0, out_o, out_sc, in_o, in_sc, tmp_e, tmp_sc, gamma5, skip, nullptr, 0, false,
-                          norm_partials, norm_max);
+                          norm_partials, norm_max, 0);
 }
 
 // the EO driver's source and reconstruction steps on nr-fields (FP64)
@@ -936,6 +938,83 @@ int lym_op_multiply_sloppy(lym_op* op, int prec, lym_spinor* out, const lym_spin
     }
     CHECK_LAUNCH();
     return LYM_OK;
+}
+
+int lym_op_multiply_sloppy_batch(lym_op* op, int prec, int matrix_cores, unsigned nrhs, lym_spinor* const* outs,
+                                 const lym_spinor* const* ins) {
+    int rc = check_op_ready(op);
+    if (rc) return rc;
+    if (prec != LYM_PREC_SINGLE && prec != LYM_PREC_HALF)
+        return fail(LYM_ERR_INVALID, "storage precision must be LYM_PREC_SINGLE or LYM_PREC_HALF");
+    if (!outs || !ins || !lym::batch_nr_supported((int)nrhs))
+        return fail(LYM_ERR_INVALID, "nrhs must be one of 1,2,3,4,6,8,12,16 with output and input arrays");
+    lym_ctx* c = op->ctx;
+    for (unsigned q = 0; q < nrhs; ++q) {
+        if (!outs[q] || !ins[q] || outs[q]->ctx != c || ins[q]->ctx != c) return fail(LYM_ERR_INVALID, "bad spinors");
+        for (unsigned k = 0; k < nrhs; ++k)
+            if ((const lym_spinor*)outs[q] == ins[k]) return fail(LYM_ERR_INVALID, "outputs must differ from inputs");
+    }
+    const int nr = (int)nrhs;
+    if (matrix_cores && !(prec == LYM_PREC_HALF && c->rep == LYM_REP_SU3_FUND && (nr == 2 || nr == 4 || nr == 8)))
+        return fail(LYM_ERR_UNSUPPORTED, "the matrix-core stencil is FP16 SU(3) with 2, 4 or 8 right-hand sides");
+    if ((rc = op_sloppy_data(op, prec, true))) return rc;
+    if (matrix_cores && (op->Usf ? link_kind(op->gauge) : op->sloppy_lk) != 0)
+        return fail(LYM_ERR_UNSUPPORTED, "the matrix-core stencil reads SU(3) links with all three rows");
+    const size_t S = c->g.stride;
+    const size_t es = prec == LYM_PREC_SINGLE ? 8 : 4;
+    std::vector<void*> bufs;
+    auto alloc = [&](void** p, size_t bytes) {
+        if (!rc && !(rc = dmalloc(c, p, bytes))) bufs.push_back(*p);
+    };
+    zd *B = nullptr, *X = nullptr;
+    zd** ptr_dev = nullptr;
+    void *sin = nullptr, *sout = nullptr, *smid = nullptr, *stmp = nullptr;
+    float *sin_sc = nullptr, *sout_sc = nullptr, *smid_sc = nullptr, *stmp_sc = nullptr;
+    alloc((void**)&B, sizeof(zd) * 24 * S * nr);
+    alloc((void**)&X, sizeof(zd) * 24 * S * nr);
+    alloc((void**)&ptr_dev, sizeof(zd*) * nr);
+    for (void** f : {&sin, &sout, &smid, &stmp}) alloc(f, 24 * S * nr * es);
+    for (float** f : {&sin_sc, &sout_sc, &smid_sc, &stmp_sc}) alloc((void**)f, 2 * S * nr * sizeof(float));
+    if (!rc) {
+        std::vector<zd*> hp(nr);
+        for (int q = 0; q < nr; ++q) hp[q] = reinterpret_cast<zd*>(ins[q]->d);
+        HIPCHECK(hipMemcpyAsync(ptr_dev, hp.data(), sizeof(zd*) * nr, hipMemcpyHostToDevice, c->stream));
+        lym::launch_pack(ptr_dev, nr, B, full_n(c), c->stream);
+        lym::launch_spinor_to_sloppy(prec, B, sin, sin_sc, 2 * (long)S * nr, (int)(S * nr), c->stream);
+        const size_t po = 12 * S * nr * es, so = S * nr;  // parity offsets (bytes of the field, scales)
+        auto pf = [&](void* f, int p) { return (void*)((char*)f + p * po); };
+        auto pscl = [&](float* f, int p) { return f + p * so; };
+        void* res = op->power == 2 ? sout : smid;
+        float* res_sc = op->power == 2 ? sout_sc : smid_sc;
+        if (op->kind == LYM_OP_EO_CLOVER) {
+            lym_internal::apply_schur_nr(op, prec, nr, 0, pf(smid, 1), pscl(smid_sc, 1), pf(sin, 1), pscl(sin_sc, 1),
+                                         stmp, stmp_sc, op->gamma5, nullptr, nullptr, 0, true, nullptr, 0, matrix_cores);
+            if (op->power == 2)
+                lym_internal::apply_schur_nr(op, prec, nr, 0, pf(sout, 1), pscl(sout_sc, 1), pf(smid, 1),
+                                             pscl(smid_sc, 1), stmp, stmp_sc, op->gamma5, nullptr, nullptr, 0, true,
+                                             nullptr, 0, matrix_cores);
+            // even sites copied through in FP64, odd sites from the sloppy result
+            lym::launch_spinor_from_sloppy(prec, pf(res, 1), pscl(res_sc, 1), X + 12 * S * nr, (long)S * nr,
+                                           (int)(S * nr), c->stream);
+            HIPCHECK(hipMemcpyAsync(X, B, sizeof(zd) * 12 * S * nr, hipMemcpyDeviceToDevice, c->stream));
+        } else {
+            lym_internal::apply_full_nr(op, prec, nr, 0, smid, smid_sc, sin, sin_sc, op->gamma5, nullptr, nullptr, 0,
+                                        true, nullptr, 0, matrix_cores);
+            if (op->power == 2)
+                lym_internal::apply_full_nr(op, prec, nr, 0, sout, sout_sc, smid, smid_sc, op->gamma5, nullptr,
+                                            nullptr, 0, true, nullptr, 0, matrix_cores);
+            lym::launch_spinor_from_sloppy(prec, res, res_sc, X, 2 * (long)S * nr, (int)(S * nr), c->stream);
+        }
+        for (int q = 0; q < nr; ++q) hp[q] = reinterpret_cast<zd*>(outs[q]->d);
+        HIPCHECK(hipMemcpyAsync(ptr_dev, hp.data(), sizeof(zd*) * nr, hipMemcpyHostToDevice, c->stream));
+        lym::launch_unpack(X, nr, ptr_dev, full_n(c), c->stream);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = fail(LYM_ERR_DEVICE, "batched sloppy apply: %s", hipGetErrorString(e));
+    }
+    (void)hipStreamSynchronize(c->stream);
+    for (void* p : bufs) (void)hipFree(p);
+    return rc;
 }
 
 int lym_op_multiply_add(lym_op* op, lym_spinor* out, const lym_spinor* v1, const lym_spinor* v2, double are,

@@ -215,6 +215,10 @@ __device__ __forceinline__ void hop_pair(const DslashParams& a, int p, int site,
     if constexpr (pin_end) pin(acc);
 }
 
+template <int PR, int DIAG, bool INV, bool NORM, bool LO>
+__device__ __forceinline__ double dslash_epilogue(const DslashParams& a, int p, int site, int r,
+                                                  const cplx<typename Prec<PR>::T> acc[12]);
+
 // NORM (low precision only): returns |v|^2 of the site's output as computed,
 // before rounding to the storage precision; else 0.  A template parameter, so
 // kernels without it compile as if the sum did not exist.  LO: lane r has its
@@ -222,8 +226,6 @@ __device__ __forceinline__ void hop_pair(const DslashParams& a, int p, int site,
 template <int PR, int LK, int DIAG, bool INV, bool NORM = false, bool LO = false>
 __device__ __forceinline__ double dslash_site(const DslashParams& a, int p, int site, int r) {
     typedef typename Prec<PR>::T T;
-    typedef typename Prec<PR>::S S;
-    const int ST = a.g.stride;
     int c[4];
     eo_coords(a.g, p, site, c);
     cplx<T> acc[12];
@@ -240,6 +242,18 @@ __device__ __forceinline__ double dslash_site(const DslashParams& a, int p, int 
         hop_pair<2, PR, LK>(a, p, site, r, c, acc);
         hop_pair<3, PR, LK>(a, p, site, r, c, acc);
     }
+    return dslash_epilogue<PR, DIAG, INV, NORM, LO>(a, p, site, r, acc);
+}
+
+// The site's output from its hopping sum acc: v = diag(psi_x) + hop_sign
+// (-kappa acc), then R^-1, gamma5, the axpy and the store (the value
+// returned is as for dslash_site)
+template <int PR, int DIAG, bool INV, bool NORM, bool LO>
+__device__ __forceinline__ double dslash_epilogue(const DslashParams& a, int p, int site, int r,
+                                                  const cplx<typename Prec<PR>::T> acc[12]) {
+    typedef typename Prec<PR>::T T;
+    typedef typename Prec<PR>::S S;
+    const int ST = a.g.stride;
     const double kap = LO ? a.kappa_dev[r] : (a.kappa_dev ? *a.kappa_dev : a.kappa);
     const T hk = (T)(-kap * a.hop_sign);
     // this lane's clover blocks
@@ -605,8 +619,259 @@ __global__ __launch_bounds__(2 * LYM_DSLASH_THREADS) LYM_OCC void k_dslash_split
     if constexpr (NORM) block_sum_store(n2, a.norm_partials);  // every thread reaches it
 }
 
+// ---------------------------------------------------------------- multi-RHS FP16 stencil on the matrix cores
+// k_dslash_mfma: the hopping term of a batch of nr = 2, 4 or 8 right-hand
+// sides per site (FP16 storage, SU(3) links with all three rows) with the link
+// product on the matrix cores, one v_mfma_f32_16x16x16_f16 per hop and site:
+//   C (16 x 16) = A (16 x 16) . B (16 x 16), FP16 inputs, FP32 products and sums
+//   B column n = 2 r + h (RHS r, spin half h = 0, 1); B rows k 0-5: the
+//     neighbour spinor's spin h, colour k / 2, (re, im); k 6-11: its partner
+//     spin P_h of the direction (negated for h = 1 when c_1 = -c_0); 12-15 zero
+//   A rows m 0-5: the direct output (spin h), colour m / 2, (re, im); m 6-11:
+//     the partner output (spin P_h); 12-15 zero.  A is the real form of the
+//     3x3 blocks  forward: direct [U | c U], partner [c* U | U];
+//                 backward: direct [U+ | -c U+], partner [-c* U+ | U+]
+//     with c = i^C0 of the direction,
+// so the projection (1 -+ gamma_mu), the link multiply and the
+// reconstruction all happen inside the MFMA on the stored FP16 values, with
+// no rounding before the FP32 sums.  The neighbour's FP16 scale (one per site
+// and RHS, i.e. per column), the boundary sign and the sign of partner rows
+// for h = 1 multiply C in FP32.  Lane l holds A row l % 16 and B column
+// l % 16 at k = 4 (l / 16) + 0..3, and C column l % 16, rows 4 (l / 16) +
+// 0..3 (checked on the device).  A wave works through its sites one at a
+// time (the site, its neighbours and the link bases are wave-uniform); the
+// hopping sums go through LDS to one thread per (site, RHS), which runs the
+// epilogue of the VALU stencil (diagonal, R^-1, gamma5, axpy, FP16 store).
+typedef _Float16 hv4 __attribute__((ext_vector_type(4)));
+typedef float fv4 __attribute__((ext_vector_type(4)));
+
+// z i^p on FP16 values (exact: a swap and sign changes)
+__device__ __forceinline__ h2_t mul_ip(h2_t z, int p) {
+    const bool sw = (p & 1) != 0;
+    _Float16 x = sw ? z.y : z.x, y = sw ? z.x : z.y;
+    if (p == 1 || p == 2) x = -x;
+    if (p >= 2) y = -y;
+    h2_t o;
+    o.x = x;
+    o.y = y;
+    return o;
+}
+
+constexpr int MFMA_LDS_STRIDE = 26;  // floats per (site, RHS) slot: 24 used, padded against bank conflicts
+
+// lane constants of the MFMA stencil
+struct MfmaLane {
+    int rr, hh;         // B/C column: RHS and spin half
+    bool bval;          // column rr < nr and k-group < 3: this lane loads B
+    bool aval;          // A row < 12 and k-group < 3: this lane loads A
+    int arow, aore;     // A row: 0 direct, 1 partner; 0 real row, 1 imaginary row
+    bool partP[2];      // A/B k slot s: colour block of the partner spin (else of spin h)
+    unsigned lofs_f[2], lofs_b[2];  // link entry of slot s (in units of the link component stride)
+    int kgrp;           // k-group l / 16
+};
+
+template <int MU>
+__device__ __forceinline__ int mfma_phase(const MfmaLane& L, int s, bool fwd) {
+    constexpr int C0 = Dir<MU>::C0;
+    if (L.arow == 0 && L.partP[s]) return fwd ? C0 : (C0 + 2) & 3;
+    if (L.arow == 1 && !L.partP[s]) return fwd ? (4 - C0) & 3 : (6 - C0) & 3;
+    return 0;
+}
+
+// Operands of one hop (direction MU, forward or backward) of the wave's
+// current site: A (the link block), B (the neighbour spinor's components) and
+// the neighbour's FP16 scale.  Branch-free: lanes without an operand load a
+// valid address and select zero, so the loads of all eight hops of a site can
+// be issued before the first MFMA waits on them.  lsite: the link's site
+// index on parity lp; nb: the neighbour spinor's index on parity q
+template <int MU, bool FWD>
+__device__ __forceinline__ void mfma_load(const DslashParams& a, const MfmaLane& L, int lp, int lsite, int q, int nb,
+                                          const unsigned sofs[2], bool use_b, hv4& Av, hv4& Bv, float& sc) {
+    const int ST = a.g.stride;
+    const h2_t* lb = reinterpret_cast<const h2_t*>(a.links) + (size_t)((lp * 4 + MU) * 9) * ST + lsite;
+    const h2_t* sb = reinterpret_cast<const h2_t*>(a.in[q]) + (size_t)nb * a.nr * 4;
+    constexpr bool SIGN1 = Dir<MU>::C1 != Dir<MU>::C0;  // c_1 = -c_0
+    const _Float16 hz = (_Float16)0;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        h2_t z = fld<h2_t>(lb, L.aval ? (FWD ? L.lofs_f[s] : L.lofs_b[s]) : 0u);
+        if (!FWD) z.y = -z.y;
+        z = mul_ip(z, mfma_phase<MU>(L, s, FWD));
+        Av[2 * s] = L.aval ? (L.aore ? z.y : z.x) : hz;
+        Av[2 * s + 1] = L.aval ? (L.aore ? z.x : (_Float16)(-z.y)) : hz;
+        h2_t v = fld<h2_t>(sb, use_b ? sofs[s] : 0u);
+        if (SIGN1 && L.hh && L.partP[s]) {
+            v.x = -v.x;
+            v.y = -v.y;
+        }
+        Bv[2 * s] = use_b ? v.x : hz;
+        Bv[2 * s + 1] = use_b ? v.y : hz;
+    }
+    const float f = fld<float>(a.in_sc[q] + (size_t)nb * a.nr, use_b ? (unsigned)L.rr : 0u);
+    sc = use_b ? f : 0.f;
+}
+
+// G += f * (A . B) for one hop: f = the neighbour's scale, with the boundary
+// sign and, for partner rows of columns h = 1 where c_1 = -c_0, a minus sign
+template <int MU>
+__device__ __forceinline__ void mfma_acc(const MfmaLane& L, const hv4& Av, const hv4& Bv, float sc, bool flip,
+                                         fv4& G) {
+    constexpr bool SIGN1 = Dir<MU>::C1 != Dir<MU>::C0;
+    const fv4 zero = {0.f, 0.f, 0.f, 0.f};
+    const fv4 Cv = __builtin_amdgcn_mfma_f32_16x16x16f16(Av, Bv, zero, 0, 0, 0);
+    const float f = flip ? -sc : sc;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        // C row 4 k + t is a partner row for k = 1, t >= 2 and for k = 2
+        const bool prow = L.kgrp == 2 || (L.kgrp == 1 && t >= 2);
+        G[t] += ((SIGN1 && L.hh && prow) ? -f : f) * Cv[t];
+    }
+}
+
+template <int DIAG, bool INV, bool LO>
+__global__ __launch_bounds__(LYM_DSLASH_THREADS) void k_dslash_mfma(DslashParams a) {
+    __shared__ float accs[LYM_DSLASH_THREADS * MFMA_LDS_STRIDE];
+    if (a.skip && *a.skip) return;
+    const BlockSweep b = block_sweep(a);
+    const int nr = a.nr, spb = a.spb;
+    const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+    const int spw = 64 / nr;  // sites per wave (nr divides 64)
+    MfmaLane L;
+    {
+        const int n = lane & 15;
+        L.kgrp = lane >> 4;
+        L.rr = n >> 1;
+        L.hh = n & 1;
+        L.bval = L.rr < nr && L.kgrp < 3;
+        L.arow = n < 6 ? 0 : 1;
+        L.aore = n & 1;
+        L.aval = n < 12 && L.kgrp < 3;
+        const int ai = (n % 6) >> 1;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int idx = 2 * L.kgrp + s;  // 0..5: (spin h, colour 0..2), then (partner, colour 0..2)
+            const int j = idx % 3;
+            L.partP[s] = idx >= 3;
+            L.lofs_f[s] = (unsigned)((3 * ai + j) * a.g.stride);
+            L.lofs_b[s] = (unsigned)((3 * j + ai) * a.g.stride);
+        }
+    }
+    // B offsets (h2 elements from the neighbour's base) per direction and slot
+    unsigned sofs[4][2];
+    {
+        const size_t ks = (size_t)a.g.stride * nr;
+        const int P0[4] = {Dir<0>::P0, Dir<1>::P0, Dir<2>::P0, Dir<3>::P0};
+        const int P1[4] = {Dir<0>::P1, Dir<1>::P1, Dir<2>::P1, Dir<3>::P1};
+#pragma unroll
+        for (int mu = 0; mu < 4; ++mu)
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int idx = 2 * L.kgrp + s;
+                const int j = idx % 3;
+                const int spin = idx < 3 ? L.hh : (L.hh ? P1[mu] : P0[mu]);
+                const int comp = 3 * spin + j;  // 16-byte vector comp / 4, slot comp % 4
+                sofs[mu][s] = L.kgrp < 3 ? (unsigned)(((comp >> 2) * ks + L.rr) * 4 + (comp & 3)) : 0u;
+            }
+    }
+    const bool anti = a.g.bc_t < 0;
+    const bool skip_col = L.rr < nr && a.lane_flags && a.lane_flags[L.rr] == a.lane_skip_value;
+    for (int lb = b.lo + b.k; lb < b.hi; lb += b.nphys) {
+        int p, bi;
+        block_of(a, b, lb, p, bi);
+        const int q = p ^ 1;
+        for (int sl = 0; sl < spw; ++sl) {
+            const int ls = wv * spw + sl;
+            const int site = __builtin_amdgcn_readfirstlane(bi * spb + ls);
+            if (ls >= spb || site >= a.g.Vh) break;
+            int c[4];
+            eo_coords(a.g, p, site, c);
+            const bool use_b = L.bval && !skip_col;
+            // neighbour indices (wave-uniform) and the operands of all eight hops
+            hv4 Ah[8], Bh[8];
+            float sch[8];
+            bool fl[8];
+#define LYM_MFMA_LOAD(MU)                                                                                        \
+    {                                                                                                            \
+        int cu[4] = {c[0], c[1], c[2], c[3]}, cd[4] = {c[0], c[1], c[2], c[3]};                                  \
+        cu[MU] = c[MU] + 1 == a.g.L[MU] ? 0 : c[MU] + 1;                                                         \
+        cd[MU] = c[MU] == 0 ? a.g.L[MU] - 1 : c[MU] - 1;                                                         \
+        const int iu = __builtin_amdgcn_readfirstlane(eo_index(a.g, cu[0], cu[1], cu[2], cu[3]));                \
+        const int id = __builtin_amdgcn_readfirstlane(eo_index(a.g, cd[0], cd[1], cd[2], cd[3]));                \
+        mfma_load<MU, true>(a, L, p, site, q, iu, sofs[MU], use_b, Ah[2 * MU], Bh[2 * MU], sch[2 * MU]);         \
+        mfma_load<MU, false>(a, L, q, id, q, id, sofs[MU], use_b, Ah[2 * MU + 1], Bh[2 * MU + 1], sch[2 * MU + 1]); \
+        fl[2 * MU] = MU == 3 && anti && c[3] == a.g.L[3] - 1;                                                    \
+        fl[2 * MU + 1] = MU == 3 && anti && c[3] == 0;                                                           \
+    }
+            LYM_MFMA_LOAD(0)
+            LYM_MFMA_LOAD(1)
+            LYM_MFMA_LOAD(2)
+            LYM_MFMA_LOAD(3)
+#undef LYM_MFMA_LOAD
+            fv4 G0 = {0.f, 0.f, 0.f, 0.f}, G1 = G0;
+            mfma_acc<0>(L, Ah[0], Bh[0], sch[0], fl[0], G0);
+            mfma_acc<0>(L, Ah[1], Bh[1], sch[1], fl[1], G0);
+            mfma_acc<1>(L, Ah[2], Bh[2], sch[2], fl[2], G0);
+            mfma_acc<1>(L, Ah[3], Bh[3], sch[3], fl[3], G0);
+            mfma_acc<2>(L, Ah[4], Bh[4], sch[4], fl[4], G1);
+            mfma_acc<2>(L, Ah[5], Bh[5], sch[5], fl[5], G1);
+            mfma_acc<3>(L, Ah[6], Bh[6], sch[6], fl[6], G1);
+            mfma_acc<3>(L, Ah[7], Bh[7], sch[7], fl[7], G1);
+            // directions 0, 1 send the partner half of column h to spin 3 - h,
+            // directions 2, 3 to spin 2 + h: the lane of column h = 0 collects
+            // spin 2 (its G1, its neighbour's G0), the lane of h = 1 spin 3
+            float other[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) other[t] = __shfl_xor(G0[t], 1, 64);
+            if (L.bval) {
+                float* slot = accs + (ls * nr + L.rr) * MFMA_LDS_STRIDE;
+#pragma unroll
+                for (int t = 0; t < 4; t += 2) {
+                    const int m = 4 * L.kgrp + t;  // C rows m (re), m + 1 (im)
+                    float re, im;
+                    int comp;
+                    if (m < 6) {
+                        re = G0[t] + G1[t];
+                        im = G0[t + 1] + G1[t + 1];
+                        comp = 3 * L.hh + (m >> 1);
+                    } else {
+                        re = G1[t] + other[t];
+                        im = G1[t + 1] + other[t + 1];
+                        comp = 3 * (2 + L.hh) + ((m - 6) >> 1);
+                    }
+                    slot[2 * comp] = re;
+                    slot[2 * comp + 1] = im;
+                }
+            }
+        }
+        __syncthreads();
+        {
+            const int t = (int)threadIdx.x;
+            const int ls = t / nr, r = t - ls * nr;
+            const int site = bi * spb + ls;
+            if (ls < spb && site < a.g.Vh && !(a.lane_flags && a.lane_flags[r] == a.lane_skip_value)) {
+                cplx<float> acc[12];
+                const float* slot = accs + t * MFMA_LDS_STRIDE;
+#pragma unroll
+                for (int k = 0; k < 12; ++k) acc[k] = mk(slot[2 * k], slot[2 * k + 1]);
+                (void)dslash_epilogue<2, DIAG, INV, false, LO>(a, p, site, r, acc);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 template <int PR, int LK, int DIAG, bool INV>
 static void launch_one(const DslashParams& a, int nblocks, hipStream_t st) {
+    if constexpr (PR == 2 && LK == 0) {
+        // matrix-core form: FP16 SU(3) batches of 2, 4 or 8 RHS in whole waves
+        if (a.mfma && (a.nr == 2 || a.nr == 4 || a.nr == 8) && (a.spb * a.nr) % 64 == 0) {
+            if (a.lane_ops)
+                hipLaunchKernelGGL((k_dslash_mfma<DIAG, INV, true>), dim3(nblocks), dim3(a.spb * a.nr), 0, st, a);
+            else
+                hipLaunchKernelGGL((k_dslash_mfma<DIAG, INV, false>), dim3(nblocks), dim3(a.spb * a.nr), 0, st, a);
+            return;
+        }
+    }
     if (a.lane_ops) {  // per-lane operators: the serving engine's batches (never split, no output norms)
         hipLaunchKernelGGL((k_dslash<PR, LK, DIAG, INV, false, true>), dim3(nblocks), dim3(a.spb * a.nr), 0, st, a);
         return;

@@ -42,6 +42,7 @@ struct DslashParams {
     int xslice_blocks;     // set by the launcher: site blocks per x-slice and parity when an x-slice
                            // is a whole number of blocks, else 0
     int allow_split;       // the caller allows the two-threads-per-site form (low precision)
+    int mfma;              // the caller allows the matrix-core form (FP16 SU(3) multi-RHS, nr 2, 4 or 8)
     int split;             // set by the launcher: threads per half of that form, or 0
     double* norm_partials; // optional (FP32/FP16 only): workgroup b writes the sum of |out|^2 over
     int norm_max;          //   its sites to norm_partials[b], when the launch has at most norm_max workgroups

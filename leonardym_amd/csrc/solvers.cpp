@@ -48,14 +48,16 @@ int apply_schur_sloppy(lym_op* op, int prec, void* out_o, float* out_sc, const v
                        int norm_max = 0);
 // batch_links: the batched solver's low-precision links (all rows where the
 // single-RHS copy keeps rows 0, 1)
+// mfma: the matrix-core form of the FP16 stencil where it applies (SU(3)
+// links with all rows, 2, 4 or 8 RHS); elsewhere the VALU form runs
 int apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* out_sc, const void* in,
                   const float* in_sc, int gamma5, const int* skip, const int* lane_flags = nullptr,
                   int lane_skip_value = 0, bool batch_links = false, double* norm_partials = nullptr,
-                  int norm_max = 0);
+                  int norm_max = 0, int mfma = 0);
 int apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, float* out_sc, const void* in_o,
                    const float* in_sc, void* tmp_e, float* tmp_sc, int gamma5, const int* skip,
                    const int* lane_flags = nullptr, int lane_skip_value = 0, bool batch_links = false,
-                   double* norm_partials = nullptr, int norm_max = 0);
+                   double* norm_partials = nullptr, int norm_max = 0, int mfma = 0);
 void eo_source_nr(lym_op* op, int nr, const zd* b_e, const zd* b_o, zd* te, zd* src_o);
 void eo_reconstruct_nr(lym_op* op, int nr, const zd* b_e, const zd* x_o, zd* x_e);
 }  // namespace lym_internal
@@ -82,6 +84,8 @@ struct lym_solver {
     // multi-RHS batches
     int rhs_block;          // RHS per batch (0: autotuned)
     int tuned_nr, tuned_threads;
+    int matrix_cores;       // the batched FP16 stencil may run on the matrix cores
+    int tuned_mfma;         // the form the last batched call chose
     int bnr, bdom, bprec;   // nr, domain parities and sloppy precision the batch buffers hold
     zd* bw[10];             // FP64 nr-fields
     void* bsv[4];           // sloppy nr-fields p, Ap, Q^2 intermediate, Schur intermediate
@@ -616,16 +620,17 @@ struct BatchOp {
     float* mid_sc;
     void* tmp;
     float* tmp_sc;
+    int mfma = 0;  // matrix-core form of the sloppy FP16 stencil
     // lane_flags/lane_skip_value: RHS lanes whose flag equals the value do nothing
     void apply(int prec, void* out, float* out_sc, const void* in, const float* in_sc, const int* skip,
                const int* lane_flags = nullptr, int lane_skip_value = 0) const {
         auto one = [&](void* o, float* osc, const void* i, const float* isc) {
             if (half)
                 lym_internal::apply_schur_nr(op, prec, nr, threads, o, osc, i, isc, tmp, tmp_sc, gamma5, skip,
-                                             lane_flags, lane_skip_value, true);
+                                             lane_flags, lane_skip_value, true, nullptr, 0, prec ? mfma : 0);
             else
                 lym_internal::apply_full_nr(op, prec, nr, threads, o, osc, i, isc, gamma5, skip, lane_flags,
-                                            lane_skip_value, true);
+                                            lane_skip_value, true, nullptr, 0, prec ? mfma : 0);
         };
         if (power == 1) {
             one(out, out_sc, in, in_sc);
@@ -639,7 +644,7 @@ struct BatchOp {
 // Autotuner: RHS per batch and workgroup size, by timing the sloppy Q apply
 // per right-hand side; cached per lattice shape, link kind and precision.
 struct TuneKey {
-    int dims[4], lk, prec, half;
+    int dims[4], lk, prec, half, mfma_allowed;
     bool operator<(const TuneKey& o) const { return std::memcmp(this, &o, sizeof(*this)) < 0; }
 };
 // cached choice: (nr, threads, largest nr the choice was tuned over)
@@ -650,6 +655,7 @@ constexpr int TUNE_NR[N_TUNE] = {4, 6, 8, 12, 16};
 struct TuneVal {
     int maxnr;
     int threads[N_TUNE];
+    int mfma[N_TUNE];  // the matrix-core form was the faster one
     double ms[N_TUNE];
 };
 
@@ -657,7 +663,7 @@ struct TuneVal {
 // batches it makes (full batches of nr, and the last batch of the remainder
 // padded to the next supported width, at the time of the smallest measured
 // width that holds it), among the widths <= maxnr.
-void choose_batch(const TuneVal& tv, unsigned nrhs, int maxnr, int* nr_out, int* thr_out) {
+void choose_batch(const TuneVal& tv, unsigned nrhs, int maxnr, int* nr_out, int* thr_out, int* mfma_out) {
     double best = 1e300;
     for (int i = 0; i < N_TUNE; ++i) {
         const int nr = TUNE_NR[i];
@@ -674,6 +680,7 @@ void choose_batch(const TuneVal& tv, unsigned nrhs, int maxnr, int* nr_out, int*
             best = cost;
             *nr_out = nr;
             *thr_out = tv.threads[i];
+            *mfma_out = tv.mfma[i];
         }
     }
 }
@@ -686,7 +693,10 @@ std::mutex& tune_mutex() {
     return m;
 }
 
-int autotune_batch(lym_solver* s, lym_op* op, bool half, int prec, unsigned nrhs, int* nr_out, int* thr_out) {
+// mfma_ok: the matrix-core form applies to this operator and precision and the
+// solver allows it; it is then timed beside the VALU form for the widths it has
+int autotune_batch(lym_solver* s, lym_op* op, bool half, int prec, unsigned nrhs, bool mfma_ok, int* nr_out,
+                   int* thr_out, int* mfma_out) {
     lym_ctx* c = s->ctx;
     TuneKey key;
     std::memset(&key, 0, sizeof(key));
@@ -694,6 +704,8 @@ int autotune_batch(lym_solver* s, lym_op* op, bool half, int prec, unsigned nrhs
     key.lk = lym_internal::link_kind_of(op->gauge);
     key.prec = prec;
     key.half = half;
+    key.mfma_allowed = mfma_ok;
+    *mfma_out = 0;
     int maxnr = 4;
     for (int k : TUNE_NR)
         if ((unsigned)k <= std::max(nrhs, 4u)) maxnr = k;
@@ -705,7 +717,7 @@ int autotune_batch(lym_solver* s, lym_op* op, bool half, int prec, unsigned nrhs
         if (it != tune_cache().end() && it->second.maxnr >= maxnr) {
             *nr_out = 4;
             *thr_out = 256;
-            choose_batch(it->second, nrhs, maxnr, nr_out, thr_out);
+            choose_batch(it->second, nrhs, maxnr, nr_out, thr_out, mfma_out);
             return LYM_OK;
         }
     }
@@ -717,22 +729,28 @@ int autotune_batch(lym_solver* s, lym_op* op, bool half, int prec, unsigned nrhs
         const int nr = TUNE_NR[i];
         tv.ms[i] = 0;
         tv.threads[i] = 256;
+        tv.mfma[i] = 0;
         if (nr > maxnr) continue;
         double best = 1e300;
-        for (int thr : {64, 128, 256}) {
-            if (thr < nr) continue;
-            BatchOp B{op, half, 1, 1, nr, thr, s->bsv[2], s->bsvsc[2], s->bsv[3], s->bsvsc[3]};
-            B.apply(prec, s->bsv[1], s->bsvsc[1], s->bsv[0], s->bsvsc[0], nullptr);  // warm
-            HIPCHECK(hipEventRecord(c->ev_start, c->stream));
-            for (int k = 0; k < 3; ++k) B.apply(prec, s->bsv[1], s->bsvsc[1], s->bsv[0], s->bsvsc[0], nullptr);
-            HIPCHECK(hipEventRecord(c->ev_stop, c->stream));
-            HIPCHECK(hipEventSynchronize(c->ev_stop));
-            float ms = 0;
-            (void)hipEventElapsedTime(&ms, c->ev_start, c->ev_stop);
-            if (ms / 3.0 < best * 0.98) {
-                best = ms / 3.0;
-                tv.ms[i] = best;
-                tv.threads[i] = thr;
+        for (int form = 0; form < 2; ++form) {
+            if (form == 1 && !(mfma_ok && (nr == 4 || nr == 8))) continue;
+            for (int thr : {64, 128, 256}) {
+                if (thr < nr) continue;
+                BatchOp B{op, half, 1, 1, nr, thr, s->bsv[2], s->bsvsc[2], s->bsv[3], s->bsvsc[3]};
+                B.mfma = form;
+                B.apply(prec, s->bsv[1], s->bsvsc[1], s->bsv[0], s->bsvsc[0], nullptr);  // warm
+                HIPCHECK(hipEventRecord(c->ev_start, c->stream));
+                for (int k = 0; k < 3; ++k) B.apply(prec, s->bsv[1], s->bsvsc[1], s->bsv[0], s->bsvsc[0], nullptr);
+                HIPCHECK(hipEventRecord(c->ev_stop, c->stream));
+                HIPCHECK(hipEventSynchronize(c->ev_stop));
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, c->ev_start, c->ev_stop);
+                if (ms / 3.0 < best * 0.98) {
+                    best = ms / 3.0;
+                    tv.ms[i] = best;
+                    tv.threads[i] = thr;
+                    tv.mfma[i] = form;
+                }
             }
         }
     }
@@ -742,7 +760,7 @@ int autotune_batch(lym_solver* s, lym_op* op, bool half, int prec, unsigned nrhs
     }
     *nr_out = 4;
     *thr_out = 256;
-    choose_batch(tv, nrhs, maxnr, nr_out, thr_out);
+    choose_batch(tv, nrhs, maxnr, nr_out, thr_out, mfma_out);
     return LYM_OK;
 }
 
@@ -753,7 +771,7 @@ int next_supported(int n) {
 }
 
 // one batch of nb <= nr RHS (lanes nb..nr-1 padded with zero sources)
-int run_batch(lym_solver* s, lym_op* op, bool eo, bool half, int nr, int thr, int nb,
+int run_batch(lym_solver* s, lym_op* op, bool eo, bool half, int nr, int thr, int mfma, int nb,
               const lym_spinor* const* sources, lym_spinor* const* solutions, const double* precs,
               lym_solve_info* infos) {
     lym_ctx* c = s->ctx;
@@ -784,6 +802,7 @@ int run_batch(lym_solver* s, lym_op* op, bool eo, bool half, int nr, int thr, in
     const unsigned pw = eo ? 2u : op->power;
     BatchOp H64{op, dom == 1, g5, pw, nr, thr, mid, nullptr, tmp, nullptr};
     BatchOp HS{op, dom == 1, g5, pw, nr, thr, s->bsv[2], s->bsvsc[2], s->bsv[3], s->bsvsc[3]};
+    HS.mfma = mfma;
     // FP64 start: x = b, r = b - A x
     lym::launch_copy(x, b, ndom, st);
     H64.apply(0, Ax, nullptr, x, nullptr, nullptr);
@@ -891,6 +910,19 @@ void forget_op_captures(const lym_op* op) {
 
 extern "C" {
 
+int lym_solver_set_matrix_cores(lym_solver* s, int enable) {
+    if (!s) return fail(LYM_ERR_INVALID, "null solver");
+    s->matrix_cores = enable ? 1 : 0;
+    return LYM_OK;
+}
+
+int lym_solver_get_matrix_cores(const lym_solver* s, int* enabled, int* used) {
+    if (!s) return fail(LYM_ERR_INVALID, "null solver");
+    if (enabled) *enabled = s->matrix_cores;
+    if (used) *used = s->tuned_mfma;
+    return LYM_OK;
+}
+
 int lym_solver_set_rhs_block(lym_solver* s, unsigned nrhs_per_batch) {
     if (!s) return fail(LYM_ERR_INVALID, "null solver");
     if (nrhs_per_batch != 0 && !lym::batch_nr_supported((int)nrhs_per_batch))
@@ -923,19 +955,23 @@ int lym_solver_solve_batch(lym_solver* s, lym_op* op, unsigned nrhs, const lym_s
     int rc = lym_internal::op_ready(op, eo);
     if (!rc) rc = lym_internal::op_sloppy(op, s->sloppy_prec, true);
     if (rc) return rc;
-    int nr = s->rhs_block, thr = 0;
+    // the matrix-core form: FP16 SU(3) with all three link rows in the batch copy
+    const bool mfma_ok = s->matrix_cores && s->sloppy_prec == LYM_PREC_HALF && c->rep == LYM_REP_SU3_FUND &&
+                         op->Usf != nullptr;
+    int nr = s->rhs_block, thr = 0, mfma = mfma_ok ? 1 : 0;
     if (nr == 0) {
-        rc = autotune_batch(s, op, eo || half, s->sloppy_prec, nrhs, &nr, &thr);
+        rc = autotune_batch(s, op, eo || half, s->sloppy_prec, nrhs, mfma_ok, &nr, &thr, &mfma);
         if (rc) return rc;
         s->tuned_nr = nr;
         s->tuned_threads = thr;
     }
+    s->tuned_mfma = mfma;
     HIPCHECK(hipEventRecord(c->ev_start, c->stream));
     for (unsigned q0 = 0; q0 < nrhs; q0 += nr) {
         int nb = (int)std::min<unsigned>(nr, nrhs - q0);
         int nrb = next_supported(nb);
         rc = ensure_batch(s, nrb, (eo || half) ? 1 : 2, s->sloppy_prec);
-        if (!rc) rc = run_batch(s, op, eo, half, nrb, thr, nb, sources + q0, solutions + q0,
+        if (!rc) rc = run_batch(s, op, eo, half, nrb, thr, mfma, nb, sources + q0, solutions + q0,
                                 precisions ? precisions + q0 : nullptr, infos + q0);
         if (rc) return rc;
     }
