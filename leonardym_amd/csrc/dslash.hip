@@ -215,16 +215,18 @@ __device__ __forceinline__ void hop_pair(const DslashParams& a, int p, int site,
     if constexpr (pin_end) pin(acc);
 }
 
-template <int PR, int DIAG, bool INV, bool NORM, bool LO>
+template <int PR, int DIAG, bool INV, bool NORM, bool LO, bool UPD = false>
 __device__ __forceinline__ double dslash_epilogue(const DslashParams& a, int p, int site, int r,
-                                                  const cplx<typename Prec<PR>::T> acc[12]);
+                                                  const cplx<typename Prec<PR>::T> acc[12], float ual = 0.f);
 
 // NORM (low precision only): returns |v|^2 of the site's output as computed,
 // before rounding to the storage precision; else 0.  A template parameter, so
 // kernels without it compile as if the sum did not exist.  LO: lane r has its
-// own kappa and clover blocks (a.lane_ops); also a template parameter.
-template <int PR, int LK, int DIAG, bool INV, bool NORM = false, bool LO = false>
-__device__ __forceinline__ double dslash_site(const DslashParams& a, int p, int site, int r) {
+// own kappa and clover blocks (a.lane_ops); also a template parameter.  UPD:
+// the fused mixed-CG update with step ual (a.upd_*): returns the site's
+// |r_s|^2 after it, and stores no output.
+template <int PR, int LK, int DIAG, bool INV, bool NORM = false, bool LO = false, bool UPD = false>
+__device__ __forceinline__ double dslash_site(const DslashParams& a, int p, int site, int r, float ual = 0.f) {
     typedef typename Prec<PR>::T T;
     int c[4];
     eo_coords(a.g, p, site, c);
@@ -242,15 +244,15 @@ __device__ __forceinline__ double dslash_site(const DslashParams& a, int p, int 
         hop_pair<2, PR, LK>(a, p, site, r, c, acc);
         hop_pair<3, PR, LK>(a, p, site, r, c, acc);
     }
-    return dslash_epilogue<PR, DIAG, INV, NORM, LO>(a, p, site, r, acc);
+    return dslash_epilogue<PR, DIAG, INV, NORM, LO, UPD>(a, p, site, r, acc, ual);
 }
 
 // The site's output from its hopping sum acc: v = diag(psi_x) + hop_sign
 // (-kappa acc), then R^-1, gamma5, the axpy and the store (the value
 // returned is as for dslash_site)
-template <int PR, int DIAG, bool INV, bool NORM, bool LO>
+template <int PR, int DIAG, bool INV, bool NORM, bool LO, bool UPD>
 __device__ __forceinline__ double dslash_epilogue(const DslashParams& a, int p, int site, int r,
-                                                  const cplx<typename Prec<PR>::T> acc[12]) {
+                                                  const cplx<typename Prec<PR>::T> acc[12], float ual) {
     typedef typename Prec<PR>::T T;
     typedef typename Prec<PR>::S S;
     const int ST = a.g.stride;
@@ -323,6 +325,26 @@ __device__ __forceinline__ double dslash_epilogue(const DslashParams& a, int p, 
         const cplx<T> al = mk((T)a.alpha.re, (T)a.alpha.im);
 #pragma unroll
         for (int k = 0; k < 12; ++k) v[k] = v[k] + al * w[k];
+    }
+    if constexpr (UPD) {
+        // x_s += alpha p, r_s -= alpha v (v in FP32, never rounded to storage)
+        static_assert(PR >= 1, "the fused update is the mixed-precision solver's");
+        cplx<T> pv[12];
+        load_spinor<PR>(SpinorRef<PR>{reinterpret_cast<const S*>(a.upd_p[p]), a.upd_p_sc[p]}, ST, 1, site, 0, pv);
+        cplx<float>* xs = reinterpret_cast<cplx<float>*>(a.upd_xs[p]);
+        cplx<float>* rs = reinterpret_cast<cplx<float>*>(a.upd_rs[p]);
+        double n2 = 0;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            cplx<float> x = fld<cplx<float>>(xs + (size_t)k * ST, (unsigned)site);
+            cplx<float> rr = fld<cplx<float>>(rs + (size_t)k * ST, (unsigned)site);
+            x = x + ual * pv[k];
+            rr = rr - ual * v[k];
+            fst<cplx<float>, false>(xs + (size_t)k * ST, (unsigned)site, x);
+            fst<cplx<float>, false>(rs + (size_t)k * ST, (unsigned)site, rr);
+            n2 += (double)rr.re * rr.re + (double)rr.im * rr.im;
+        }
+        return n2;
     }
     store_spinor<PR>(SpinorOut<PR>{reinterpret_cast<S*>(a.out[p]), a.out_sc[p]}, ST, a.nr, site, r, v);
     if constexpr (NORM) {  // the site's 24 squares in FP32, the sum over sites in FP64
@@ -436,14 +458,52 @@ __device__ __forceinline__ void block_of(const DslashParams& a, const BlockSweep
     (void)block_of_at(a, b, lb, (int)blockIdx.x, p, bi);
 }
 
-// NORM: also the per-workgroup output norms (a.norm_partials); LO: per-lane operators
-template <int PR, int LK, int DIAG, bool INV, bool NORM = false, bool LO = false>
+// The fused update's step: alpha = rr_s / (sum of the n <p, Ap> partials),
+// folded by every workgroup in one fixed order (so all get the same alpha);
+// workgroup 0 stores alpha and rr_prev = rr_s in the state.  Every thread of
+// the workgroup must call it.
+__device__ __forceinline__ float upd_alpha(const DslashParams& a) {
+    __shared__ double sh[16];
+    double v = 0;
+    for (int k = (int)threadIdx.x; k < a.upd_ndot; k += (int)blockDim.x) v += a.upd_dot[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sh[w];
+    MCGState* st = reinterpret_cast<MCGState*>(a.upd_state);
+    const double rr = st->rr_s;
+    const double alpha = rr / t;
+    __syncthreads();  // every thread has read rr_s before workgroup 0 moves it to rr_prev
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->alpha = alpha;
+        st->rr_prev = rr;
+    }
+    return (float)alpha;
+}
+
+// NORM: also the per-workgroup output norms (a.norm_partials); LO: per-lane
+// operators; UPD: the fused mixed-CG update instead of the output store
+template <int PR, int LK, int DIAG, bool INV, bool NORM = false, bool LO = false, bool UPD = false>
 __global__ __launch_bounds__(LYM_DSLASH_THREADS) LYM_OCC void k_dslash(DslashParams a) {
     static_assert(!NORM || PR >= 1, "output norms are the low-precision stencil's");
+    static_assert(!(NORM && UPD), "one per-workgroup sum per launch");
     if (a.skip && *a.skip) return;
     const BlockSweep b = block_sweep(a);
     const int ls = (int)threadIdx.x / a.nr, r = (int)threadIdx.x - ls * a.nr;
-    if constexpr (!NORM) {
+    if constexpr (UPD) {
+        // one RHS, no lane flags: every thread holds a site slot and reaches the sums
+        const float ual = upd_alpha(a);
+        double n2 = 0;
+        for (int lb = b.lo + b.k; lb < b.hi; lb += b.nphys) {
+            int p, bi;
+            block_of(a, b, lb, p, bi);
+            const int site = bi * a.spb + (int)threadIdx.x;
+            if (site < a.g.Vh) n2 += dslash_site<PR, LK, DIAG, INV, false, false, true>(a, p, site, 0, ual);
+        }
+        block_sum_store(n2, a.upd_rpart);
+    } else if constexpr (!NORM) {
         if (ls >= a.spb) return;
         if (a.lane_flags && a.lane_flags[r] == a.lane_skip_value) return;
         for (int lb = b.lo + b.k; lb < b.hi; lb += b.nphys) {
@@ -531,9 +591,10 @@ __device__ __forceinline__ void epilogue_half_lp(const DslashParams& a, int p, i
 // blockDim.x = 2 * TH, TH a multiple of 64 (the launcher's thread count), so
 // every wave lies in one half.  Every thread reaches both barriers of every
 // block of the sweep (invalid lanes carry zeros and store nothing).
-template <int PR, int LK, int DIAG, bool INV, bool NORM = false>
+template <int PR, int LK, int DIAG, bool INV, bool NORM = false, bool UPD = false>
 __global__ __launch_bounds__(2 * LYM_DSLASH_THREADS) LYM_OCC void k_dslash_split(DslashParams a) {
     static_assert(PR >= 1, "the split form is the low-precision stencil's");
+    static_assert(!(NORM && UPD), "one per-workgroup sum per launch");
     typedef float T;
     typedef typename Prec<PR>::S S;
     __shared__ cplx<T> xch[2][6][LYM_DSLASH_THREADS];
@@ -547,6 +608,8 @@ __global__ __launch_bounds__(2 * LYM_DSLASH_THREADS) LYM_OCC void k_dslash_split
     const bool lane_ok = ls < a.spb && !(a.lane_flags && a.lane_flags[r] == a.lane_skip_value);
     const size_t ks = (size_t)a.g.stride * a.nr;
     [[maybe_unused]] double n2 = 0;
+    [[maybe_unused]] float ual = 0.f;
+    if constexpr (UPD) ual = upd_alpha(a);
     for (int lb = b.lo + b.k; lb < b.hi; lb += b.nphys) {
         int p, bi;
         block_of(a, b, lb, p, bi);
@@ -595,7 +658,27 @@ __global__ __launch_bounds__(2 * LYM_DSLASH_THREADS) LYM_OCC void k_dslash_split
         }
         const unsigned o = (unsigned)site * (unsigned)a.nr + (unsigned)r;
         S e[6];
-        if constexpr (PR == 2) {
+        if constexpr (UPD) {
+            // x_s += alpha p, r_s -= alpha v on this half's components 6 h + k
+            if (ok) {
+                load_half_lp<PR>(reinterpret_cast<const S*>(a.upd_p[p]), ks, o, h, e);
+                const T psc = PR == 2 ? fld<float>(a.upd_p_sc[p], o) : (T)1;
+                const int ST = a.g.stride;
+                cplx<float>* xs = reinterpret_cast<cplx<float>*>(a.upd_xs[p]) + (size_t)(6 * h) * ST;
+                cplx<float>* rs = reinterpret_cast<cplx<float>*>(a.upd_rs[p]) + (size_t)(6 * h) * ST;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    cplx<float> x = fld<cplx<float>>(xs + (size_t)k * ST, (unsigned)site);
+                    cplx<float> rr = fld<cplx<float>>(rs + (size_t)k * ST, (unsigned)site);
+                    x = x + ual * (psc * to_c<PR>(e[k]));
+                    rr = rr - ual * v[k];
+                    fst<cplx<float>, false>(xs + (size_t)k * ST, (unsigned)site, x);
+                    fst<cplx<float>, false>(rs + (size_t)k * ST, (unsigned)site, rr);
+                    n2 += (double)rr.re * rr.re + (double)rr.im * rr.im;
+                }
+            }
+            __syncthreads();  // every xch read of this block is done before the next block writes it
+        } else if constexpr (PR == 2) {
             float m = 0.f;
 #pragma unroll
             for (int k = 0; k < 6; ++k) m = fmaxf(m, fmaxf(fabsf(v[k].re), fabsf(v[k].im)));
@@ -617,6 +700,7 @@ __global__ __launch_bounds__(2 * LYM_DSLASH_THREADS) LYM_OCC void k_dslash_split
         }
     }
     if constexpr (NORM) block_sum_store(n2, a.norm_partials);  // every thread reaches it
+    if constexpr (UPD) block_sum_store(n2, a.upd_rpart);
 }
 
 // ---------------------------------------------------------------- multi-RHS FP16 stencil on the matrix cores
@@ -876,9 +960,19 @@ static void launch_one(const DslashParams& a, int nblocks, hipStream_t st) {
         hipLaunchKernelGGL((k_dslash<PR, LK, DIAG, INV, false, true>), dim3(nblocks), dim3(a.spb * a.nr), 0, st, a);
         return;
     }
-    // output norms: asked for only on launches with a diagonal and no R^-1
-    // (the last launch of a Schur apply, a full-lattice apply)
+    // the fused mixed-CG update and output norms: asked for only on launches
+    // with a diagonal and no R^-1 (the last launch of a Schur apply, a
+    // full-lattice apply)
     if constexpr (PR >= 1 && DIAG >= 1 && !INV) {
+        if (a.upd_state) {
+            if (a.split)
+                hipLaunchKernelGGL((k_dslash_split<PR, LK, DIAG, INV, false, true>), dim3(nblocks), dim3(2 * a.split),
+                                   0, st, a);
+            else
+                hipLaunchKernelGGL((k_dslash<PR, LK, DIAG, INV, false, false, true>), dim3(nblocks),
+                                   dim3(a.spb * a.nr), 0, st, a);
+            return;
+        }
         if (a.norm_partials) {
             if (a.split)
                 hipLaunchKernelGGL((k_dslash_split<PR, LK, DIAG, INV, true>), dim3(nblocks), dim3(2 * a.split), 0, st,
@@ -953,12 +1047,17 @@ int launch_dslash(DslashParams a, int link_kind, int diag, bool inv, hipStream_t
     if (a.prec == 0 || diag == 0 || inv || a.nr != 1 || a.lane_flags || threads % 64 != 0 ||
         nblocks > a.norm_max)
         a.norm_partials = nullptr;
+    // the fused update: under the same conditions (the caller checks the return
+    // value and runs the separate update when it is 0)
+    if (a.prec == 0 || diag == 0 || inv || a.nr != 1 || a.lane_flags || a.lane_ops || threads % 64 != 0 ||
+        nblocks > a.norm_max || a.norm_partials)
+        a.upd_state = nullptr;
     switch (a.prec) {
         case 0: launch_pr<0>(a, link_kind, diag, inv, nblocks, st); break;
         case 1: launch_pr<1>(a, link_kind, diag, inv, nblocks, st); break;
         default: launch_pr<2>(a, link_kind, diag, inv, nblocks, st); break;
     }
-    return a.norm_partials ? nblocks : 0;
+    return (a.norm_partials || a.upd_state) ? nblocks : 0;
 }
 
 }  // namespace lym

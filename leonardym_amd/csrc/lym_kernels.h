@@ -51,6 +51,37 @@ struct DslashParams {
     // A_sc + r * Asc_lane), Ainv + r * Ainv_lane_bytes (Ainv_sc + r * Ainvsc_lane)
     int lane_ops;
     long A_lane_bytes, Ainv_lane_bytes, Asc_lane, Ainvsc_lane;
+    // Fused update of the mixed-precision CG (single RHS, FP32/FP16 storage; the
+    // last launch of A p): instead of storing its output v = (A p)_x, the
+    // launch updates x_s += alpha p and r_s -= alpha v at every site, with
+    // alpha = rr_s / (the sum of the upd_ndot <p, Ap> partials at upd_dot) and
+    // rr_s from the solver state upd_state (an MCGState); workgroup b writes the
+    // sum of |r_s|^2 over its sites to upd_rpart[b], workgroup 0 stores alpha
+    // and rr_prev = rr_s in the state.  p: storage precision; x_s, r_s: complex
+    // float [12][stride] per parity.
+    void* upd_state;
+    const double* upd_dot;
+    int upd_ndot;
+    const void* upd_p[2];
+    const float* upd_p_sc[2];
+    void* upd_xs[2];
+    void* upd_rs[2];
+    double* upd_rpart;
+};
+
+// Arguments of the fused mixed-CG update (DslashParams::upd_*) for one vector
+// of nparity parities: p in storage precision with its scales, x_s and r_s
+// complex float [nparity][12][stride]; dot: the <p, Ap> partials (count set by
+// the caller once known); rpart: NORM_FOLD_MAX doubles for the |r_s|^2 partials
+struct UpdArgs {
+    void* state;
+    const double* dot;
+    int ndot;
+    const void* p;
+    const float* p_sc;
+    void* xs;
+    void* rs;
+    double* rpart;
 };
 
 // link_kind: 0 complex 18, 1 complex 12, 2 real 9, 3 real 6; diag: 0 none, 1 identity, 2 clover.
@@ -167,9 +198,10 @@ void launch_accum_xs(zd* x, const cplx<float>* xs, long nsites, int stride, hipS
 void launch_rel_resid(MCGState* s, zd* r, cplx<float>* rs, const zd* b, const zd* Ax, long nsites, int stride,
                       double* partials, hipStream_t st);
 // p = r_s + beta p after launch_mcg_update, with beta and the state step taken
-// from the update's |r_s|^2 partials
+// from the update's |r_s|^2 partials (nupd > 0: the nupd partials at upd_part
+// of an update fused into the stencil)
 void launch_mcg_p(int prec, MCGState* s, void* p, float* p_sc, const cplx<float>* rs, long nsites, int stride,
-                  const double* partials, hipStream_t st);
+                  const double* partials, hipStream_t st, const double* upd_part = nullptr, int nupd = 0);
 
 // ---------------------------------------------------------------- multi-RHS mixed CG (batch.hip)
 constexpr int BMAX = 16;

@@ -592,13 +592,21 @@ __global__ __launch_bounds__(RED_BLOCKS) void k_rel_final(MCGState* s, const dou
 // partly updated, which nothing reads).
 template <int PR>
 __global__ __launch_bounds__(MR_THREADS) void k_mcg_p(MCGState* s, SVec<PR> p, const cplx<float>* rs, long nsites,
-                                                      int stride, const double* rpart) {
+                                                      int stride, const double* rpart, int nrpart) {
     __shared__ int stop;
     if (threadIdx.x == 0) stop = s->done;
     __syncthreads();
     if (stop) return;
-    double pl[PART_PER_THREAD];
-    m_partials_load(rpart, pl);
+    // nrpart partials (the fused update's, one per stencil workgroup), else RED_BLOCKS
+    constexpr int NP = NORM_PER_THREAD > PART_PER_THREAD ? NORM_PER_THREAD : PART_PER_THREAD;
+    double pl[NP];
+    if (nrpart > 0) {
+        m_partials_load_n(rpart, nrpart, pl);
+    } else {
+        m_partials_load(rpart, pl);
+#pragma unroll
+        for (int i = PART_PER_THREAD; i < NP; ++i) pl[i] = 0.0;
+    }
     // the first site's vectors load while the partials are folded
     long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long jstep = (long)gridDim.x * blockDim.x;
@@ -607,7 +615,7 @@ __global__ __launch_bounds__(MR_THREADS) void k_mcg_p(MCGState* s, SVec<PR> p, c
         sv_load<PR>(p, stride, j, a);
         f_load(rs, stride, j, r);
     }
-    const double rn = m_fold_block(pl);
+    const double rn = m_fold_block_n(pl);
     const double beta = rn / s->rr_prev;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (rn > s->maxrr) s->maxrr = rn;
@@ -689,14 +697,15 @@ void launch_rel_resid(MCGState* s, zd* r, cplx<float>* rs, const zd* b, const zd
 }
 
 void launch_mcg_p(int prec, MCGState* s, void* p, float* p_sc, const cplx<float>* rs, long nsites, int stride,
-                  const double* partials, hipStream_t st) {
-    const double* rpart = partials + RED_BLOCKS;  // |r_s|^2 partials of launch_mcg_update
+                  const double* partials, hipStream_t st, const double* upd_part, int nupd) {
+    // |r_s|^2 partials: launch_mcg_update's, or nupd of the fused update
+    const double* rpart = nupd > 0 ? upd_part : partials + RED_BLOCKS;
     if (prec == 1)
         hipLaunchKernelGGL(k_mcg_p<1>, dim3(RED_BLOCKS), dim3(MR_THREADS), 0, st, s, sv<1>(p, p_sc), rs, nsites, stride,
-                           rpart);
+                           rpart, nupd);
     else
         hipLaunchKernelGGL(k_mcg_p<2>, dim3(RED_BLOCKS), dim3(MR_THREADS), 0, st, s, sv<2>(p, p_sc), rs, nsites, stride,
-                           rpart);
+                           rpart, nupd);
 }
 
 }  // namespace lym

@@ -41,11 +41,14 @@ int op_sloppy(lym_op* op, int prec, bool full_rows = false);
 // the sloppy applies can also write per-workgroup partial sums of |out|^2
 // (return value: how many; 0 when norm_partials is null or the launch has more
 // than norm_max workgroups)
+// upd: the fused mixed-CG update on the last launch (the return value is then
+// its partial count, 0 when the launch could not take it)
 int apply_full_sloppy(lym_op* op, int prec, void* out, float* out_sc, const void* in, const float* in_sc, int gamma5,
-                      const int* skip, double* norm_partials = nullptr, int norm_max = 0);
+                      const int* skip, double* norm_partials = nullptr, int norm_max = 0,
+                      const lym::UpdArgs* upd = nullptr);
 int apply_schur_sloppy(lym_op* op, int prec, void* out_o, float* out_sc, const void* in_o, const float* in_sc,
                        void* tmp_e, float* tmp_sc, int gamma5, const int* skip, double* norm_partials = nullptr,
-                       int norm_max = 0);
+                       int norm_max = 0, const lym::UpdArgs* upd = nullptr);
 // batch_links: the batched solver's low-precision links (all rows where the
 // single-RHS copy keeps rows 0, 1)
 // mfma: the matrix-core form of the FP16 stencil where it applies (SU(3)
@@ -53,11 +56,12 @@ int apply_schur_sloppy(lym_op* op, int prec, void* out_o, float* out_sc, const v
 int apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* out_sc, const void* in,
                   const float* in_sc, int gamma5, const int* skip, const int* lane_flags = nullptr,
                   int lane_skip_value = 0, bool batch_links = false, double* norm_partials = nullptr,
-                  int norm_max = 0, int mfma = 0);
+                  int norm_max = 0, int mfma = 0, const lym::UpdArgs* upd = nullptr);
 int apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, float* out_sc, const void* in_o,
                    const float* in_sc, void* tmp_e, float* tmp_sc, int gamma5, const int* skip,
                    const int* lane_flags = nullptr, int lane_skip_value = 0, bool batch_links = false,
-                   double* norm_partials = nullptr, int norm_max = 0, int mfma = 0);
+                   double* norm_partials = nullptr, int norm_max = 0, int mfma = 0,
+                   const lym::UpdArgs* upd = nullptr);
 void eo_source_nr(lym_op* op, int nr, const zd* b_e, const zd* b_o, zd* te, zd* src_o);
 void eo_reconstruct_nr(lym_op* op, int nr, const zd* b_e, const zd* x_o, zd* x_e);
 }  // namespace lym_internal
@@ -81,6 +85,7 @@ struct lym_solver {
     float* svsc[4];         // their per-site scales [2][stride]
     lym::cplx<float>* xs;   // FP32 x_s, r_s [2][12][stride]
     lym::cplx<float>* rs;
+    double* upd_part;       // |r_s|^2 partials of the fused update (NORM_FOLD_MAX)
     // multi-RHS batches
     int rhs_block;          // RHS per batch (0: autotuned)
     int tuned_nr, tuned_threads;
@@ -439,6 +444,8 @@ int ensure_mixed_work(lym_solver* s, int prec) {
     }
     if (hipMalloc((void**)&s->xs, 24 * S * 8) != hipSuccess || hipMalloc((void**)&s->rs, 24 * S * 8) != hipSuccess)
         return fail(LYM_ERR_NOMEM, "FP32 vectors");
+    if (!s->upd_part && hipMalloc((void**)&s->upd_part, sizeof(double) * lym::NORM_FOLD_MAX) != hipSuccess)
+        return fail(LYM_ERR_NOMEM, "update partials");
     if (hipMemsetAsync(s->xs, 0, 24 * S * 8, c->stream) != hipSuccess ||
         hipMemsetAsync(s->rs, 0, 24 * S * 8, c->stream) != hipSuccess)
         return fail(LYM_ERR_DEVICE, "hipMemsetAsync");
@@ -460,20 +467,31 @@ struct SloppyOp {
     // out = Op in.  With power 2 and norm_partials set, the first apply also
     // writes per-workgroup partials of |Q in|^2 = <in, Q^2 in> (Q hermitian:
     // gamma5 set); returns how many (0: not asked for, or too many workgroups).
+    // With upd also set (and the norm partials written), the second apply's
+    // last launch runs the fused mixed-CG update instead of storing out;
+    // *nupd receives its partial count (0: not fused, out was written).
     int apply(void* out, float* out_sc, const void* in, const float* in_sc, const int* skip,
-              double* norm_partials = nullptr) const {
-        auto one = [&](void* o, float* osc, const void* i, const float* isc, double* np) {
+              double* norm_partials = nullptr, lym::UpdArgs* upd = nullptr, int* nupd = nullptr) const {
+        auto one = [&](void* o, float* osc, const void* i, const float* isc, double* np, const lym::UpdArgs* u) {
             if (half)
                 return lym_internal::apply_schur_sloppy(op, prec, o, osc, i, isc, tmp_e, tmp_sc, gamma5, skip, np,
-                                                        lym::NORM_FOLD_MAX);
-            return lym_internal::apply_full_sloppy(op, prec, o, osc, i, isc, gamma5, skip, np, lym::NORM_FOLD_MAX);
+                                                        lym::NORM_FOLD_MAX, u);
+            return lym_internal::apply_full_sloppy(op, prec, o, osc, i, isc, gamma5, skip, np, lym::NORM_FOLD_MAX, u);
         };
+        if (nupd) *nupd = 0;
         if (power == 1) {
-            one(out, out_sc, in, in_sc, nullptr);
+            one(out, out_sc, in, in_sc, nullptr, nullptr);
             return 0;
         }
-        const int n = one(mid, mid_sc, in, in_sc, gamma5 ? norm_partials : nullptr);
-        one(out, out_sc, mid, mid_sc, nullptr);
+        const int n = one(mid, mid_sc, in, in_sc, gamma5 ? norm_partials : nullptr, nullptr);
+        if (upd && n > 0) {
+            upd->dot = norm_partials;
+            upd->ndot = n;
+            const int m = one(out, out_sc, mid, mid_sc, nullptr, upd);
+            if (nupd) *nupd = m;
+        } else {
+            one(out, out_sc, mid, mid_sc, nullptr, nullptr);
+        }
         return n;
     }
 };
@@ -514,12 +532,23 @@ int run_mixed_cg(lym_solver* s, const LinOp& L, const SloppyOp& SL, const zd* b,
     // else a separate dot over p and Ap
     static_assert(4 * lym::RED_BLOCKS + lym::NORM_FOLD_MAX <= 8 * lym::RED_BLOCKS, "partials buffer");
     double* norm_part = c->red_partials + 4 * lym::RED_BLOCKS;
+    // the update x_s += alpha p, r_s -= alpha Ap fused into the last stencil
+    // launch of A p (LYM_FUSED_UPDATE=0 keeps the separate update launch)
+    const char* fu_env = std::getenv("LYM_FUSED_UPDATE");
+    const bool fused_upd = fused_norm && !(fu_env && fu_env[0] == '0');
+    key << fused_upd;
+    lym::UpdArgs upd{S, nullptr, 0, p, p_sc, s->xs, s->rs, s->upd_part};
     auto body = [&](unsigned it) {
-        const int nn = SL.apply(Ap, Ap_sc, p, p_sc, &S->done, fused_norm ? norm_part : nullptr);
-        if (nn == 0) lym::launch_mcg_alpha(prec, S, p, p_sc, Ap, Ap_sc, nsites, c->g.stride, c->red_partials, st);
-        lym::launch_mcg_update(prec, S, s->xs, s->rs, p, p_sc, Ap, Ap_sc, nsites, c->g.stride, c->red_partials,
-                               nn ? norm_part : c->red_partials, nn ? nn : lym::RED_BLOCKS, st);
-        lym::launch_mcg_p(prec, S, p, p_sc, s->rs, nsites, c->g.stride, c->red_partials, st);
+        int nupd = 0;
+        const int nn = SL.apply(Ap, Ap_sc, p, p_sc, &S->done, fused_norm ? norm_part : nullptr,
+                                fused_upd ? &upd : nullptr, &nupd);
+        if (nupd == 0) {
+            if (nn == 0) lym::launch_mcg_alpha(prec, S, p, p_sc, Ap, Ap_sc, nsites, c->g.stride, c->red_partials, st);
+            lym::launch_mcg_update(prec, S, s->xs, s->rs, p, p_sc, Ap, Ap_sc, nsites, c->g.stride, c->red_partials,
+                                   nn ? norm_part : c->red_partials, nn ? nn : lym::RED_BLOCKS, st);
+        }
+        lym::launch_mcg_p(prec, S, p, p_sc, s->rs, nsites, c->g.stride, c->red_partials, st,
+                          nupd ? s->upd_part : nullptr, nupd);
         if (rel_slot(it)) {
             lym::launch_rel_accum(S, x, s->xs, nsites, c->g.stride, st);
             L.apply(Ax, x, &S->skip_rel);
@@ -1019,6 +1048,7 @@ void lym_solver_destroy(lym_solver* s) {
     }
     if (s->xs) (void)hipFree(s->xs);
     if (s->rs) (void)hipFree(s->rs);
+    if (s->upd_part) (void)hipFree(s->upd_part);
     if (s->host_state) (void)hipHostFree(s->host_state);
     free_batch(s);
     drop_graphs(s);

@@ -11,7 +11,7 @@ solved on the GPU and checked against the oracle's own solves:
 import numpy as np
 import pytest
 
-from helpers import K_CLOVER, K_WILSON
+from helpers import K_CLOVER, K_WILSON, fold_bc
 
 import leonardym_amd as lym
 from test_gpu_parity import Case, g5
@@ -100,3 +100,70 @@ def test_fused_output_norm_matches_separate_dot(clover16, sloppy, monkeypatch):
         res = np.linalg.norm((b - case.orc.apply(K_CLOVER, x, gamma5=False)).ravel()) / np.linalg.norm(b.ravel())
         assert res <= 1e-10
     assert np.linalg.norm((x1 - x0).ravel()) / np.linalg.norm(x0.ravel()) <= 1e-9
+
+
+@pytest.mark.parametrize("sloppy", [lym.PREC_HALF, lym.PREC_SINGLE], ids=["fp16", "fp32"])
+def test_fused_update_matches_separate_update(clover16, sloppy, monkeypatch):
+    """Mixed CG folds x_s += alpha p, r_s -= alpha Ap and |r_s|^2 into the last
+    stencil launch of Ap (16^3x32: the two-threads-per-site form); the separate
+    update launch (LYM_FUSED_UPDATE=0) solves the same system to the same
+    stopping rule in about as many iterations."""
+    case, b, eps, xo = clover16
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("LYM_FUSED_UPDATE", fused)
+        solver = lym.Solver(case.ctx, lym.SOLVER_PRECOND_MIXED_CG, precision=eps, max_steps=3000, sloppy=sloppy)
+        X = case.spinor()
+        info = solver.solve(case.op(lym.OP_CLOVER), case.spinor(b), X)
+        assert info.converged
+        out[fused] = (info.steps, X.download())
+    (s1, x1), (s0, x0) = out["1"], out["0"]
+    print("fused update: %d iterations, separate update: %d" % (s1, s0))
+    assert abs(int(s1) - int(s0)) <= 3
+    for x in (x1, x0):
+        res = np.linalg.norm((b - case.orc.apply(K_CLOVER, x, gamma5=False)).ravel()) / np.linalg.norm(b.ravel())
+        assert res <= 1e-10
+    assert np.linalg.norm((x1 - x0).ravel()) / np.linalg.norm(x0.ravel()) <= 1e-9
+
+
+@pytest.mark.parametrize("kind", [lym.SOLVER_PRECOND_MIXED_CG, lym.SOLVER_MIXED_CG], ids=["even-odd", "full"])
+def test_fused_update_one_thread_form(kind, monkeypatch):
+    """The fused update in the one-thread-per-site stencil (24^3x32: 221184
+    sites per parity, above the two-thread form's limit), for the even-odd
+    driver (odd-half vectors) and the full-lattice mixed CG on Q^2 (both
+    parities), against the separate update and the FP64 D."""
+    dims = (24, 24, 24, 32)
+    V = int(np.prod(dims))
+    ctx = lym.Context(dims, lym.REP_SU3_FUND, -1)
+    g = lym.Gauge(ctx, fold_bc(dims, lym.gen_su3_links(dims, 0.5, 77)), folded=True)
+    b = lym.gaussian_spinor(V, 78)
+    D = lym.Operator(ctx, lym.OP_CLOVER, 1, 0.12, 1.0, False, gauge=g)
+    if kind == lym.SOLVER_PRECOND_MIXED_CG:
+        op, rhs = lym.Operator(ctx, lym.OP_CLOVER, 1, 0.12, 1.0, True, gauge=g), b
+    else:
+        # CG on Q^2 with right-hand side Q gamma5 b solves D x = b
+        op = lym.Operator(ctx, lym.OP_CLOVER, 2, 0.12, 1.0, True, gauge=g)
+        Q = lym.Operator(ctx, lym.OP_CLOVER, 1, 0.12, 1.0, True, gauge=g)
+        g5b = b.copy()
+        g5b[:, 2:] *= -1
+        t = lym.Spinor(ctx)
+        Q.multiply(t, lym.Spinor(ctx, g5b))
+        rhs = t.download()
+    eps = (1e-11 * np.linalg.norm(rhs)) ** 2
+    out = {}
+    for fused in ("1", "0"):
+        print("fused update %s, kind %d on %s" % (fused, kind, dims), flush=True)
+        monkeypatch.setenv("LYM_FUSED_UPDATE", fused)
+        solver = lym.Solver(ctx, kind, precision=eps, max_steps=4000, sloppy=lym.PREC_HALF)
+        X = lym.Spinor(ctx)
+        info = solver.solve(op, lym.Spinor(ctx, rhs), X)
+        assert info.converged
+        Dx = lym.Spinor(ctx)
+        D.multiply(Dx, X)
+        res = np.sqrt(Dx.difference_norm(lym.Spinor(ctx, b))) / np.linalg.norm(b)
+        assert res <= 1e-10
+        out[fused] = (info.steps, X.download())
+    (s1, x1), (s0, x0) = out["1"], out["0"]
+    print("fused update: %d iterations, separate update: %d" % (s1, s0))
+    assert abs(int(s1) - int(s0)) <= max(3, s0 // 20)
+    assert np.linalg.norm((x1 - x0).ravel()) / np.linalg.norm(x0.ravel()) <= 1e-8
