@@ -170,11 +170,12 @@ def gpu_bench(args):
         if "serving" in secs:
             if args.serve_full:
                 res["serving"] = serve_bench(lym, [(lam, args.serve_requests) for lam in (1, 2, 5, 10, 20, 50)]
-                                             + [(float("inf"), args.serve_requests)], args.serve_workers)
+                                             + [(float("inf"), args.serve_requests)], args.serve_workers,
+                                             args.serve_max_batch)
             else:
                 # the 1-50 req/s sweep with fewer requests at the low rates (about 50 s), then a burst
                 res["serving"] = serve_bench(lym, [(1, 15), (2, 20), (5, 40), (10, 60), (20, 80), (50, 120),
-                                                   (float("inf"), 120)], args.serve_workers)
+                                                   (float("inf"), 120)], args.serve_workers, args.serve_max_batch)
     if args.calibrate:
         # k_copy over a full 16^3x32 spinor: 24 * stride * 16 B read and written
         g, op, src, out, eo = sets[0]
@@ -546,6 +547,7 @@ def main():
                     help="solver benchmarks to run (comma-separated)")
     ap.add_argument("--serve-requests", type=int, default=500, help="requests per rate with --serve-full")
     ap.add_argument("--serve-workers", type=int, default=4, help="server worker threads (streams)")
+    ap.add_argument("--serve-max-batch", type=int, default=4, help="most requests in one running batch")
     ap.add_argument("--serve-full", action="store_true",
                     help="serving at lambda 1,2,5,10,20,50 req/s, --serve-requests each (long)")
     ap.add_argument("--calibrate", action="store_true",
