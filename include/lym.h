@@ -199,6 +199,20 @@ int lym_solver_solve(lym_solver* s, lym_op* op, const lym_spinor* source, lym_sp
  * Starting guess x0 = source.  info[q].seconds is the wall time of the call. */
 int lym_solver_solve_batch(lym_solver* s, lym_op* op, unsigned nrhs, const lym_spinor* const* sources,
                            lym_spinor* const* solutions, const double* precisions, lym_solve_info* infos);
+/* Batched solves whose right-hand sides have their own kappa (no counterpart
+ * in the reference; the several-kappa propagators of MesonCorrelator.cpp:25-99
+ * solved together): solves D(kappas[q]) x_q = b_q with the even-odd
+ * mixed-precision CG driver of a clover operator `op` (power 1; its gauge
+ * field and csw, kappa per RHS) to |r|^2 < precisions[q] on the even-odd
+ * system (NULL: the solver's precision), as lym_solver_solve_batch does.  The
+ * RHS are lanes of one batch (width: the RHS block, 0 = 8), each with its own
+ * kappa, clover blocks and stopping point, in one stencil launch per apply;
+ * an RHS that stops frees its lane for the next one.  A result does not
+ * depend on the other RHS or the batch width.  Solver kind
+ * LYM_SOLVER_PRECOND_MIXED_CG. */
+int lym_solver_solve_batch_kappas(lym_solver* s, lym_op* op, unsigned nrhs, const double* kappas,
+                                  const lym_spinor* const* sources, lym_spinor* const* solutions,
+                                  const double* precisions, lym_solve_info* infos);
 /* RHS per batch: 0 = autotune (timed per lattice shape, link storage and
  * precision on first use), else one of 1, 2, 3, 4, 6, 8, 12, 16 */
 int lym_solver_set_rhs_block(lym_solver* s, unsigned nrhs_per_batch);

@@ -99,6 +99,7 @@ SIGNATURES = {
                                     ctypes.POINTER(SolveInfo)]),
     "lym_solver_set_rhs_block": (_I, [_P, _U]),
     "lym_solver_set_matrix_cores": (_I, [_P, _I]),
+    "lym_solver_solve_batch_kappas": (_I, [_P, _P, _U, _PD, _P, _P, _PD, ctypes.POINTER(SolveInfo)]),
     "lym_solver_get_matrix_cores": (_I, [_P, _PI, _PI]),
     "lym_op_multiply_sloppy_batch": (_I, [_P, _I, _I, _U, _P, _P]),
     "lym_server_create": (_I, [_U, ctypes.c_size_t, _U, ctypes.POINTER(_P)]),
@@ -534,6 +535,23 @@ class Solver:
 
     def set_rhs_block(self, n):
         _check(lib().lym_solver_set_rhs_block(self.h, n))
+
+    def solve_batch_kappas(self, op, kappas, sources, solutions, precisions=None):
+        """Solve D(kappa_q) x_q = b_q for every q as lanes of one batch (even-odd
+        mixed CG of a clover operator); returns one SolveInfo per RHS."""
+        n = len(sources)
+        if len(solutions) != n or len(kappas) != n:
+            raise ValueError("kappas, sources and solutions differ in length")
+        k = np.ascontiguousarray(kappas, dtype=np.float64)
+        srcs = (_P * max(n, 1))(*[s.h for s in sources])
+        sols = (_P * max(n, 1))(*[s.h for s in solutions])
+        infos = (SolveInfo * max(n, 1))()
+        prec = None
+        if precisions is not None:
+            prec = np.ascontiguousarray(precisions, dtype=np.float64)
+        _check(lib().lym_solver_solve_batch_kappas(self.h, op.h, n, k.ctypes.data_as(_PD), srcs, sols,
+                                                   prec.ctypes.data_as(_PD) if prec is not None else None, infos))
+        return [infos[i] for i in range(n)]
 
     def rhs_block(self):
         a, b = _U(), _U()

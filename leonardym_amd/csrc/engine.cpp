@@ -477,7 +477,7 @@ int engine_admit(LaneEngine* e, int n, const LaneRequest* reqs, std::vector<Lane
     for (int i = 0; i < n; ++i) {
         const int q = lane_of[i];
         double b2 = 0;
-        rc = lym_spinor_upload(e->lsrc[q], reqs[i].src);
+        rc = reqs[i].src_dev ? lym_spinor_copy(e->lsrc[q], reqs[i].src_dev) : lym_spinor_upload(e->lsrc[q], reqs[i].src);
         if (!rc) rc = lym_spinor_squared_norm(e->lsrc[q], &b2);
         if (rc) return rc;
         const int orc = slot_operator(e, q, reqs[i].kappa);
@@ -490,7 +490,7 @@ int engine_admit(LaneEngine* e, int n, const LaneRequest* reqs, std::vector<Lane
         }
         if (orc) return orc;
         const double t = 0.1 * reqs[i].tol;
-        eps[i] = t * t * b2;
+        eps[i] = reqs[i].eps > 0 ? reqs[i].eps : t * t * b2;
         e->lanes[q].busy = true;
         e->lanes[q].req = reqs[i];
         e->lanes[q].max_lanes = 0;
@@ -611,7 +611,7 @@ int engine_harvest(LaneEngine* e, std::vector<LaneResult>* done) {
             continue;
         }
         Lane& L = e->lanes[q];
-        int rc = lym_spinor_download(e->lsol[q], L.req.dst);
+        int rc = L.req.dst_dev ? lym_spinor_copy(L.req.dst_dev, e->lsol[q]) : lym_spinor_download(e->lsol[q], L.req.dst);
         if (rc) return rc;
         LaneResult res{};
         res.tag = L.req.tag;

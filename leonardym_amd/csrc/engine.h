@@ -28,6 +28,12 @@ struct LaneRequest {
     double* dst;        // host, written when the lane is harvested
     double kappa, tol;  // solve D x = b to |b - D x| <= tol |b|
     void* tag;
+    // device alternatives (a spinor of the engine's context): when set they
+    // replace src / dst; eps > 0 replaces the tolerance by the absolute stopping
+    // rule |r|^2 < eps on the even-odd system
+    const lym_spinor* src_dev = nullptr;
+    lym_spinor* dst_dev = nullptr;
+    double eps = 0;
 };
 
 struct LaneResult {

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -25,6 +26,7 @@
 #include <string>
 
 #include "../../include/lym.h"
+#include "engine.h"
 #include "lym_kernels.h"
 
 using lym::zd;
@@ -91,6 +93,14 @@ struct lym_solver {
     int tuned_nr, tuned_threads;
     int matrix_cores;       // the batched FP16 stencil may run on the matrix cores
     int tuned_mfma;         // the form the last batched call chose
+    // batches whose right-hand sides have their own kappa (lanes of a serving
+    // engine on this solver's context), kept for the next call with the same
+    // gauge field, csw, precision, width and step limit
+    lym_internal::LaneEngine* eng;
+    const lym_gauge* eng_gauge;
+    double eng_csw;
+    int eng_prec, eng_w;
+    unsigned eng_steps;
     int bnr, bdom, bprec;   // nr, domain parities and sloppy precision the batch buffers hold
     zd* bw[10];             // FP64 nr-fields
     void* bsv[4];           // sloppy nr-fields p, Ap, Q^2 intermediate, Schur intermediate
@@ -939,6 +949,87 @@ void forget_op_captures(const lym_op* op) {
 
 extern "C" {
 
+int lym_solver_solve_batch_kappas(lym_solver* s, lym_op* op, unsigned nrhs, const double* kappas,
+                                  const lym_spinor* const* sources, lym_spinor* const* solutions,
+                                  const double* precisions, lym_solve_info* infos) {
+    if (!s || !op || (nrhs && (!kappas || !sources || !solutions || !infos))) return fail(LYM_ERR_INVALID, "null argument");
+    lym_ctx* c = s->ctx;
+    if (s->kind != LYM_SOLVER_PRECOND_MIXED_CG)
+        return fail(LYM_ERR_UNSUPPORTED, "batches over several kappa need the even-odd mixed-precision CG solver");
+    if (op->ctx != c) return fail(LYM_ERR_INVALID, "operator of another context");
+    if (op->kind != LYM_OP_CLOVER || op->power != 1)
+        return fail(LYM_ERR_UNSUPPORTED, "batches over several kappa run the even-odd driver of a clover operator (power 1)");
+    for (unsigned q = 0; q < nrhs; ++q)
+        if (!sources[q] || !solutions[q] || sources[q]->ctx != c || solutions[q]->ctx != c)
+            return fail(LYM_ERR_INVALID, "spinor %u of another context", q);
+    const int w = s->rhs_block ? s->rhs_block : 8;
+    if (w > lym::BMAX) return fail(LYM_ERR_INVALID, "at most %d right-hand sides per batch", lym::BMAX);
+    int rc = lym_internal::op_ready(op, true);  // the gauge field's clover field strength
+    if (rc) return rc;
+    if (!s->eng || s->eng_gauge != op->gauge || s->eng_csw != op->csw || s->eng_prec != s->sloppy_prec ||
+        s->eng_w != w || s->eng_steps != s->max_steps) {
+        if (s->eng) lym_internal::engine_destroy(s->eng);
+        s->eng = nullptr;
+        rc = lym_internal::engine_create(c, op->gauge, op->csw, s->sloppy_prec, s->max_steps, w, &s->eng);
+        if (rc) return rc;
+        s->eng_gauge = op->gauge;
+        s->eng_csw = op->csw;
+        s->eng_prec = s->sloppy_prec;
+        s->eng_w = w;
+        s->eng_steps = s->max_steps;
+    }
+    lym_internal::LaneEngine* e = s->eng;
+    HIPCHECK(hipEventRecord(c->ev_start, c->stream));
+    // right-hand sides enter free lanes in order and leave when they stop;
+    // every chunk of iterations advances every busy lane
+    unsigned next = 0, finished = 0;
+    const unsigned max_chunks = s->max_steps / 8 + 8;
+    unsigned idle = 0;
+    while (finished < nrhs) {
+        const unsigned n = std::min<unsigned>((unsigned)lym_internal::engine_room(e), nrhs - next);
+        if (n > 0) {
+            std::vector<lym_internal::LaneRequest> reqs(n);
+            for (unsigned i = 0; i < n; ++i) {
+                const unsigned q = next + i;
+                lym_internal::LaneRequest& r = reqs[i];
+                r.src = nullptr;
+                r.dst = nullptr;
+                r.kappa = kappas[q];
+                r.tol = 0;
+                r.tag = reinterpret_cast<void*>((uintptr_t)q);
+                r.src_dev = sources[q];
+                r.dst_dev = solutions[q];
+                r.eps = precisions ? precisions[q] : s->precision;
+            }
+            std::vector<lym_internal::LaneResult> failed;
+            if ((rc = lym_internal::engine_admit(e, (int)n, reqs.data(), &failed))) return rc;
+            for (const auto& f : failed) {
+                const unsigned q = (unsigned)(uintptr_t)f.tag;
+                std::memset(&infos[q], 0, sizeof(lym_solve_info));
+                ++finished;
+            }
+            next += n;
+        }
+        if (lym_internal::engine_busy(e) == 0) continue;
+        if ((rc = lym_internal::engine_run_chunk(e))) return rc;
+        std::vector<lym_internal::LaneResult> done;
+        if ((rc = lym_internal::engine_harvest(e, &done))) return rc;
+        for (const auto& d : done) {
+            const unsigned q = (unsigned)(uintptr_t)d.tag;
+            infos[q] = d.info;
+            ++finished;
+        }
+        idle = done.empty() ? idle + 1 : 0;
+        if (idle > max_chunks) return fail(LYM_ERR_DEVICE, "batched solve made no progress");
+    }
+    HIPCHECK(hipEventRecord(c->ev_stop, c->stream));
+    HIPCHECK(hipEventSynchronize(c->ev_stop));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev_start, c->ev_stop);
+    for (unsigned q = 0; q < nrhs; ++q) infos[q].seconds = ms * 1e-3;
+    return LYM_OK;
+}
+
 int lym_solver_set_matrix_cores(lym_solver* s, int enable) {
     if (!s) return fail(LYM_ERR_INVALID, "null solver");
     s->matrix_cores = enable ? 1 : 0;
@@ -1051,6 +1142,7 @@ void lym_solver_destroy(lym_solver* s) {
     if (s->upd_part) (void)hipFree(s->upd_part);
     if (s->host_state) (void)hipHostFree(s->host_state);
     free_batch(s);
+    if (s->eng) lym_internal::engine_destroy(s->eng);
     drop_graphs(s);
     {
         std::lock_guard<std::mutex> lk(graph_mutex());

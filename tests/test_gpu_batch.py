@@ -178,3 +178,64 @@ def test_batch_after_gauge_storage_change():
         assert isw[q].steps == ifr[q].steps
         assert np.array_equal(xsw[q], xfr[q])
         assert true_residual(full, xsw[q], b) <= 1e-10
+
+
+def test_batch_over_kappas():
+    """Right-hand sides with their own kappa as lanes of one batch
+    (lym_solver_solve_batch_kappas): each meets the FP64 stopping rule at its
+    kappa (the oracle's D), gives the same bits alone, in the mix and at
+    another batch width, and agrees with the per-kappa batched solve."""
+    from helpers import OracleField, fold_bc
+    dims = (8, 8, 8, 8)
+    case = Case(dims=dims)
+    U = fold_bc(dims, lym.gen_su3_links(dims, 0.5, 11), -1)
+    kap = [0.120, 0.125, 0.130, 0.120, 0.125, 0.130, 0.1275]
+    srcs = [case.gauss() for _ in kap]
+    eps = [(1e-11 * np.linalg.norm(b)) ** 2 for b in srcs]
+    op = case.op(lym.OP_CLOVER)
+
+    def run(idx, block=0):
+        solver = lym.Solver(case.ctx, lym.SOLVER_PRECOND_MIXED_CG, max_steps=20000, sloppy=lym.PREC_HALF)
+        solver.set_rhs_block(block)
+        B = [case.spinor(srcs[q]) for q in idx]
+        X = [case.spinor() for _ in idx]
+        infos = solver.solve_batch_kappas(op, [kap[q] for q in idx], B, X, [eps[q] for q in idx])
+        return [x.download() for x in X], infos
+
+    xs, infos = run(list(range(len(kap))))
+    for q, k in enumerate(kap):
+        assert infos[q].converged, q
+        orc = OracleField(dims, U, kappa=k, csw=1.0, need_clover=True, need_inverse=False)
+        Dx = orc.apply(K_CLOVER, xs[q], gamma5=False)
+        res = np.linalg.norm((srcs[q] - Dx).ravel()) / np.linalg.norm(srcs[q].ravel())
+        print("kappa %.4f: %d iterations, residual %.2e" % (k, infos[q].steps, res))
+        assert res <= 1e-10
+    # alone and at another width: the same bits
+    for q in (0, 5):
+        alone, ia = run([q])
+        assert ia[0].steps == infos[q].steps and np.array_equal(alone[0], xs[q]), q
+    xs4, _ = run(list(range(len(kap))), block=4)
+    assert all(np.array_equal(a, b) for a, b in zip(xs4, xs))
+    # the per-kappa batched solve reaches the same solutions
+    for k in (0.120, 0.130):
+        idx = [q for q in range(len(kap)) if kap[q] == k]
+        solver = lym.Solver(case.ctx, lym.SOLVER_PRECOND_MIXED_CG, max_steps=20000, sloppy=lym.PREC_HALF)
+        X = [case.spinor() for _ in idx]
+        inf = solver.solve_batch(case.op(lym.OP_CLOVER, kappa=k), [case.spinor(srcs[q]) for q in idx], X,
+                                 [eps[q] for q in idx])
+        for j, q in enumerate(idx):
+            assert inf[j].converged
+            x = X[j].download()
+            assert np.linalg.norm((x - xs[q]).ravel()) / np.linalg.norm(x.ravel()) < 1e-9
+
+
+def test_batch_over_kappas_refuses_what_it_cannot_run():
+    case = Case(dims=(4, 4, 4, 4))
+    b = [case.spinor(case.gauss())]
+    x = [case.spinor()]
+    s = lym.Solver(case.ctx, lym.SOLVER_MIXED_CG, sloppy=lym.PREC_HALF)
+    with pytest.raises(lym.LymError):
+        s.solve_batch_kappas(case.op(lym.OP_CLOVER), [0.12], b, x)
+    s = lym.Solver(case.ctx, lym.SOLVER_PRECOND_MIXED_CG, sloppy=lym.PREC_HALF)
+    with pytest.raises(lym.LymError):
+        s.solve_batch_kappas(case.op(lym.OP_CLOVER, power=2), [0.12], b, x)
