@@ -440,13 +440,13 @@ __global__ void k_clover_apply_nr(Geom g, const zd* __restrict__ blocks, const z
     for (int k = 0; k < 12; ++k) out[((long)k * g.stride + idx) * nr + r] = o[k];
 }
 
-__global__ void k_clover_apply_lanes(Geom g, const zd* __restrict__ blocks, long lane_elems, const int* mask,
-                                     const zd* __restrict__ in, zd* __restrict__ out, int nr) {
+__global__ void k_clover_apply_lanes(Geom g, const zd* __restrict__ blocks, long lane_elems, const int* lane_set,
+                                     const int* mask, const zd* __restrict__ in, zd* __restrict__ out, int nr) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)g.Vh * nr) return;
     const int idx = (int)(t / nr), r = (int)(t % nr);
     if (!mask[r]) return;
-    const zd* B = blocks + r * lane_elems;
+    const zd* B = blocks + (lane_set ? lane_set[r] : r) * lane_elems;
     zd s[12], o[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) s[k] = in[((long)k * g.stride + idx) * nr + r];
@@ -567,11 +567,11 @@ void launch_relayout(int esize, const void* in, void* out, long rows, int w_old,
             break;
     }
 }
-void launch_clover_apply_lanes(const Geom& g, const zd* blocks, long lane_elems, const int* mask, const zd* in,
-                               zd* out, int nr, hipStream_t st) {
+void launch_clover_apply_lanes(const Geom& g, const zd* blocks, long lane_elems, const int* lane_set, const int* mask,
+                               const zd* in, zd* out, int nr, hipStream_t st) {
     long n = (long)g.Vh * nr;
-    hipLaunchKernelGGL(k_clover_apply_lanes, dim3((n + 255) / 256), dim3(256), 0, st, g, blocks, lane_elems, mask, in,
-                       out, nr);
+    hipLaunchKernelGGL(k_clover_apply_lanes, dim3((n + 255) / 256), dim3(256), 0, st, g, blocks, lane_elems, lane_set,
+                       mask, in, out, nr);
 }
 void launch_clover_apply_nr(const Geom& g, const zd* blocks, const zd* in, zd* out, int nr, hipStream_t st) {
     long n = (long)g.Vh * nr;

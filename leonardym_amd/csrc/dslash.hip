@@ -264,11 +264,13 @@ __device__ __forceinline__ double dslash_epilogue(const DslashParams& a, int p, 
     const float* Asc_ = a.A_sc;
     const float* Aisc_ = a.Ainv_sc;
     if constexpr (LO) {
-        Ab_ = reinterpret_cast<const char*>(a.A) + r * a.A_lane_bytes;
-        Ai_ = reinterpret_cast<const char*>(a.Ainv) + r * a.Ainv_lane_bytes;
+        // lanes of one kappa share a block set
+        const int bs = a.lane_set ? a.lane_set[r] : r;
+        Ab_ = reinterpret_cast<const char*>(a.A) + bs * a.A_lane_bytes;
+        Ai_ = reinterpret_cast<const char*>(a.Ainv) + bs * a.Ainv_lane_bytes;
         if constexpr (PR == 2) {
-            Asc_ = a.A_sc + r * a.Asc_lane;
-            Aisc_ = a.Ainv_sc + r * a.Ainvsc_lane;
+            Asc_ = a.A_sc + bs * a.Asc_lane;
+            Aisc_ = a.Ainv_sc + bs * a.Ainvsc_lane;
         }
     }
     cplx<T> v[12];

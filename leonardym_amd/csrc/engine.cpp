@@ -87,6 +87,16 @@ struct LaneEngine {
     int* mask_h = nullptr;
     int* flag_d = nullptr;      // clover inverse: [0] singular blocks, [1] small pivots
     int* flag_h = nullptr;
+    int* lset_d = nullptr;      // the block set of every lane (read by the stencil)
+    int* lset_h = nullptr;
+    // block set s holds the clover blocks, their even-site inverses and their
+    // low-precision copies for kappa set_kappa[s]; lanes of one kappa share a
+    // set, so a kappa already in a set is not built again
+    double set_kappa[lym::BMAX] = {};
+    bool set_valid[lym::BMAX] = {};
+    int set_refs[lym::BMAX] = {};
+    unsigned long set_used[lym::BMAX] = {};
+    unsigned long set_tick = 0;
     zd** ptr_d = nullptr;
     zd** ptr_h = nullptr;
     std::vector<lym_spinor*> lsrc, lsol;  // single-RHS transfer spinors per lane
@@ -118,6 +128,7 @@ bool full_rows(const lym_gauge* g) {
 void lane_ops(LaneEngine* e, lym::DslashParams& a, int prec) {
     a.prec = prec;
     a.lane_ops = 1;
+    a.lane_set = e->lset_d;
     a.kappa_dev = e->kappa;
     const long S = e->S;
     if (prec == 0) {
@@ -356,11 +367,13 @@ int engine_create(lym_ctx* ctx, const lym_gauge* g, double csw, int prec, unsign
     A((void**)&e->Sd, sizeof(lym::BState));
     A((void**)&e->mask_d, sizeof(int) * lym::BMAX);
     A((void**)&e->flag_d, sizeof(int) * 2);
+    A((void**)&e->lset_d, sizeof(int) * lym::BMAX);
     A((void**)&e->ptr_d, sizeof(zd*) * lym::BMAX);
     if (!rc && (hipHostMalloc((void**)&e->Sh, sizeof(lym::BState), hipHostMallocDefault) != hipSuccess ||
                 hipHostMalloc((void**)&e->mask_h, sizeof(int) * lym::BMAX, hipHostMallocDefault) != hipSuccess ||
                 hipHostMalloc((void**)&e->flag_h, sizeof(int) * 2, hipHostMallocDefault) != hipSuccess ||
-                hipHostMalloc((void**)&e->ptr_h, sizeof(zd*) * lym::BMAX, hipHostMallocDefault) != hipSuccess))
+                hipHostMalloc((void**)&e->ptr_h, sizeof(zd*) * lym::BMAX, hipHostMallocDefault) != hipSuccess ||
+                hipHostMalloc((void**)&e->lset_h, sizeof(int) * lym::BMAX, hipHostMallocDefault) != hipSuccess))
         rc = fail(LYM_ERR_DEVICE, "hipHostMalloc");
     for (int q = 0; !rc && q < wmax; ++q) {
         lym_spinor *a = nullptr, *b = nullptr;
@@ -389,6 +402,10 @@ int engine_create(lym_ctx* ctx, const lym_gauge* g, double csw, int prec, unsign
     if (!rc) {
         std::memset(e->Sh, 0, sizeof(lym::BState));
         for (int q = 0; q < lym::BMAX; ++q) e->Sh->done[q] = 1;
+        for (int q = 0; q < lym::BMAX; ++q) e->lset_h[q] = q < e->wcap ? q : 0;
+        if (hipMemcpyAsync(e->lset_d, e->lset_h, sizeof(int) * lym::BMAX, hipMemcpyHostToDevice, ctx->stream) !=
+            hipSuccess)
+            rc = fail(LYM_ERR_DEVICE, "lane block sets");
         e->Sh->all_done = 1;
         e->Sh->skip_rel = 1;
         e->Sh->max_steps = (int)max_steps;
@@ -415,6 +432,7 @@ void engine_destroy(LaneEngine* e) {
     if (e->mask_h) (void)hipHostFree(e->mask_h);
     if (e->flag_h) (void)hipHostFree(e->flag_h);
     if (e->ptr_h) (void)hipHostFree(e->ptr_h);
+    if (e->lset_h) (void)hipHostFree(e->lset_h);
     delete e;
 }
 
@@ -426,15 +444,14 @@ int engine_busy(const LaneEngine* e) {
 
 int engine_room(const LaneEngine* e) { return e->wmax - engine_busy(e); }
 
-// kappa_q, A_q = 1 + kappa_q csw gamma5 B, its even-site inverse and the
-// low-precision copies, for slot q
-static int slot_operator(LaneEngine* e, int q, double kappa) {
+// block set q: A = 1 + kappa csw gamma5 B, its even-site inverse and the
+// low-precision copies
+static int build_set(LaneEngine* e, int q, double kappa) {
     lym_ctx* c = e->ctx;
     hipStream_t st = c->stream;
     const long S = e->S;
     zd* Aq = e->A + 72 * S * q;
     zd* Iq = e->Ainv + 36 * S * q;
-    lym::launch_set_double(e->kappa + q, kappa, st);
     lym::launch_clover_A(c->g, reinterpret_cast<const zd*>(e->g->B), kappa * e->csw, Aq, st);
     // a unit block for every lane stays exact at csw = 0 (B is then ignored)
     ECHECK(hipMemsetAsync(e->flag_d, 0, 2 * sizeof(int), st));
@@ -480,15 +497,36 @@ int engine_admit(LaneEngine* e, int n, const LaneRequest* reqs, std::vector<Lane
         rc = reqs[i].src_dev ? lym_spinor_copy(e->lsrc[q], reqs[i].src_dev) : lym_spinor_upload(e->lsrc[q], reqs[i].src);
         if (!rc) rc = lym_spinor_squared_norm(e->lsrc[q], &b2);
         if (rc) return rc;
-        const int orc = slot_operator(e, q, reqs[i].kappa);
-        if (orc == LYM_ERR_SINGULAR) {
-            LaneResult f{};
-            f.tag = reqs[i].tag;
-            f.error = orc;
-            failed->push_back(f);
-            continue;
+        // the lane's kappa, and its block set: one already built for this
+        // kappa, else a set no lane uses (an empty one first, else the least
+        // recently used); there are wcap sets and at most wmax <= wcap lanes
+        lym::launch_set_double(e->kappa + q, reqs[i].kappa, st);
+        int bs = -1;
+        for (int s = 0; s < e->wcap && bs < 0; ++s)
+            if (e->set_valid[s] && e->set_kappa[s] == reqs[i].kappa) bs = s;
+        if (bs < 0) {
+            for (int s = 0; s < e->wcap && bs < 0; ++s)
+                if (!e->set_valid[s] && e->set_refs[s] == 0) bs = s;
+            for (int s = 0; s < e->wcap; ++s)
+                if (e->set_refs[s] == 0 && (bs < 0 || (e->set_valid[bs] && e->set_used[s] < e->set_used[bs])))
+                    bs = s;
+            if (bs < 0) return fail(LYM_ERR_DEVICE, "serving engine: no free block set");
+            e->set_valid[bs] = false;
+            const int orc = build_set(e, bs, reqs[i].kappa);
+            if (orc == LYM_ERR_SINGULAR) {
+                LaneResult f{};
+                f.tag = reqs[i].tag;
+                f.error = orc;
+                failed->push_back(f);
+                continue;
+            }
+            if (orc) return orc;
+            e->set_valid[bs] = true;
+            e->set_kappa[bs] = reqs[i].kappa;
         }
-        if (orc) return orc;
+        ++e->set_refs[bs];
+        e->set_used[bs] = ++e->set_tick;
+        e->lset_h[q] = bs;
         const double t = 0.1 * reqs[i].tol;
         eps[i] = reqs[i].eps > 0 ? reqs[i].eps : t * t * b2;
         e->lanes[q].busy = true;
@@ -504,6 +542,7 @@ int engine_admit(LaneEngine* e, int n, const LaneRequest* reqs, std::vector<Lane
     rc = write_state(e);
     if (rc) return rc;
     ECHECK(hipMemcpyAsync(e->mask_d, e->mask_h, sizeof(int) * lym::BMAX, hipMemcpyHostToDevice, st));
+    ECHECK(hipMemcpyAsync(e->lset_d, e->lset_h, sizeof(int) * lym::BMAX, hipMemcpyHostToDevice, st));
     // pack the admitted sources into their lanes of B
     for (int q = 0; q < w; ++q) e->ptr_h[q] = e->mask_h[q] ? reinterpret_cast<zd*>(e->lsrc[q]->d) : nullptr;
     ECHECK(hipMemcpyAsync(e->ptr_d, e->ptr_h, sizeof(zd*) * w, hipMemcpyHostToDevice, st));
@@ -512,7 +551,7 @@ int engine_admit(LaneEngine* e, int n, const LaneRequest* reqs, std::vector<Lane
     // src_o = b_o - D_oe R_ee^-1 b_e, rhs = Qhat gamma5 src_o
     const zd* b_e = e->B;
     const zd* b_o = e->B + 12 * S * w;
-    lym::launch_clover_apply_lanes(c->g, e->Ainv, 36 * S, e->mask_d, b_e, e->te, w, st);
+    lym::launch_clover_apply_lanes(c->g, e->Ainv, 36 * S, e->lset_d, e->mask_d, b_e, e->te, w, st);
     lym::DslashParams a = base(e, 0);
     a.hop_sign = -1.0;
     a.parity = 1;
@@ -625,6 +664,7 @@ int engine_harvest(LaneEngine* e, std::vector<LaneResult>* done) {
         res.info.applies = 1 + (unsigned)H->step[q] + (unsigned)H->nrel[q];
         done->push_back(res);
         L.busy = false;
+        --e->set_refs[e->lset_h[q]];
     }
     // narrow the batch when its busy lanes fit a smaller width
     if (hi == 0) {

@@ -51,6 +51,7 @@ struct DslashParams {
     // A_sc + r * Asc_lane), Ainv + r * Ainv_lane_bytes (Ainv_sc + r * Ainvsc_lane)
     int lane_ops;
     long A_lane_bytes, Ainv_lane_bytes, Asc_lane, Ainvsc_lane;
+    const int* lane_set;  // lane_ops: lane r reads the block set lane_set[r] (NULL: set r)
     // Fused update of the mixed-precision CG (single RHS, FP32/FP16 storage; the
     // last launch of A p): instead of storing its output v = (A p)_x, the
     // launch updates x_s += alpha p and r_s -= alpha v at every site, with
@@ -242,7 +243,9 @@ void launch_lane_sub(zd* out, const zd* a, const zd* b, long rows, int nr, const
 void launch_relayout(int esize, const void* in, void* out, long rows, int w_old, int w_new, hipStream_t st);
 void launch_clover_apply_nr(const Geom& g, const zd* blocks, const zd* in, zd* out, int nr, hipStream_t st);
 // the same with lane r's blocks at blocks + r * lane_elems, only for lanes with mask[r] != 0
-void launch_clover_apply_lanes(const Geom& g, const zd* blocks, long lane_elems, const int* mask, const zd* in,
+// lane r applies block set lane_set[r] (lane_set NULL: set r)
+void launch_clover_apply_lanes(const Geom& g, const zd* blocks, long lane_elems, const int* lane_set, const int* mask,
+                               const zd* in,
                                zd* out, int nr, hipStream_t st);
 
 // x = uniform [0,1) complex on real sites, zero on padding (BiCGStab start for a zero source)
