@@ -384,10 +384,30 @@ static void set_upd(lym::DslashParams& a, const lym::UpdArgs& u, const lym_ctx* 
     }
 }
 
+// the batched solver's fused steps on a launch: vectors of nparity parities
+// (1: odd-half multi-RHS fields on the odd-parity launch, 2: full lattice)
+static void set_bfuse(lym::DslashParams& a, const lym::BFuse& f, const lym_ctx* c, int prec, int nr, int nparity) {
+    const size_t S = c->g.stride;
+    a.bmode = f.mode;
+    a.balpha = f.balpha;
+    for (int q = 0; q < 2; ++q) {
+        const int vq = nparity == 2 ? q : 0;
+        if (nparity == 1 && q == 0) continue;
+        a.bsite[q] = f.bsite + vq * S * nr;
+        if (f.mode == 2) {
+            a.upd_p[q] = (const char*)f.p + vq * 12 * S * nr * selem(prec);
+            a.upd_p_sc[q] = f.p_sc ? f.p_sc + vq * S * nr : nullptr;
+            a.upd_xs[q] = (char*)f.xs + vq * 12 * S * nr * sizeof(lym::cplx<float>);
+            a.upd_rs[q] = (char*)f.rs + vq * 12 * S * nr * sizeof(lym::cplx<float>);
+        }
+    }
+}
+
 // full lattice, nr right-hand sides, storage precision prec: out = Op in
 int apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* out_sc, const void* in,
                   const float* in_sc, int gamma5, const int* skip, const int* lane_flags, int lane_skip_value,
-                  bool batch_links, double* norm_partials, int norm_max, int mfma, const lym::UpdArgs* upd) {
+                  bool batch_links, double* norm_partials, int norm_max, int mfma, const lym::UpdArgs* upd,
+                  const lym::BFuse* bf) {
     lym_ctx* c = op->ctx;
     const size_t po = 12 * (size_t)c->g.stride * nr * selem(prec);
     const size_t so = (size_t)c->g.stride * nr;
@@ -415,6 +435,7 @@ int apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* o
     a.norm_max = norm_max;
     a.mfma = mfma;
     if (upd) set_upd(a, *upd, c, prec, 2);
+    if (bf) set_bfuse(a, *bf, c, prec, nr, 2);
     return lym::launch_dslash(a, lk, op_has_clover(op) ? 2 : 1, false, c->stream);
 }
 
@@ -422,7 +443,7 @@ int apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* o
 int apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, float* out_sc, const void* in_o,
                    const float* in_sc, void* tmp_e, float* tmp_sc, int gamma5, const int* skip,
                    const int* lane_flags, int lane_skip_value, bool batch_links, double* norm_partials,
-                   int norm_max, int mfma, const lym::UpdArgs* upd) {
+                   int norm_max, int mfma, const lym::UpdArgs* upd, const lym::BFuse* bf) {
     lym_ctx* c = op->ctx;
     lym::DslashParams a = base_params(op);
     const int lk = set_op_data(a, op, prec, batch_links);
@@ -452,20 +473,21 @@ int apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, float
     b.norm_partials = norm_partials;
     b.norm_max = norm_max;
     if (upd) set_upd(b, *upd, c, prec, 1);
+    if (bf) set_bfuse(b, *bf, c, prec, nr, 1);
     return lym::launch_dslash(b, lk, op->csw != 0.0 ? 2 : 1, false, c->stream);
 }
 
 int apply_full_sloppy(lym_op* op, int prec, void* out, float* out_sc, const void* in, const float* in_sc, int gamma5,
                       const int* skip, double* norm_partials, int norm_max, const lym::UpdArgs* upd = nullptr) {
     return apply_full_nr(op, prec, 1, 0, out, out_sc, in, in_sc, gamma5, skip, nullptr, 0, false, norm_partials,
-                         norm_max, 0, upd);
+                         norm_max, 0, upd, nullptr);
 }
 
 int apply_schur_sloppy(lym_op* op, int prec, void* out_o, float* out_sc, const void* in_o, const float* in_sc,
                        void* tmp_e, float* tmp_sc, int gamma5, const int* skip, double* norm_partials, int norm_max,
                        const lym::UpdArgs* upd = nullptr) {
     return apply_schur_nr(op, prec, 1, 0, out_o, out_sc, in_o, in_sc, tmp_e, tmp_sc, gamma5, skip, nullptr, 0, false,
-                          norm_partials, norm_max, 0, upd);
+                          norm_partials, norm_max, 0, upd, nullptr);
 }
 
 // the EO driver's source and reconstruction steps on nr-fields (FP64)
@@ -1010,21 +1032,21 @@ int lym_op_multiply_sloppy_batch(lym_op* op, int prec, int matrix_cores, unsigne
         float* res_sc = op->power == 2 ? sout_sc : smid_sc;
         if (op->kind == LYM_OP_EO_CLOVER) {
             lym_internal::apply_schur_nr(op, prec, nr, 0, pf(smid, 1), pscl(smid_sc, 1), pf(sin, 1), pscl(sin_sc, 1),
-                                         stmp, stmp_sc, op->gamma5, nullptr, nullptr, 0, true, nullptr, 0, matrix_cores, nullptr);
+                                         stmp, stmp_sc, op->gamma5, nullptr, nullptr, 0, true, nullptr, 0, matrix_cores, nullptr, nullptr);
             if (op->power == 2)
                 lym_internal::apply_schur_nr(op, prec, nr, 0, pf(sout, 1), pscl(sout_sc, 1), pf(smid, 1),
                                              pscl(smid_sc, 1), stmp, stmp_sc, op->gamma5, nullptr, nullptr, 0, true,
-                                             nullptr, 0, matrix_cores, nullptr);
+                                             nullptr, 0, matrix_cores, nullptr, nullptr);
             // even sites copied through in FP64, odd sites from the sloppy result
             lym::launch_spinor_from_sloppy(prec, pf(res, 1), pscl(res_sc, 1), X + 12 * S * nr, (long)S * nr,
                                            (int)(S * nr), c->stream);
             HIPCHECK(hipMemcpyAsync(X, B, sizeof(zd) * 12 * S * nr, hipMemcpyDeviceToDevice, c->stream));
         } else {
             lym_internal::apply_full_nr(op, prec, nr, 0, smid, smid_sc, sin, sin_sc, op->gamma5, nullptr, nullptr, 0,
-                                        true, nullptr, 0, matrix_cores, nullptr);
+                                        true, nullptr, 0, matrix_cores, nullptr, nullptr);
             if (op->power == 2)
                 lym_internal::apply_full_nr(op, prec, nr, 0, sout, sout_sc, smid, smid_sc, op->gamma5, nullptr,
-                                            nullptr, 0, true, nullptr, 0, matrix_cores, nullptr);
+                                            nullptr, 0, true, nullptr, 0, matrix_cores, nullptr, nullptr);
             lym::launch_spinor_from_sloppy(prec, res, res_sc, X, 2 * (long)S * nr, (int)(S * nr), c->stream);
         }
         for (int q = 0; q < nr; ++q) hp[q] = reinterpret_cast<zd*>(outs[q]->d);

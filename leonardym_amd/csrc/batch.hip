@@ -212,6 +212,19 @@ __global__ __launch_bounds__(BC * NR) void k_bcg_dot(const BState* s, const type
     chunk_store<NR>(v, partials, nchunks);
 }
 
+// chunk sums of one value per (site, RHS) written by a fused stencil launch
+// (bsite[j * NR + q]); lanes that are done contribute zero, as in k_bcg_dot
+template <int NR>
+__global__ __launch_bounds__(BC * NR) void k_bcg_chunksum(const BState* s, const double* bsite, long nsites,
+                                                          double* partials, long nchunks) {
+    if (s->all_done) return;
+    long j;
+    int q;
+    double v = 0;
+    if (chunk_pair<NR>(nsites, j, q) && !s->done[q]) v = bsite[j * NR + q];
+    chunk_store<NR>(v, partials, nchunks);
+}
+
 template <int NR>
 __global__ __launch_bounds__(FT) void k_bcg_alpha_final(BState* s, const double* partials, long nchunks) {
     if (s->all_done) return;
@@ -501,6 +514,14 @@ static void bcg_step_impl(int which, BState* s, const BcgArgs& a, hipStream_t st
             break;
         case 5:
             hipLaunchKernelGGL((k_bcg_p<PR, NR>), dim3(eb), dim3(256), 0, st, s, p, a.p_sc, a.rs, a.nsites, a.stride);
+            break;
+        case 7:
+            hipLaunchKernelGGL((k_bcg_chunksum<NR>), cg, cb, 0, st, s, a.bsite, a.nsites, a.partials, nch);
+            hipLaunchKernelGGL((k_bcg_alpha_final<NR>), dim3(1), fb, 0, st, s, a.partials, nch);
+            break;
+        case 8:
+            hipLaunchKernelGGL((k_bcg_chunksum<NR>), cg, cb, 0, st, s, a.bsite, a.nsites, a.partials, nch);
+            hipLaunchKernelGGL((k_bcg_update_final<NR>), dim3(1), fb, 0, st, s, a.partials, nch);
             break;
         default:
             hipLaunchKernelGGL((k_baccum<NR>), dim3(eb), dim3(256), 0, st, s, a.x, a.xs, a.nsites, a.stride, a.mask);

@@ -215,7 +215,7 @@ __device__ __forceinline__ void hop_pair(const DslashParams& a, int p, int site,
     if constexpr (pin_end) pin(acc);
 }
 
-template <int PR, int DIAG, bool INV, bool NORM, bool LO, bool UPD = false>
+template <int PR, int DIAG, bool INV, bool NORM, bool LO, bool UPD = false, int BM = 0>
 __device__ __forceinline__ double dslash_epilogue(const DslashParams& a, int p, int site, int r,
                                                   const cplx<typename Prec<PR>::T> acc[12], float ual = 0.f);
 
@@ -225,7 +225,7 @@ __device__ __forceinline__ double dslash_epilogue(const DslashParams& a, int p, 
 // own kappa and clover blocks (a.lane_ops); also a template parameter.  UPD:
 // the fused mixed-CG update with step ual (a.upd_*): returns the site's
 // |r_s|^2 after it, and stores no output.
-template <int PR, int LK, int DIAG, bool INV, bool NORM = false, bool LO = false, bool UPD = false>
+template <int PR, int LK, int DIAG, bool INV, bool NORM = false, bool LO = false, bool UPD = false, int BM = 0>
 __device__ __forceinline__ double dslash_site(const DslashParams& a, int p, int site, int r, float ual = 0.f) {
     typedef typename Prec<PR>::T T;
     int c[4];
@@ -244,13 +244,13 @@ __device__ __forceinline__ double dslash_site(const DslashParams& a, int p, int 
         hop_pair<2, PR, LK>(a, p, site, r, c, acc);
         hop_pair<3, PR, LK>(a, p, site, r, c, acc);
     }
-    return dslash_epilogue<PR, DIAG, INV, NORM, LO, UPD>(a, p, site, r, acc, ual);
+    return dslash_epilogue<PR, DIAG, INV, NORM, LO, UPD, BM>(a, p, site, r, acc, ual);
 }
 
 // The site's output from its hopping sum acc: v = diag(psi_x) + hop_sign
 // (-kappa acc), then R^-1, gamma5, the axpy and the store (the value
 // returned is as for dslash_site)
-template <int PR, int DIAG, bool INV, bool NORM, bool LO, bool UPD>
+template <int PR, int DIAG, bool INV, bool NORM, bool LO, bool UPD, int BM>
 __device__ __forceinline__ double dslash_epilogue(const DslashParams& a, int p, int site, int r,
                                                   const cplx<typename Prec<PR>::T> acc[12], float ual) {
     typedef typename Prec<PR>::T T;
@@ -328,6 +328,30 @@ __device__ __forceinline__ double dslash_epilogue(const DslashParams& a, int p, 
 #pragma unroll
         for (int k = 0; k < 12; ++k) v[k] = v[k] + al * w[k];
     }
+    if constexpr (BM == 2) {
+        // batch lane r: x_s += alpha_r p, r_s -= alpha_r v (v in FP32, never
+        // rounded to storage), the site's |r_s|^2 to bsite
+        static_assert(PR >= 1, "the fused update is the mixed-precision solver's");
+        const float al = (float)a.balpha[r];
+        cplx<T> pv[12];
+        load_spinor<PR>(SpinorRef<PR>{reinterpret_cast<const S*>(a.upd_p[p]), a.upd_p_sc[p]}, ST, a.nr, site, r, pv);
+        cplx<float>* xs = reinterpret_cast<cplx<float>*>(a.upd_xs[p]);
+        cplx<float>* rs = reinterpret_cast<cplx<float>*>(a.upd_rs[p]);
+        const size_t o = (size_t)site * a.nr + r, ks = (size_t)ST * a.nr;
+        double n2 = 0;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            cplx<float> x = xs[k * ks + o];
+            cplx<float> rr = rs[k * ks + o];
+            x = x + al * pv[k];
+            rr = rr - al * v[k];
+            xs[k * ks + o] = x;
+            rs[k * ks + o] = rr;
+            n2 += (double)rr.re * rr.re + (double)rr.im * rr.im;
+        }
+        a.bsite[p][o] = n2;
+        return 0;
+    }
     if constexpr (UPD) {
         // x_s += alpha p, r_s -= alpha v (v in FP32, never rounded to storage)
         static_assert(PR >= 1, "the fused update is the mixed-precision solver's");
@@ -349,6 +373,12 @@ __device__ __forceinline__ double dslash_epilogue(const DslashParams& a, int p, 
         return n2;
     }
     store_spinor<PR>(SpinorOut<PR>{reinterpret_cast<S*>(a.out[p]), a.out_sc[p]}, ST, a.nr, site, r, v);
+    if constexpr (BM == 1) {  // lane r's |v|^2 at this site: the 24 squares in FP32
+        float n2 = 0.f;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) n2 += v[k].re * v[k].re + v[k].im * v[k].im;
+        a.bsite[p][(size_t)site * a.nr + r] = (double)n2;
+    }
     if constexpr (NORM) {  // the site's 24 squares in FP32, the sum over sites in FP64
         float n2 = 0.f;
 #pragma unroll
@@ -487,7 +517,7 @@ __device__ __forceinline__ float upd_alpha(const DslashParams& a) {
 
 // NORM: also the per-workgroup output norms (a.norm_partials); LO: per-lane
 // operators; UPD: the fused mixed-CG update instead of the output store
-template <int PR, int LK, int DIAG, bool INV, bool NORM = false, bool LO = false, bool UPD = false>
+template <int PR, int LK, int DIAG, bool INV, bool NORM = false, bool LO = false, bool UPD = false, int BM = 0>
 __global__ __launch_bounds__(LYM_DSLASH_THREADS) LYM_OCC void k_dslash(DslashParams a) {
     static_assert(!NORM || PR >= 1, "output norms are the low-precision stencil's");
     static_assert(!(NORM && UPD), "one per-workgroup sum per launch");
@@ -512,7 +542,7 @@ __global__ __launch_bounds__(LYM_DSLASH_THREADS) LYM_OCC void k_dslash(DslashPar
             int p, bi;
             block_of(a, b, lb, p, bi);
             const int site = bi * a.spb + ls;
-            if (site < a.g.Vh) dslash_site<PR, LK, DIAG, INV, false, LO>(a, p, site, r);
+            if (site < a.g.Vh) dslash_site<PR, LK, DIAG, INV, false, LO, false, BM>(a, p, site, r);
         }
     } else {
         // one RHS, no lane flags (the launcher's condition for output norms):
@@ -950,7 +980,7 @@ template <int PR, int LK, int DIAG, bool INV>
 static void launch_one(const DslashParams& a, int nblocks, hipStream_t st) {
     if constexpr (PR == 2 && LK == 0) {
         // matrix-core form: FP16 SU(3) batches of 2, 4 or 8 RHS in whole waves
-        if (a.mfma && (a.nr == 2 || a.nr == 4 || a.nr == 8) && (a.spb * a.nr) % 64 == 0) {
+        if (a.mfma && !a.bmode && (a.nr == 2 || a.nr == 4 || a.nr == 8) && (a.spb * a.nr) % 64 == 0) {
             if (a.lane_ops)
                 hipLaunchKernelGGL((k_dslash_mfma<DIAG, INV, true>), dim3(nblocks), dim3(a.spb * a.nr), 0, st, a);
             else
@@ -961,6 +991,19 @@ static void launch_one(const DslashParams& a, int nblocks, hipStream_t st) {
     if (a.lane_ops) {  // per-lane operators: the serving engine's batches (never split, no output norms)
         hipLaunchKernelGGL((k_dslash<PR, LK, DIAG, INV, false, true>), dim3(nblocks), dim3(a.spb * a.nr), 0, st, a);
         return;
+    }
+    if constexpr (PR >= 1 && DIAG >= 1 && !INV) {
+        // the batched solver's fused steps (one thread per (site, RHS), never split)
+        if (a.bmode == 1) {
+            hipLaunchKernelGGL((k_dslash<PR, LK, DIAG, INV, false, false, false, 1>), dim3(nblocks),
+                               dim3(a.spb * a.nr), 0, st, a);
+            return;
+        }
+        if (a.bmode == 2) {
+            hipLaunchKernelGGL((k_dslash<PR, LK, DIAG, INV, false, false, false, 2>), dim3(nblocks),
+                               dim3(a.spb * a.nr), 0, st, a);
+            return;
+        }
     }
     // the fused mixed-CG update and output norms: asked for only on launches
     // with a diagonal and no R^-1 (the last launch of a Schur apply, a
@@ -1024,7 +1067,7 @@ static int plan_dslash(DslashParams& a) {
     a.spb = threads / a.nr;
     if (a.spb < 1) a.spb = 1;
     // two threads per site: low-precision single-RHS applies that allow it
-    a.split = (LYM_SPLIT_LOWP && a.allow_split && !a.lane_ops && a.prec >= 1 && threads % 64 == 0 &&
+    a.split = (LYM_SPLIT_LOWP && a.allow_split && !a.lane_ops && !a.bmode && a.prec >= 1 && threads % 64 == 0 &&
                (long)a.g.Vh * a.nr <= LYM_SPLIT_MAX_SITES)
                   ? threads
                   : 0;

@@ -68,6 +68,16 @@ struct DslashParams {
     void* upd_xs[2];
     void* upd_rs[2];
     double* upd_rpart;
+    // The same two steps for a batch, one value per (site, RHS) so that every
+    // RHS sums its terms over the batched solver's fixed site chunks:
+    // bmode 1 (the launch that writes Q p): also bsite[p][site * nr + r] =
+    // |(Q p)_site|^2 of lane r; bmode 2 (the last launch of Q^2 p): no output
+    // store; x_s += alpha_r p and r_s -= alpha_r v (alpha from balpha[r]; p,
+    // x_s, r_s as upd_p / upd_xs / upd_rs, multi-RHS fields) and bsite = the
+    // site's |r_s|^2 of lane r.
+    int bmode;
+    double* bsite[2];
+    const double* balpha;
 };
 
 // Arguments of the fused mixed-CG update (DslashParams::upd_*) for one vector
@@ -229,8 +239,20 @@ struct BcgArgs {
     int max_steps;
     double* partials;  // BMAX * ceil(nsites / BATCH_CHUNK) doubles
     const int* mask = nullptr;  // steps 0 (init) and 6 (final accumulate): only lanes with mask[q] != 0
+    const double* bsite = nullptr;  // steps 7, 8: one value per (site, RHS), site-major
 };
-// which: 0 init, 1 alpha, 2 update, 3 reliable accumulate, 4 reliable residual, 5 p update, 6 final accumulate
+// the fused batch steps on one stencil launch (DslashParams::bmode)
+struct BFuse {
+    int mode;
+    double* bsite;
+    const double* balpha;
+    const void* p;
+    const float* p_sc;
+    void* xs;
+    void* rs;
+};
+// which: 0 init, 1 alpha, 2 update, 3 reliable accumulate, 4 reliable residual, 5 p update, 6 final accumulate,
+// 7 alpha and 8 the update's sums from the per-(site, RHS) values at bsite (a stencil's bmode 1 / 2)
 bool launch_bcg_step(int which, int prec, int nr, BState* s, const BcgArgs& a, hipStream_t st);
 bool batch_nr_supported(int nr);
 void launch_pack(const zd* const* srcs_dev, int nr, zd* multi, long n, hipStream_t st);

@@ -58,12 +58,13 @@ int apply_schur_sloppy(lym_op* op, int prec, void* out_o, float* out_sc, const v
 int apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* out_sc, const void* in,
                   const float* in_sc, int gamma5, const int* skip, const int* lane_flags = nullptr,
                   int lane_skip_value = 0, bool batch_links = false, double* norm_partials = nullptr,
-                  int norm_max = 0, int mfma = 0, const lym::UpdArgs* upd = nullptr);
+                  int norm_max = 0, int mfma = 0, const lym::UpdArgs* upd = nullptr,
+                  const lym::BFuse* bf = nullptr);
 int apply_schur_nr(lym_op* op, int prec, int nr, int threads, void* out_o, float* out_sc, const void* in_o,
                    const float* in_sc, void* tmp_e, float* tmp_sc, int gamma5, const int* skip,
                    const int* lane_flags = nullptr, int lane_skip_value = 0, bool batch_links = false,
                    double* norm_partials = nullptr, int norm_max = 0, int mfma = 0,
-                   const lym::UpdArgs* upd = nullptr);
+                   const lym::UpdArgs* upd = nullptr, const lym::BFuse* bf = nullptr);
 void eo_source_nr(lym_op* op, int nr, const zd* b_e, const zd* b_o, zd* te, zd* src_o);
 void eo_reconstruct_nr(lym_op* op, int nr, const zd* b_e, const zd* x_o, zd* x_e);
 }  // namespace lym_internal
@@ -107,6 +108,7 @@ struct lym_solver {
     float* bsvsc[4];
     lym::cplx<float>* bxs;
     lym::cplx<float>* brs;
+    double* bsite;          // one value per (site, RHS): the fused steps' terms
     double* bpartials;
     void* bstate;
     void* bhost;
@@ -597,7 +599,9 @@ void free_batch(lym_solver* s) {
     }
     if (s->bxs) (void)hipFree(s->bxs);
     if (s->brs) (void)hipFree(s->brs);
+    if (s->bsite) (void)hipFree(s->bsite);
     s->bxs = s->brs = nullptr;
+    s->bsite = nullptr;
     s->bnr = 0;
     s->bdom = 0;
 }
@@ -642,6 +646,8 @@ int ensure_batch(lym_solver* s, int nr, int dom, int prec) {
     }
     if ((rc = dzalloc(c, (void**)&s->bxs, 12 * S * dom * nr * 8))) return rc;
     if ((rc = dzalloc(c, (void**)&s->brs, 12 * S * dom * nr * 8))) return rc;
+    // zero at the padding sites (past the parity's sites), which no launch writes
+    if ((rc = dzalloc(c, (void**)&s->bsite, S * dom * nr * sizeof(double)))) return rc;
     s->bnr = nr;
     s->bdom = dom;
     s->bprec = prec;
@@ -867,6 +873,7 @@ int run_batch(lym_solver* s, lym_op* op, bool eo, bool half, int nr, int thr, in
     a.stride = (int)S;
     a.max_steps = (int)s->max_steps;
     a.partials = s->bpartials;
+    a.bsite = s->bsite;
     if (!lym::launch_bcg_step(0, prec, nr, Sdev, a, st)) return fail(LYM_ERR_UNSUPPORTED, "batch of %d RHS", nr);
     HIPCHECK(hipMemcpyAsync(Sh, Sdev, sizeof(lym::BState), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
@@ -876,11 +883,31 @@ int run_batch(lym_solver* s, lym_op* op, bool eo, bool half, int nr, int thr, in
     key << 4 << s << nr << thr << prec << eo << half << g5 << pw << x << r << b << Ax << mid << tmp << a.xs << a.rs
         << a.p << a.Ap << a.p_sc << a.Ap_sc << a.nsites << HS.mid << HS.tmp << HS.mid_sc << HS.tmp_sc;
     key_op(key, op);
+    // The even-odd sloppy Qhat^2 p with the CG's two sums fused into its
+    // stencil launches (LYM_FUSED_BATCH=0 keeps the separate dot and update
+    // launches): the launch writing Qhat p also leaves |(Qhat p)_site|^2 per
+    // (site, RHS), so <p, A p> = |Qhat p|^2 sums over the fixed chunks; the
+    // last launch updates x_s and r_s from its unrounded FP32 output instead of
+    // storing A p and leaves |r_s|^2 per (site, RHS).
+    const char* fb_env = std::getenv("LYM_FUSED_BATCH");
+    const bool fused = prec >= 1 && dom == 1 && g5 == 1 && pw == 2 && !(fb_env && fb_env[0] == '0');
+    key << fused << s->bsite;
+    const lym::BFuse bf1{1, s->bsite, nullptr, nullptr, nullptr, nullptr, nullptr};
+    const lym::BFuse bf2{2, s->bsite, &Sdev->alpha[0], a.p, a.p_sc, a.xs, a.rs};
     auto body = [&](unsigned it) {
         // finished RHS skip the sloppy apply; only RHS with a due update take the FP64 one
-        HS.apply(prec, a.Ap, a.Ap_sc, a.p, a.p_sc, &Sdev->all_done, Sdev->done, 1);
-        lym::launch_bcg_step(1, prec, nr, Sdev, a, st);
-        lym::launch_bcg_step(2, prec, nr, Sdev, a, st);
+        if (fused) {
+            lym_internal::apply_schur_nr(op, prec, nr, thr, HS.mid, HS.mid_sc, a.p, a.p_sc, HS.tmp, HS.tmp_sc, 1,
+                                         &Sdev->all_done, Sdev->done, 1, true, nullptr, 0, mfma, nullptr, &bf1);
+            lym::launch_bcg_step(7, prec, nr, Sdev, a, st);
+            lym_internal::apply_schur_nr(op, prec, nr, thr, nullptr, nullptr, HS.mid, HS.mid_sc, HS.tmp, HS.tmp_sc,
+                                         1, &Sdev->all_done, Sdev->done, 1, true, nullptr, 0, mfma, nullptr, &bf2);
+            lym::launch_bcg_step(8, prec, nr, Sdev, a, st);
+        } else {
+            HS.apply(prec, a.Ap, a.Ap_sc, a.p, a.p_sc, &Sdev->all_done, Sdev->done, 1);
+            lym::launch_bcg_step(1, prec, nr, Sdev, a, st);
+            lym::launch_bcg_step(2, prec, nr, Sdev, a, st);
+        }
         lym::launch_bcg_step(5, prec, nr, Sdev, a, st);
         if (rel_slot(it)) {
             lym::launch_bcg_step(3, prec, nr, Sdev, a, st);

@@ -128,6 +128,31 @@ def test_batch_graph_replay_equals_eager(case, monkeypatch):
         assert np.array_equal(xe[q], xg[q])
 
 
+@pytest.mark.parametrize("sloppy", [lym.PREC_HALF, lym.PREC_SINGLE])
+def test_fused_batch_steps_against_separate_launches(case, sloppy, monkeypatch):
+    """The even-odd batch with its CG sums fused into the stencil launches
+    (|Qhat p|^2 per site from the launch writing Qhat p; x_s, r_s and |r_s|^2
+    from the last launch's unrounded output) against the separate dot and
+    update launches: both reach the FP64 stopping rule, the solutions agree to
+    the solve's tolerance, the iteration counts to a few per cent, and the
+    fused form gives the same bits in batches of 4 and 1."""
+    srcs = [case.gauss() for _ in range(5)]
+    eps = [(1e-11 * np.linalg.norm(b)) ** 2 for b in srcs]
+    op = case.op(lym.OP_CLOVER)
+    monkeypatch.setenv("LYM_FUSED_BATCH", "0")
+    xs_sep, inf_sep, _ = solve_batch(case, op, lym.SOLVER_PRECOND_MIXED_CG, srcs, 4, eps, sloppy)
+    monkeypatch.delenv("LYM_FUSED_BATCH")
+    xs4, inf4, _ = solve_batch(case, op, lym.SOLVER_PRECOND_MIXED_CG, srcs, 4, eps, sloppy)
+    xs1, inf1, _ = solve_batch(case, op, lym.SOLVER_PRECOND_MIXED_CG, srcs, 1, eps, sloppy)
+    print("iterations separate %s fused %s" % ([int(i.steps) for i in inf_sep], [int(i.steps) for i in inf4]))
+    for q, b in enumerate(srcs):
+        assert inf_sep[q].converged and inf4[q].converged and inf1[q].converged
+        assert inf4[q].steps == inf1[q].steps and np.array_equal(xs4[q], xs1[q])
+        assert abs(int(inf4[q].steps) - int(inf_sep[q].steps)) <= max(3, 0.05 * inf_sep[q].steps)
+        assert true_residual(case, xs4[q], b) <= 1e-10
+        assert np.linalg.norm((xs4[q] - xs_sep[q]).ravel()) / np.linalg.norm(xs_sep[q].ravel()) < 1e-9
+
+
 def test_point_sources_16x16x16x32():
     """Point sources (a localised source decays below the smallest normal
     float within a few dozen hops): mixed CG with FP16 storage, alone and
