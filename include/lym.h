@@ -14,8 +14,12 @@
  *            (FermionicGroup, source/MatrixTypedef.h:61-90)
  * with sites in t-fastest lexicographic order
  *   idx = ((x*Ly + y)*Lz + z)*Lt + t   (source/MPILattice/LocalLayout.h:149-155).
- * Lt must be even; even-odd operators need all four extents even
- * (source/inverters/PreconditionedBiCGStab.cpp:43).
+ * Any extents >= 2 (source/MPILattice/LocalLayout.h:34-110).  Even-odd
+ * operators, the even-odd driver, low-precision applies, mixed-precision,
+ * batched and served solves need all four extents even
+ * (source/inverters/PreconditionedBiCGStab.cpp:43); on a lattice with an odd
+ * extent the full-lattice FP64 operators, multiplyAdd and the FP64 CG and
+ * BiCGStab solvers run, and the rest returns LYM_ERR_UNSUPPORTED.
  */
 #ifndef LYM_H
 #define LYM_H

@@ -252,7 +252,7 @@ void apply_full1(lym_op* op, zd* out, const zd* in, const zd* v2, zd alpha, int 
     }
     a.alpha = alpha;
     a.gamma5 = gamma5;
-    a.parity = -1;
+    a.parity = c->g.tsh ? -1 : 0;  // both parities, or the one class of a lattice with an odd extent
     a.skip = skip;
     lym::launch_dslash(a, link_kind(op->gauge), op_has_clover(op) ? 2 : 1, false, c->stream);
 }
@@ -427,7 +427,7 @@ int apply_full_nr(lym_op* op, int prec, int nr, int threads, void* out, float* o
         }
     }
     a.gamma5 = gamma5;
-    a.parity = -1;
+    a.parity = c->g.tsh ? -1 : 0;
     a.skip = skip;
     a.lane_flags = lane_flags;
     a.lane_skip_value = lane_skip_value;
@@ -554,14 +554,14 @@ int lym_ctx_create(const int dims[4], int rep, int bc_t, lym_ctx** out) {
     *out = nullptr;
     for (int i = 0; i < 4; ++i)
         if (dims[i] < 2) return fail(LYM_ERR_INVALID, "lattice extent %d must be >= 2", dims[i]);
-    if (LYM_T_SLOW ? dims[0] % 2 : dims[3] % 2)
-        return fail(LYM_ERR_INVALID, "%s = %d must be even for the even-odd device layout", LYM_T_SLOW ? "Lx" : "Lt",
-                    LYM_T_SLOW ? dims[0] : dims[3]);
     if (rep != LYM_REP_SU3_FUND && rep != LYM_REP_SU2_ADJ) return fail(LYM_ERR_INVALID, "unknown representation %d", rep);
     if (bc_t != 1 && bc_t != -1) return fail(LYM_ERR_INVALID, "bc_t must be +1 or -1");
     long V = (long)dims[0] * dims[1] * dims[2] * dims[3];
+    // all extents even: the even-odd layout; an odd extent: one site class
+    // (only the full-lattice FP64 operators and solvers run there)
+    const int tsh = (dims[0] | dims[1] | dims[2] | dims[3]) % 2 == 0 ? 1 : 0;
     // a field component is read through 32-bit buffer offsets below 2 GiB
-    if (V / 2 > (1L << 27) - 64) return fail(LYM_ERR_INVALID, "lattice too large (at most 2^28 sites)");
+    if ((V >> tsh) > (1L << 27) - 64) return fail(LYM_ERR_INVALID, "lattice too large (at most 2^27 sites per class)");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(LYM_ERR_DEVICE, "no HIP device available");
     lym_ctx* c = new lym_ctx();
@@ -571,10 +571,11 @@ int lym_ctx_create(const int dims[4], int rep, int bc_t, lym_ctx** out) {
         c->g.L[i] = dims[i];
     }
     c->V = V;
-    c->g.Lth = dims[3] / 2;
-    c->g.Lxh = dims[0] / 2;
-    c->g.Vh = (int)(V / 2);
-    c->g.stride = (int)(((V / 2) + 63) / 64 * 64);
+    c->g.tsh = tsh;
+    c->g.Lth = dims[3] >> tsh;
+    c->g.Lxh = dims[0] >> tsh;
+    c->g.Vh = (int)(V >> tsh);
+    c->g.stride = (int)(((V >> tsh) + 63) / 64 * 64);
     c->g.bc_t = bc_t;
     c->rep = rep;
     (void)hipGetDevice(&c->device);
@@ -946,6 +947,7 @@ int lym_op_multiply_sloppy(lym_op* op, int prec, lym_spinor* out, const lym_spin
     if (out == in) return fail(LYM_ERR_INVALID, "multiply needs distinct input and output");
     if (prec != LYM_PREC_SINGLE && prec != LYM_PREC_HALF)
         return fail(LYM_ERR_INVALID, "storage precision must be LYM_PREC_SINGLE or LYM_PREC_HALF");
+    if (!op->ctx->g.tsh) return fail(LYM_ERR_UNSUPPORTED, "low-precision applies need all lattice extents even");
     rc = op_sloppy_data(op, prec, false);
     if (rc) return rc;
     lym_ctx* c = op->ctx;
@@ -990,6 +992,7 @@ int lym_op_multiply_sloppy_batch(lym_op* op, int prec, int matrix_cores, unsigne
     if (rc) return rc;
     if (prec != LYM_PREC_SINGLE && prec != LYM_PREC_HALF)
         return fail(LYM_ERR_INVALID, "storage precision must be LYM_PREC_SINGLE or LYM_PREC_HALF");
+    if (!op->ctx->g.tsh) return fail(LYM_ERR_UNSUPPORTED, "low-precision applies need all lattice extents even");
     if (!outs || !ins || !lym::batch_nr_supported((int)nrhs))
         return fail(LYM_ERR_INVALID, "nrhs must be one of 1,2,3,4,6,8,12,16 with output and input arrays");
     lym_ctx* c = op->ctx;

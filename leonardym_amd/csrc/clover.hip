@@ -42,7 +42,7 @@ __device__ __forceinline__ M3 m3dag(const M3& A) {
 
 template <int LK>
 __device__ __forceinline__ M3 get_link(const Geom& g, const void* links, const int c[4], int mu) {
-    int p = (c[0] + c[1] + c[2] + c[3]) & 1;
+    int p = site_parity(g, c);
     int idx = eo_index(g, c[0], c[1], c[2], c[3]);
     LinkMat<LK, double> L;
     load_link<0, LK>(links, g.stride, p, mu, idx, L);
@@ -334,7 +334,7 @@ __global__ void k_clover_apply(Geom g, const zd* __restrict__ blocks, const zd* 
 }
 
 void launch_clover_B(const Geom& g, const void* links, int link_kind, zd* B, hipStream_t st) {
-    int nb = 2 * ((g.Vh + 127) / 128);
+    int nb = (1 + g.tsh) * ((g.Vh + 127) / 128);
     switch (link_kind) {
         case 0: hipLaunchKernelGGL(k_clover_B<0>, dim3(nb), dim3(128), 0, st, g, links, B); break;
         case 1: hipLaunchKernelGGL(k_clover_B<1>, dim3(nb), dim3(128), 0, st, g, links, B); break;

@@ -87,12 +87,12 @@ __global__ __launch_bounds__(256) void k_meson_timeslice(Geom g, const double* s
     const long nr = (long)g.L[0] * g.L[1] * g.L[2];
     double s = 0;
     for (long r = threadIdx.x; r < nr; r += blockDim.x) {
-        // r = (x Ly + y) Lz + z; lexicographic site r Lt + t, parity x + y + z + t
+        // r = (x Ly + y) Lz + z; lexicographic site r Lt + t, class (x + y + z + t) & tsh
         const int z = (int)(r % g.L[2]);
         const long r2 = r / g.L[2];
         const int y = (int)(r2 % g.L[1]), x = (int)(r2 / g.L[1]);
-        const int par = (x + y + z + t) & 1;
-        s += v[(long)par * g.stride + r * g.Lth + (t >> 1)];
+        const int par = (x + y + z + t) & g.tsh;
+        s += v[(long)par * g.stride + r * g.Lth + (t >> g.tsh)];
     }
     sh[threadIdx.x] = s;
     __syncthreads();

@@ -211,7 +211,7 @@ __device__ __forceinline__ void hop_pair(const DslashParams& a, int p, int site,
                                          int lbase = 0) {
     typedef typename Prec<PR>::T T;
     const Geom& g = a.g;
-    const int q = p ^ 1;
+    const int q = p ^ g.tsh;
     int cu[4] = {c[0], c[1], c[2], c[3]}, cd[4] = {c[0], c[1], c[2], c[3]};
     cu[MU] = c[MU] + 1 == g.L[MU] ? 0 : c[MU] + 1;
     cd[MU] = c[MU] == 0 ? g.L[MU] - 1 : c[MU] - 1;
@@ -611,7 +611,7 @@ __global__ __launch_bounds__(LYM_DSLASH_THREADS) LYM_OCC void k_dslash_stg(Dslas
         const int base = bi * LYM_DSLASH_THREADS;
         __syncthreads();  // the previous block's reads of stg are done
         if (base + t < a.g.Vh) {
-            const zd* src = reinterpret_cast<const zd*>(a.in[p ^ 1]);
+            const zd* src = reinterpret_cast<const zd*>(a.in[p ^ a.g.tsh]);
 #pragma unroll
             for (int k = 0; k < 12; ++k)
                 stg[k * LYM_DSLASH_THREADS + t] = fld<zd>(src + (size_t)k * a.g.stride, (unsigned)(base + t));
@@ -960,7 +960,7 @@ __global__ __launch_bounds__(LYM_DSLASH_THREADS) void k_dslash_mfma(DslashParams
     for (int lb = b.lo + b.k; lb < b.hi; lb += b.nphys) {
         int p, bi;
         block_of(a, b, lb, p, bi);
-        const int q = p ^ 1;
+        const int q = p ^ a.g.tsh;
         for (int sl = 0; sl < spw; ++sl) {
             const int ls = wv * spw + sl;
             const int site = __builtin_amdgcn_readfirstlane(bi * spb + ls);
@@ -1196,18 +1196,22 @@ extern "C" int lym_launch_plan(const int dims[4], int parity, int nr, int thread
     if (!dims || !grid || !spb || !nbpp || !xslice_blocks) return fail(LYM_ERR_INVALID, "null argument");
     for (int i = 0; i < 4; ++i)
         if (dims[i] < 2) return fail(LYM_ERR_INVALID, "lattice extent %d must be >= 2", dims[i]);
-    if (dims[3] % 2) return fail(LYM_ERR_INVALID, "Lt must be even");
-    if (parity < -1 || parity > 1 || nr < 1 || nr > 16 || threads < 0 || threads > LYM_DSLASH_THREADS)
+    // the context's layout: even-odd when all extents are even, else one class
+    // (whose launches are parity 0)
+    const int tsh = (dims[0] | dims[1] | dims[2] | dims[3]) % 2 == 0 ? 1 : 0;
+    if (parity < -1 || parity > 1 || nr < 1 || nr > 16 || threads < 0 || threads > LYM_DSLASH_THREADS ||
+        (!tsh && parity != 0))
         return fail(LYM_ERR_INVALID, "bad launch arguments");
     const long V = (long)dims[0] * dims[1] * dims[2] * dims[3];
-    if (V / 2 > (1L << 27) - 64) return fail(LYM_ERR_INVALID, "lattice too large");
+    if ((V >> tsh) > (1L << 27) - 64) return fail(LYM_ERR_INVALID, "lattice too large");
     lym::DslashParams a;
     std::memset(&a, 0, sizeof(a));
     for (int i = 0; i < 4; ++i) a.g.L[i] = dims[i];
-    a.g.Lth = dims[3] / 2;
-    a.g.Lxh = dims[0] / 2;
-    a.g.Vh = (int)(V / 2);
-    a.g.stride = (int)((V / 2 + 63) / 64 * 64);
+    a.g.tsh = tsh;
+    a.g.Lth = dims[3] >> tsh;
+    a.g.Lxh = dims[0] >> tsh;
+    a.g.Vh = (int)(V >> tsh);
+    a.g.stride = (int)(((V >> tsh) + 63) / 64 * 64);
     a.g.bc_t = -1;
     a.nr = nr;
     a.threads = threads;

@@ -93,7 +93,7 @@ __device__ __forceinline__ void load_sp(const zd* f, const Geom& g, int p, int i
 template <int MU, int LK>
 __device__ void force_link(const ForceParams& a, const int c[4], long lex) {
     const Geom& g = a.g;
-    const int p = (c[0] + c[1] + c[2] + c[3]) & 1;
+    const int p = site_parity(g, c);
     const int i = eo_index(g, c[0], c[1], c[2], c[3]);
     int cu[4] = {c[0], c[1], c[2], c[3]};
     cu[MU] = c[MU] + 1 == g.L[MU] ? 0 : c[MU] + 1;
@@ -116,9 +116,9 @@ __device__ void force_link(const ForceParams& a, const int c[4], long lex) {
         const zd* Bf = ord == 0 ? a.Y : a.X;
         zd A0[12], Au[12], B0[12], Bu[12];
         load_sp(Af, g, p, i, A0);
-        load_sp(Af, g, p ^ 1, iu, Au);
+        load_sp(Af, g, p ^ g.tsh, iu, Au);
         load_sp(Bf, g, p, i, B0);
-        load_sp(Bf, g, p ^ 1, iu, Bu);
+        load_sp(Bf, g, p ^ g.tsh, iu, Bu);
         // t = -kappa g5 (1 - g_mu) B(x+mu)
         zd t[12];
         proj<MU>(Bu, -1.0, a.kappa, t);

@@ -16,10 +16,10 @@ __device__ __forceinline__ void lex_coords(const Geom& g, long lex, int c[4]) {
 
 __global__ void k_spinor_to_eo(Geom g, const zd* __restrict__ aos, zd* __restrict__ eo) {
     long lex = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (lex >= 2L * g.Vh) return;
+    if (lex >= ((long)g.Vh << g.tsh)) return;
     int c[4];
     lex_coords(g, lex, c);
-    int p = (c[0] + c[1] + c[2] + c[3]) & 1;
+    int p = site_parity(g, c);
     int idx = eo_index(g, c[0], c[1], c[2], c[3]);
     zd* o = eo + (size_t)p * 12 * g.stride + idx;
 #pragma unroll
@@ -28,10 +28,10 @@ __global__ void k_spinor_to_eo(Geom g, const zd* __restrict__ aos, zd* __restric
 
 __global__ void k_spinor_from_eo(Geom g, const zd* __restrict__ eo, zd* __restrict__ aos) {
     long lex = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (lex >= 2L * g.Vh) return;
+    if (lex >= ((long)g.Vh << g.tsh)) return;
     int c[4];
     lex_coords(g, lex, c);
-    int p = (c[0] + c[1] + c[2] + c[3]) & 1;
+    int p = site_parity(g, c);
     int idx = eo_index(g, c[0], c[1], c[2], c[3]);
     const zd* s = eo + (size_t)p * 12 * g.stride + idx;
 #pragma unroll
@@ -42,10 +42,10 @@ __global__ void k_spinor_from_eo(Geom g, const zd* __restrict__ eo, zd* __restri
 template <class E>
 __global__ void k_links_to_eo(Geom g, const E* __restrict__ aos, int ncomp, int unfold_bc, E* __restrict__ eo) {
     long lex = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (lex >= 2L * g.Vh) return;
+    if (lex >= ((long)g.Vh << g.tsh)) return;
     int c[4];
     lex_coords(g, lex, c);
-    int p = (c[0] + c[1] + c[2] + c[3]) & 1;
+    int p = site_parity(g, c);
     int idx = eo_index(g, c[0], c[1], c[2], c[3]);
     for (int mu = 0; mu < 4; ++mu) {
         const E* s = aos + (lex * 4 + mu) * 9;
@@ -61,18 +61,18 @@ __global__ void k_links_to_eo(Geom g, const E* __restrict__ aos, int ncomp, int 
 }
 
 void launch_spinor_to_eo(const Geom& g, const zd* aos, zd* eo, hipStream_t st) {
-    long V = 2L * g.Vh;
+    long V = (long)g.Vh << g.tsh;
     hipLaunchKernelGGL(k_spinor_to_eo, dim3((V + 255) / 256), dim3(256), 0, st, g, aos, eo);
 }
 
 void launch_spinor_from_eo(const Geom& g, const zd* eo, zd* aos, hipStream_t st) {
-    long V = 2L * g.Vh;
+    long V = (long)g.Vh << g.tsh;
     hipLaunchKernelGGL(k_spinor_from_eo, dim3((V + 255) / 256), dim3(256), 0, st, g, eo, aos);
 }
 
 void launch_links_to_eo(const Geom& g, const void* links_aos, int real_links, int ncomp, int unfold_bc, void* eo,
                         hipStream_t st) {
-    long V = 2L * g.Vh;
+    long V = (long)g.Vh << g.tsh;
     if (real_links)
         hipLaunchKernelGGL(k_links_to_eo<double>, dim3((V + 255) / 256), dim3(256), 0, st, g,
                            (const double*)links_aos, ncomp, unfold_bc, (double*)eo);

@@ -37,13 +37,20 @@ namespace lym {
 #define LYM_T_SLOW 0
 #endif
 
+// Two site classes (tsh = 1): the even-odd layout, eo = lex >> 1, parity
+// (x + y + z + t) & 1, every neighbour in the other parity -- lattices whose
+// four extents are all even.  One class (tsh = 0): a lattice with an odd
+// extent has no such two-colouring under periodic wrapping, so all sites form
+// "parity 0" in lexicographic order (eo = lex) and a hop stays in class 0.
+// The same index arithmetic covers both: parity & tsh, lex >> tsh, p ^ tsh.
 struct Geom {
     int L[4];    // Lx, Ly, Lz, Lt
-    int Lth;     // Lt / 2
-    int Lxh;     // Lx / 2
-    int Vh;      // sites per parity
+    int Lth;     // Lt >> tsh
+    int Lxh;     // Lx >> tsh
+    int Vh;      // sites per class
     int stride;  // component stride (Vh rounded up to 64)
     int bc_t;    // +1 periodic, -1 antiperiodic in time
+    int tsh;     // 1: even-odd layout; 0: one class (a lattice with an odd extent)
 };
 
 // ---------------------------------------------------------------- complex
@@ -105,11 +112,11 @@ __device__ __forceinline__ void eo_coords(const Geom& g, int p, int idx, int c[4
     r /= g.L[1];
     c[2] = r % g.L[2];
     c[3] = r / g.L[2];
-    c[0] = 2 * xh + ((c[1] + c[2] + c[3] + p) & 1);
+    c[0] = (xh << g.tsh) + ((c[1] + c[2] + c[3] + p) & g.tsh);
 }
 
 __device__ __forceinline__ int eo_index(const Geom& g, int x, int y, int z, int t) {
-    return ((t * g.L[2] + z) * g.L[1] + y) * g.Lxh + (x >> 1);
+    return ((t * g.L[2] + z) * g.L[1] + y) * g.Lxh + (x >> g.tsh);
 }
 #else
 // device order t fastest (the reference's lexicographic order), eo = lex >> 1
@@ -120,12 +127,15 @@ __device__ __forceinline__ void eo_coords(const Geom& g, int p, int idx, int c[4
     r /= g.L[2];
     c[1] = r % g.L[1];
     c[0] = r / g.L[1];
-    c[3] = 2 * th + ((c[0] + c[1] + c[2] + p) & 1);
+    c[3] = (th << g.tsh) + ((c[0] + c[1] + c[2] + p) & g.tsh);
 }
 
 __device__ __forceinline__ int eo_index(const Geom& g, int x, int y, int z, int t) {
-    return ((x * g.L[1] + y) * g.L[2] + z) * g.Lth + (t >> 1);
+    return ((x * g.L[1] + y) * g.L[2] + z) * g.Lth + (t >> g.tsh);
 }
+
+// class of the site at coordinates c
+__device__ __forceinline__ int site_parity(const Geom& g, const int c[4]) { return (c[0] + c[1] + c[2] + c[3]) & g.tsh; }
 #endif
 
 // XCD-aware block order: MI355X deals workgroups round-robin over its 8 XCDs;

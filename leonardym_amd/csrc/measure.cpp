@@ -80,8 +80,8 @@ extern "C" int lym_meson_correlators(lym_solver* solver, lym_op* op, const int s
     }
     const int* s = source_site;
     const long lex = (((long)s[0] * c->dims[1] + s[1]) * c->dims[2] + s[2]) * c->dims[3] + s[3];
-    const int par = (s[0] + s[1] + s[2] + s[3]) & 1;
-    const long eo = lex >> 1;
+    const int par = (s[0] + s[1] + s[2] + s[3]) & g.tsh;
+    const long eo = lex >> g.tsh;
     for (int al = 0; al < 4; ++al)
         for (int col = 0; col < 3; ++col) {
             // column c * 4 + alpha holds the source (alpha, c) (MesonCorrelator.cpp:94-98)

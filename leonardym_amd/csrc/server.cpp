@@ -393,6 +393,8 @@ int lym_server_set_solver(lym_server* S, int sloppy_prec, unsigned max_steps) {
 int lym_server_add_lattice(lym_server* S, const int dims[4], int rep, int bc_t, const double* links,
                            int links_have_bc_folded, int storage, double csw, int* lattice_id) {
     if (!S || !dims || !links || !lattice_id) return fail(LYM_ERR_INVALID, "null argument");
+    if ((dims[0] | dims[1] | dims[2] | dims[3]) % 2)
+        return fail(LYM_ERR_UNSUPPORTED, "served lattices need all extents even (the engine runs the even-odd driver)");
     (void)hipSetDevice(S->device);
     auto L = std::make_unique<Lattice>();
     std::memcpy(L->dims, dims, sizeof(L->dims));
@@ -403,15 +405,13 @@ int lym_server_add_lattice(lym_server* S, const int dims[4], int rep, int bc_t, 
     int rc = lym_ctx_create(dims, rep, bc_t, &L->base);
     if (!rc) rc = lym_gauge_create(L->base, links, links_have_bc_folded, storage, &L->gauge);
     // Build the gauge field's clover blocks once, before workers share it (a
-    // lazy build by the first worker would race with the others).  Requests on
-    // a lattice with all extents even run the even-odd driver, which needs the
-    // blocks whatever csw is (csw = 0 gives unit diagonal blocks); the
-    // even-odd operator always builds them.  Otherwise only a clover term needs them.
-    const bool all_even = dims[0] % 2 == 0 && dims[1] % 2 == 0 && dims[2] % 2 == 0 && dims[3] % 2 == 0;
-    if (!rc && (all_even || csw != 0.0)) {
+    // lazy build by the first worker would race with the others).  Requests
+    // run the even-odd driver, which needs the blocks whatever csw is (csw = 0
+    // gives unit diagonal blocks); the even-odd operator always builds them.
+    if (!rc) {
         lym_op* op = nullptr;
         lym_spinor *a = nullptr, *b = nullptr;
-        rc = lym_op_create(L->base, all_even ? LYM_OP_EO_CLOVER : LYM_OP_CLOVER, 1, 0.125, csw, 1, &op);
+        rc = lym_op_create(L->base, LYM_OP_EO_CLOVER, 1, 0.125, csw, 1, &op);
         if (!rc) rc = lym_op_set_gauge(op, L->gauge);
         if (!rc) rc = lym_spinor_create(L->base, &a);
         if (!rc) rc = lym_spinor_create(L->base, &b);

@@ -390,7 +390,9 @@ int run_bicgstab(lym_solver* s, const LinOp& L, const zd* b, zd* xout, const zd*
         lym::launch_copy(x, b, n, st);
     } else {
         // the reference draws a time-seeded random start here; a fixed seed is used instead
-        lym::launch_fill_uniform(x, (int)(n / half_n(c)), c->g.stride, c->g.Vh, 0x5eedull, st);
+        // (one class on a lattice with an odd extent: its second half stays zero)
+        lym::launch_fill_uniform(x, std::min((int)(n / half_n(c)), 1 + c->g.tsh), c->g.stride, c->g.Vh, 0x5eedull,
+                                 st);
     }
     L.apply(t, x, nullptr);
     lym::launch_bicg_setup(r, rh, p, nu, b, t, n, st);
@@ -986,6 +988,7 @@ int lym_solver_solve_batch_kappas(lym_solver* s, lym_op* op, unsigned nrhs, cons
     if (op->ctx != c) return fail(LYM_ERR_INVALID, "operator of another context");
     if (op->kind != LYM_OP_CLOVER || op->power != 1)
         return fail(LYM_ERR_UNSUPPORTED, "batches over several kappa run the even-odd driver of a clover operator (power 1)");
+    if (!c->g.tsh) return fail(LYM_ERR_UNSUPPORTED, "batches over several kappa need all lattice extents even");
     for (unsigned q = 0; q < nrhs; ++q)
         if (!sources[q] || !solutions[q] || sources[q]->ctx != c || solutions[q]->ctx != c)
             return fail(LYM_ERR_INVALID, "spinor %u of another context", q);
@@ -1095,6 +1098,7 @@ int lym_solver_solve_batch(lym_solver* s, lym_op* op, unsigned nrhs, const lym_s
         if (!sources[q] || !solutions[q] || sources[q]->ctx != c || solutions[q]->ctx != c)
             return fail(LYM_ERR_INVALID, "spinor %u of another context", q);
     if (op->ctx != c) return fail(LYM_ERR_INVALID, "operator of another context");
+    if (!c->g.tsh) return fail(LYM_ERR_UNSUPPORTED, "batched solves need all lattice extents even");
     const bool eo = s->kind == LYM_SOLVER_PRECOND_MIXED_CG && s->even_odd && op->power == 1 &&
                     (op->kind == LYM_OP_CLOVER || op->kind == LYM_OP_EO_CLOVER) && c->dims[0] % 2 == 0 &&
                     c->dims[1] % 2 == 0 && c->dims[2] % 2 == 0 && c->dims[3] % 2 == 0;
@@ -1242,6 +1246,9 @@ int lym_solver_solve(lym_solver* s, lym_op* op, const lym_spinor* source, lym_sp
         (s->kind != LYM_SOLVER_BICGSTAB || op->kind == LYM_OP_EO_CLOVER))
         return fail(LYM_ERR_UNSUPPORTED, "a shift is supported by plain BiCGStab on full-lattice operators only "
                                          "(the even-odd operator has no multiplyAdd)");
+    if (mixed && !c->g.tsh)
+        return fail(LYM_ERR_UNSUPPORTED, "mixed-precision solves need all lattice extents even (FP64 CG and "
+                                         "BiCGStab run on lattices with an odd extent)");
     int rc = lym_internal::op_ready(op, eo_driver);
     if (rc) return rc;
     rc = ensure_work(s);

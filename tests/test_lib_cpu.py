@@ -161,8 +161,9 @@ def test_argument_errors_are_codes_before_any_device_work():
     import ctypes
     L = lym.lib()
     out = ctypes.c_void_p()
-    # above the largest lattice (2^28 sites: 32-bit buffer offsets per field component)
-    for dims, what in (((4, 4, 4, 3), b"even"), ((4, 4, 4, 1), b">= 2"), ((256, 256, 256, 64), b"2^28")):
+    # above the largest lattice (2^27 sites per site class: 32-bit buffer offsets
+    # per field component; a lattice with an odd extent has one class)
+    for dims, what in (((4, 4, 4, 1), b">= 2"), ((256, 256, 256, 64), b"2^27"), ((3, 512, 512, 256), b"2^27")):
         d = (ctypes.c_int * 4)(*dims)
         assert L.lym_ctx_create(d, lym.REP_SU3_FUND, -1, ctypes.byref(out)) == lym.ERR_INVALID
         assert what in L.lym_last_error()
@@ -191,15 +192,24 @@ def test_stencil_block_mapping_is_a_bijection():
     lattices = [(4, 4, 4, 4), (6, 6, 6, 8), (8, 8, 8, 8), (4, 6, 2, 8), (12, 12, 12, 12), (16, 16, 16, 32),
                 (16, 8, 12, 16), (18, 4, 4, 32), (10, 4, 4, 32), (8, 4, 8, 32), (16, 4, 4, 12), (24, 24, 24, 48),
                 (24, 24, 24, 32), (32, 32, 32, 64), (5, 7, 3, 6)]
+    lattices += [(5, 8, 8, 16), (3, 3, 3, 3), (4, 4, 4, 5)]
     for dims, parity, (nr, threads) in itertools.product(
             lattices, (-1, 0, 1), [(1, 0), (1, 64), (1, 128), (2, 0), (3, 192), (4, 128), (6, 0), (8, 0), (12, 0),
                                    (16, 256)]):
+        # a lattice with an odd extent has one site class: launches are parity 0
+        # over all V sites
+        odd = any(d % 2 for d in dims)
+        if odd and parity != 0:
+            with pytest.raises(lym.LymError):
+                lym.launch_plan(dims, parity, nr=nr, threads=threads)
+            continue
+        per_class = int(np.prod(dims)) // (1 if odd else 2)
         for lowp in ((False, True) if nr == 1 else (False,)):
             r = lym.launch_plan(dims, parity, nr=nr, threads=threads, lowp=lowp)
             v = r["visits"]
             want = np.zeros_like(v)
             want[[0, 1] if parity < 0 else [parity]] = 1
             assert np.array_equal(v, want), (dims, parity, nr, threads, lowp)
-            assert r["spb"] * r["nbpp"] >= int(np.prod(dims)) // 2 > r["spb"] * (r["nbpp"] - 1)
+            assert r["spb"] * r["nbpp"] >= per_class > r["spb"] * (r["nbpp"] - 1)
             seen += r["branches"] > 0
     assert np.all(seen > 0)  # every renumbering ran in at least one shape
