@@ -72,27 +72,23 @@ __device__ M3 field_strength(const Geom& g, const void* links, const int c[4], i
     shift(g, c, n, +1, xpn);
     shift(g, xn, m, +1, xnpm);
     shift(g, xm, n, +1, xmpn);
-    M3 Um_x = get_link<LK>(g, links, c, m), Un_x = get_link<LK>(g, links, c, n);
-    M3 Um_xm = get_link<LK>(g, links, xm, m), Un_xm = get_link<LK>(g, links, xm, n);
-    M3 Q;
+    // each leaf loads its own four links (the repeats hit in cache): fewer
+    // link matrices live at once
+    M3 Q = m3mul(m3mul(m3dag(get_link<LK>(g, links, xm, m)), m3dag(get_link<LK>(g, links, xmn, n))),
+                 m3mul(get_link<LK>(g, links, xmn, m), get_link<LK>(g, links, xn, n)));
     {
-        M3 l = m3mul(m3mul(m3dag(Um_xm), m3dag(get_link<LK>(g, links, xmn, n))),
-                     m3mul(get_link<LK>(g, links, xmn, m), get_link<LK>(g, links, xn, n)));
-        Q = l;
-    }
-    {
-        M3 Un_xn = get_link<LK>(g, links, xn, n);
-        M3 l = m3mul(m3mul(m3dag(Un_xn), get_link<LK>(g, links, xn, m)),
-                     m3mul(get_link<LK>(g, links, xnpm, n), m3dag(Um_x)));
+        M3 l = m3mul(m3mul(m3dag(get_link<LK>(g, links, xn, n)), get_link<LK>(g, links, xn, m)),
+                     m3mul(get_link<LK>(g, links, xnpm, n), m3dag(get_link<LK>(g, links, c, m))));
         for (int k = 0; k < 9; ++k) Q.a[k] = Q.a[k] + l.a[k];
     }
     {
-        M3 l = m3mul(m3mul(Um_x, get_link<LK>(g, links, xpm, n)),
-                     m3mul(m3dag(get_link<LK>(g, links, xpn, m)), m3dag(Un_x)));
+        M3 l = m3mul(m3mul(get_link<LK>(g, links, c, m), get_link<LK>(g, links, xpm, n)),
+                     m3mul(m3dag(get_link<LK>(g, links, xpn, m)), m3dag(get_link<LK>(g, links, c, n))));
         for (int k = 0; k < 9; ++k) Q.a[k] = Q.a[k] + l.a[k];
     }
     {
-        M3 l = m3mul(m3mul(Un_x, m3dag(get_link<LK>(g, links, xmpn, m))), m3mul(m3dag(Un_xm), Um_xm));
+        M3 l = m3mul(m3mul(get_link<LK>(g, links, c, n), m3dag(get_link<LK>(g, links, xmpn, m))),
+                     m3mul(m3dag(get_link<LK>(g, links, xm, n)), get_link<LK>(g, links, xm, m)));
         for (int k = 0; k < 9; ++k) Q.a[k] = Q.a[k] + l.a[k];
     }
     M3 F;
@@ -132,67 +128,86 @@ __device__ __forceinline__ void load_herm6(const zd* base, int stride, int idx, 
         }
 }
 
+// the diagonal and lower entries of a 3x3 spin block: (0,0) (1,1) (2,2) (1,0) (2,0) (2,1)
+__device__ constexpr int DI[6] = {0, 1, 2, 1, 2, 2}, DJ[6] = {0, 1, 2, 0, 0, 1};
+
+// one plane's F into the spin blocks of the upper (du, X) and lower (dl, Y) block
+template <int PL>
+__device__ __forceinline__ void acc_plane(const M3& F, zd du[6], zd dl[6], zd X[9], zd Y[9]) {
+    if constexpr (PL == 0 || PL == 5) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const zd f = ti(F.a[3 * DI[k] + DJ[k]]);
+            du[k] = PL == 0 ? du[k] - f : du[k] + f;
+            dl[k] = dl[k] + f;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const zd f = F.a[k];
+            if constexpr (PL == 1) {
+                X[k] = X[k] - f;
+                Y[k] = Y[k] + f;
+            } else if constexpr (PL == 4) {
+                X[k] = X[k] - f;
+                Y[k] = Y[k] - f;
+            } else if constexpr (PL == 2) {
+                X[k] = X[k] + ti(f);
+                Y[k] = Y[k] + ti(f);
+            } else {
+                X[k] = X[k] - ti(f);
+                Y[k] = Y[k] + ti(f);
+            }
+        }
+    }
+}
+
+// the packed hermitian 6x6 of spin blocks (0,0) = d, (1,1) = -d, (1,0) = X:
+// diagonal pairs, then the lower entries row by row (the store_herm6 order)
+__device__ __forceinline__ void store_spin_blocks(zd* base, int stride, int idx, const zd d[6], const zd X[9]) {
+    base[idx] = mk(d[0].re, d[1].re);
+    base[(size_t)stride + idx] = mk(d[2].re, -d[0].re);
+    base[(size_t)2 * stride + idx] = mk(-d[1].re, -d[2].re);
+    const zd e[15] = {d[3],  d[4], d[5],                                     // rows 1, 2
+                      X[0],  X[1], X[2],                                     // row 3
+                      X[3],  X[4], X[5], -d[3],                              // row 4
+                      X[6],  X[7], X[8], -d[4], -d[5]};                      // row 5
+#pragma unroll
+    for (int k = 0; k < 15; ++k) base[(size_t)(3 + k) * stride + idx] = e[k];
+}
+
+// One thread per site.  Each XCD takes a contiguous range of site blocks with
+// the two parities of a block interleaved, so the links a site's leaves share
+// with its neighbours (both parities) are fetched once into that XCD's L2;
+// dealt round-robin, neighbouring blocks ran on different XCDs and every XCD
+// fetched them again.
 template <int LK>
 __global__ __launch_bounds__(128) void k_clover_B(Geom g, const void* links, zd* B) {
-    const int nbp = (g.Vh + 127) / 128;
-    int p = blockIdx.x / nbp;
-    int idx = (blockIdx.x % nbp) * 128 + threadIdx.x;
+    const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    const int p = g.tsh ? (lb & 1) : 0;
+    const int idx = (g.tsh ? lb >> 1 : lb) * 128 + (int)threadIdx.x;
     if (idx >= g.Vh) return;
     int c[4];
     eo_coords(g, p, idx, c);
-    const int pm[6] = {0, 0, 0, 1, 1, 2}, pn[6] = {1, 2, 3, 2, 3, 3};
-    zd hu[36], hl[36];
-    for (int k = 0; k < 36; ++k) hu[k] = hl[k] = mk(0.0, 0.0);
-    for (int pl = 0; pl < 6; ++pl) {
-        M3 F = field_strength<LK>(g, links, c, pm[pl], pn[pl]);
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) {
-                zd f = F.a[3 * i + j];
-                // upper block: rows/cols 3*s + colour, s = spin 0, 1
-                // lower block: s = spin 2, 3 (local 0, 1)
-                switch (pl) {
-                    case 0:  // F0: i(-F0) on (0,0), i(+F0) on (1,1); lower i(+F0) (2,2), i(-F0) (3,3)
-                        hu[6 * i + j] = hu[6 * i + j] - ti(f);
-                        hu[6 * (3 + i) + 3 + j] = hu[6 * (3 + i) + 3 + j] + ti(f);
-                        hl[6 * i + j] = hl[6 * i + j] + ti(f);
-                        hl[6 * (3 + i) + 3 + j] = hl[6 * (3 + i) + 3 + j] - ti(f);
-                        break;
-                    case 5:  // F5: +i on (0,0), -i on (1,1); lower +i (2,2), -i (3,3)
-                        hu[6 * i + j] = hu[6 * i + j] + ti(f);
-                        hu[6 * (3 + i) + 3 + j] = hu[6 * (3 + i) + 3 + j] - ti(f);
-                        hl[6 * i + j] = hl[6 * i + j] + ti(f);
-                        hl[6 * (3 + i) + 3 + j] = hl[6 * (3 + i) + 3 + j] - ti(f);
-                        break;
-                    case 1:  // F1: +1 (0,1), -1 (1,0); lower -1 (2,3), +1 (3,2)
-                        hu[6 * i + 3 + j] = hu[6 * i + 3 + j] + f;
-                        hu[6 * (3 + i) + j] = hu[6 * (3 + i) + j] - f;
-                        hl[6 * i + 3 + j] = hl[6 * i + 3 + j] - f;
-                        hl[6 * (3 + i) + j] = hl[6 * (3 + i) + j] + f;
-                        break;
-                    case 4:  // F4: +1 (0,1), -1 (1,0); lower +1 (2,3), -1 (3,2)
-                        hu[6 * i + 3 + j] = hu[6 * i + 3 + j] + f;
-                        hu[6 * (3 + i) + j] = hu[6 * (3 + i) + j] - f;
-                        hl[6 * i + 3 + j] = hl[6 * i + 3 + j] + f;
-                        hl[6 * (3 + i) + j] = hl[6 * (3 + i) + j] - f;
-                        break;
-                    case 2:  // F2: +i on all four off-diagonal spin blocks
-                        hu[6 * i + 3 + j] = hu[6 * i + 3 + j] + ti(f);
-                        hu[6 * (3 + i) + j] = hu[6 * (3 + i) + j] + ti(f);
-                        hl[6 * i + 3 + j] = hl[6 * i + 3 + j] + ti(f);
-                        hl[6 * (3 + i) + j] = hl[6 * (3 + i) + j] + ti(f);
-                        break;
-                    default:  // F3: -i upper off-diagonal, +i lower off-diagonal
-                        hu[6 * i + 3 + j] = hu[6 * i + 3 + j] - ti(f);
-                        hu[6 * (3 + i) + j] = hu[6 * (3 + i) + j] - ti(f);
-                        hl[6 * i + 3 + j] = hl[6 * i + 3 + j] + ti(f);
-                        hl[6 * (3 + i) + j] = hl[6 * (3 + i) + j] + ti(f);
-                        break;
-                }
-            }
-    }
+    // The two 6x6 blocks (spins 0,1 and 2,3; rows 3 s + colour) as the 3x3
+    // spin blocks they are made of: upper (0,0) = i(F5 - F0), (1,1) = -(0,0),
+    // (1,0) = -F1 + iF2 - iF3 - F4; lower (0,0) = i(F0 + F5), (1,1) = -(0,0),
+    // (1,0) = F1 + iF2 + iF3 - F4 (planes 0..5 = 01 02 03 12 13 23), summed in
+    // plane order.  The (0,1) blocks are the adjoints of (1,0) and are not stored.
+    zd du[6], dl[6], X[9], Y[9];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) du[k] = dl[k] = mk(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) X[k] = Y[k] = mk(0.0, 0.0);
+    acc_plane<0>(field_strength<LK>(g, links, c, 0, 1), du, dl, X, Y);
+    acc_plane<1>(field_strength<LK>(g, links, c, 0, 2), du, dl, X, Y);
+    acc_plane<2>(field_strength<LK>(g, links, c, 0, 3), du, dl, X, Y);
+    acc_plane<3>(field_strength<LK>(g, links, c, 1, 2), du, dl, X, Y);
+    acc_plane<4>(field_strength<LK>(g, links, c, 1, 3), du, dl, X, Y);
+    acc_plane<5>(field_strength<LK>(g, links, c, 2, 3), du, dl, X, Y);
     zd* base = B + (size_t)p * 36 * g.stride;
-    store_herm6(base, g.stride, idx, hu);
-    store_herm6(base + (size_t)18 * g.stride, g.stride, idx, hl);
+    store_spin_blocks(base, g.stride, idx, du, X);
+    store_spin_blocks(base + (size_t)18 * g.stride, g.stride, idx, dl, Y);
 }
 
 // A = 1 + kc gamma5 B: upper block 1 + kc B_u, lower block 1 - kc B_l
