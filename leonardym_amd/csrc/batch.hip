@@ -213,15 +213,17 @@ __global__ __launch_bounds__(BC * NR) void k_bcg_dot(const BState* s, const type
 }
 
 // chunk sums of one value per (site, RHS) written by a fused stencil launch
-// (bsite[j * NR + q]); lanes that are done contribute zero, as in k_bcg_dot
+// (bsite[j * NR + q]); lanes that are done contribute zero, as in k_bcg_dot.
+// Only the vh real sites of each parity count: the stencil never writes the
+// padding slots up to the stride, which may hold values a wider batch left.
 template <int NR>
 __global__ __launch_bounds__(BC * NR) void k_bcg_chunksum(const BState* s, const double* bsite, long nsites,
-                                                          double* partials, long nchunks) {
+                                                          int stride, int vh, double* partials, long nchunks) {
     if (s->all_done) return;
     long j;
     int q;
     double v = 0;
-    if (chunk_pair<NR>(nsites, j, q) && !s->done[q]) v = bsite[j * NR + q];
+    if (chunk_pair<NR>(nsites, j, q) && !s->done[q] && j % stride < vh) v = bsite[j * NR + q];
     chunk_store<NR>(v, partials, nchunks);
 }
 
@@ -516,11 +518,13 @@ static void bcg_step_impl(int which, BState* s, const BcgArgs& a, hipStream_t st
             hipLaunchKernelGGL((k_bcg_p<PR, NR>), dim3(eb), dim3(256), 0, st, s, p, a.p_sc, a.rs, a.nsites, a.stride);
             break;
         case 7:
-            hipLaunchKernelGGL((k_bcg_chunksum<NR>), cg, cb, 0, st, s, a.bsite, a.nsites, a.partials, nch);
+            hipLaunchKernelGGL((k_bcg_chunksum<NR>), cg, cb, 0, st, s, a.bsite, a.nsites, a.stride, a.vh, a.partials,
+                               nch);
             hipLaunchKernelGGL((k_bcg_alpha_final<NR>), dim3(1), fb, 0, st, s, a.partials, nch);
             break;
         case 8:
-            hipLaunchKernelGGL((k_bcg_chunksum<NR>), cg, cb, 0, st, s, a.bsite, a.nsites, a.partials, nch);
+            hipLaunchKernelGGL((k_bcg_chunksum<NR>), cg, cb, 0, st, s, a.bsite, a.nsites, a.stride, a.vh, a.partials,
+                               nch);
             hipLaunchKernelGGL((k_bcg_update_final<NR>), dim3(1), fb, 0, st, s, a.partials, nch);
             break;
         default:
@@ -586,6 +590,28 @@ void launch_relayout(int esize, const void* in, void* out, long rows, int w_old,
         default:
             hipLaunchKernelGGL(k_relayout<float>, g, b, 0, st, (const float*)in, (float*)out, rows, w_old, w_new);
             break;
+    }
+}
+// padding slots of an nr-field: sites vh..stride-1 of each of `groups` groups
+// of stride * nr elements (lane fastest)
+template <class E>
+__global__ void k_zero_padding(E* buf, long groups, int stride, int vh, int nr) {
+    const long per = (long)(stride - vh) * nr;
+    const E z{};
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < groups * per; t += (long)gridDim.x * blockDim.x) {
+        const long g = t / per;
+        buf[(g * stride + vh) * nr + (t - g * per)] = z;
+    }
+}
+void launch_zero_padding(int esize, void* buf, long groups, int stride, int vh, int nr, hipStream_t st) {
+    if (!buf || stride <= vh) return;
+    const dim3 g(bsite_blocks(groups * (stride - vh) * nr)), b(256);
+    switch (esize) {
+        case 16: hipLaunchKernelGGL(k_zero_padding<V16>, g, b, 0, st, (V16*)buf, groups, stride, vh, nr); break;
+        case 8:
+            hipLaunchKernelGGL(k_zero_padding<cplx<float>>, g, b, 0, st, (cplx<float>*)buf, groups, stride, vh, nr);
+            break;
+        default: hipLaunchKernelGGL(k_zero_padding<float>, g, b, 0, st, (float*)buf, groups, stride, vh, nr); break;
     }
 }
 void launch_clover_apply_lanes(const Geom& g, const zd* blocks, long lane_elems, const int* lane_set, const int* mask,

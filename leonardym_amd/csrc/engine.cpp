@@ -261,6 +261,19 @@ int set_width(LaneEngine* e, int w) {
         e->Sh->rel[q] = 0;
     }
     e->width = w;
+    // padding slots of the new layout to zero (no launch writes them)
+    const int vh = e->ctx->g.Vh;
+    zd* full[] = {e->B, e->X};
+    zd* half[] = {e->x, e->r, e->Ax, e->mid, e->tmp, e->rhs, e->src, e->te};
+    for (zd* f : full) lym::launch_zero_padding(16, f, 24, (int)S, vh, w, st);
+    for (zd* f : half) lym::launch_zero_padding(16, f, 12, (int)S, vh, w, st);
+    for (int i = 0; i < 4; ++i) {
+        lym::launch_zero_padding(16, e->sv[i], G, (int)S, vh, w, st);
+        lym::launch_zero_padding(4, e->svsc[i], 1, (int)S, vh, w, st);
+    }
+    lym::launch_zero_padding(8, e->xs, 12, (int)S, vh, w, st);
+    lym::launch_zero_padding(8, e->rs, 12, (int)S, vh, w, st);
+    ECHECK(hipGetLastError());
     return write_state(e);
 }
 

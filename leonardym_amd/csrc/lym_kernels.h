@@ -240,6 +240,7 @@ struct BcgArgs {
     double* partials;  // BMAX * ceil(nsites / BATCH_CHUNK) doubles
     const int* mask = nullptr;  // steps 0 (init) and 6 (final accumulate): only lanes with mask[q] != 0
     const double* bsite = nullptr;  // steps 7, 8: one value per (site, RHS), site-major
+    int vh = 0;                     // steps 7, 8: the real sites of each parity (of stride)
 };
 // the fused batch steps on one stencil launch (DslashParams::bmode)
 struct BFuse {
@@ -263,6 +264,11 @@ void launch_unpack(const zd* multi, int nr, zd* const* dsts_dev, long n, hipStre
 void launch_lane_copy(int esize, void* dst, const void* src, long rows, int nr, const int* mask, hipStream_t st);
 void launch_lane_sub(zd* out, const zd* a, const zd* b, long rows, int nr, const int* mask, hipStream_t st);
 void launch_relayout(int esize, const void* in, void* out, long rows, int w_old, int w_new, hipStream_t st);
+// zero the padding slots (sites vh..stride-1) of an nr-field of `groups` groups
+// of stride * nr elements of esize (16, 8 or 4) bytes.  No launch writes them,
+// and a buffer reused at another width would otherwise hold, there, values of
+// real sites of the old layout, which per-RHS sums and copies read.
+void launch_zero_padding(int esize, void* buf, long groups, int stride, int vh, int nr, hipStream_t st);
 void launch_clover_apply_nr(const Geom& g, const zd* blocks, const zd* in, zd* out, int nr, hipStream_t st);
 // the same with lane r's blocks at blocks + r * lane_elems, only for lanes with mask[r] != 0
 // lane r applies block set lane_set[r] (lane_set NULL: set r)

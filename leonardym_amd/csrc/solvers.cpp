@@ -829,6 +829,19 @@ int run_batch(lym_solver* s, lym_op* op, bool eo, bool half, int nr, int thr, in
     const long nfull = 24 * S * nr, ndom = 12 * S * dom * nr;
     zd *B = s->bw[0], *X = s->bw[1], *x = s->bw[2], *r = s->bw[3], *Ax = s->bw[4], *mid = s->bw[5],
        *tmp = s->bw[6], *rhs = s->bw[7], *src = s->bw[8], *te = s->bw[9];
+    // padding slots of this width's layout to zero (the buffers may have held
+    // another width; lattices whose parity fills its stride have none)
+    {
+        const int vh = c->g.Vh, G = prec == 1 ? 6 : 3;
+        for (int i = 0; i < 10; ++i) lym::launch_zero_padding(16, s->bw[i], i < 2 ? 24 : 12 * dom, (int)S, vh, nr, st);
+        for (int i = 0; i < 4; ++i) {
+            lym::launch_zero_padding(16, s->bsv[i], G * dom, (int)S, vh, nr, st);
+            lym::launch_zero_padding(4, s->bsvsc[i], dom, (int)S, vh, nr, st);
+        }
+        lym::launch_zero_padding(8, s->bxs, 12 * dom, (int)S, vh, nr, st);
+        lym::launch_zero_padding(8, s->brs, 12 * dom, (int)S, vh, nr, st);
+        lym::launch_zero_padding(8, s->bsite, dom, (int)S, vh, nr, st);
+    }
     // pack sources
     for (int q = 0; q < nr; ++q) s->bptr_host[q] = q < nb ? reinterpret_cast<zd*>(sources[q]->d) : s->bzero;
     HIPCHECK(hipMemcpyAsync(s->bptr_dev, s->bptr_host, sizeof(zd*) * nr, hipMemcpyHostToDevice, st));
@@ -876,6 +889,7 @@ int run_batch(lym_solver* s, lym_op* op, bool eo, bool half, int nr, int thr, in
     a.max_steps = (int)s->max_steps;
     a.partials = s->bpartials;
     a.bsite = s->bsite;
+    a.vh = c->g.Vh;
     if (!lym::launch_bcg_step(0, prec, nr, Sdev, a, st)) return fail(LYM_ERR_UNSUPPORTED, "batch of %d RHS", nr);
     HIPCHECK(hipMemcpyAsync(Sh, Sdev, sizeof(lym::BState), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));

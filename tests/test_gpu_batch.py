@@ -134,7 +134,7 @@ def test_fused_batch_steps_against_separate_launches(case, sloppy, monkeypatch):
     (|Qhat p|^2 per site from the launch writing Qhat p; x_s, r_s and |r_s|^2
     from the last launch's unrounded output) against the separate dot and
     update launches: both reach the FP64 stopping rule, the solutions agree to
-    the solve's tolerance, the iteration counts to a few per cent, and the
+    the solve's tolerance, the iteration counts to about ten per cent, and the
     fused form gives the same bits in batches of 4 and 1."""
     srcs = [case.gauss() for _ in range(5)]
     eps = [(1e-11 * np.linalg.norm(b)) ** 2 for b in srcs]
@@ -148,9 +148,63 @@ def test_fused_batch_steps_against_separate_launches(case, sloppy, monkeypatch):
     for q, b in enumerate(srcs):
         assert inf_sep[q].converged and inf4[q].converged and inf1[q].converged
         assert inf4[q].steps == inf1[q].steps and np.array_equal(xs4[q], xs1[q])
-        assert abs(int(inf4[q].steps) - int(inf_sep[q].steps)) <= max(3, 0.05 * inf_sep[q].steps)
+        # the two forms round differently (unrounded FP32 sums against stored
+        # FP16 values), so the counts differ by a few iterations
+        assert abs(int(inf4[q].steps) - int(inf_sep[q].steps)) <= max(6, 0.1 * inf_sep[q].steps)
         assert true_residual(case, xs4[q], b) <= 1e-10
         assert np.linalg.norm((xs4[q] - xs_sep[q]).ravel()) / np.linalg.norm(xs_sep[q].ravel()) < 1e-9
+
+
+def test_fused_batch_steps_on_padded_parities():
+    """6^3x8 has 864 sites per parity in a stride of 896: the fused steps' sums
+    must skip the padding slots, where a wider batch (4 RHS) left values that
+    the ragged batch after it (3 RHS) would otherwise add in.  Every RHS gives
+    the same bits in batches of 4 (+3) as alone and meets the FP64 stopping
+    rule."""
+    case = Case(dims=(6, 6, 6, 8))
+    assert lym.launch_plan(case.dims, 0)["nbpp"] * 256 >= 864
+    srcs = [case.gauss() for _ in range(7)]
+    eps = [(1e-11 * np.linalg.norm(b)) ** 2 for b in srcs]
+    op = case.op(lym.OP_CLOVER)
+    xs4, inf4, _ = solve_batch(case, op, lym.SOLVER_PRECOND_MIXED_CG, srcs, 4, eps)
+    xs1, inf1, _ = solve_batch(case, op, lym.SOLVER_PRECOND_MIXED_CG, srcs, 1, eps)
+    for q, b in enumerate(srcs):
+        assert inf4[q].converged and inf1[q].converged
+        assert inf4[q].steps == inf1[q].steps and np.array_equal(xs4[q], xs1[q]), q
+        assert true_residual(case, xs4[q], b) <= 1e-10
+
+
+def test_kappa_lanes_on_padded_parities():
+    """The serving engine's lanes on 6^3x8 (padded parities): a run of 7
+    sources widens the batch as they are admitted (the fields are relaid at
+    each width); every source gives the same bits alone and in the mix and
+    meets the FP64 stopping rule at its kappa."""
+    from helpers import OracleField, fold_bc
+    dims = (6, 6, 6, 8)
+    case = Case(dims=dims)
+    U = fold_bc(dims, lym.gen_su3_links(dims, 0.5, 11), -1)
+    kap = [0.120, 0.125, 0.130, 0.120, 0.125, 0.130, 0.1275]
+    srcs = [case.gauss() for _ in kap]
+    eps = [(1e-11 * np.linalg.norm(b)) ** 2 for b in srcs]
+    op = case.op(lym.OP_CLOVER)
+
+    def run(idx, block=0):
+        solver = lym.Solver(case.ctx, lym.SOLVER_PRECOND_MIXED_CG, max_steps=20000, sloppy=lym.PREC_HALF)
+        solver.set_rhs_block(block)
+        B = [case.spinor(srcs[q]) for q in idx]
+        X = [case.spinor() for _ in idx]
+        infos = solver.solve_batch_kappas(op, [kap[q] for q in idx], B, X, [eps[q] for q in idx])
+        return [x.download() for x in X], infos
+
+    xs, infos = run(list(range(len(kap))))
+    for q in (0, 4, 6):
+        alone, ia = run([q])
+        assert ia[0].steps == infos[q].steps and np.array_equal(alone[0], xs[q]), q
+    for q, k in enumerate(kap):
+        assert infos[q].converged, q
+        orc = OracleField(dims, U, kappa=k, csw=1.0, need_clover=True, need_inverse=False)
+        Dx = orc.apply(K_CLOVER, xs[q], gamma5=False)
+        assert np.linalg.norm((srcs[q] - Dx).ravel()) / np.linalg.norm(srcs[q].ravel()) <= 1e-10
 
 
 def test_point_sources_16x16x16x32():
