@@ -239,7 +239,6 @@ int force_params(lym_op* op, const lym_spinor* X, const lym_spinor* Y, const dou
     if (op->kind != LYM_OP_WILSON && op->kind != LYM_OP_BASIC_WILSON)
         return fail(LYM_ERR_UNSUPPORTED, "the fermion force is DiracWilsonFermionForce's: Wilson operators only");
     if (!op->gauge) return fail(LYM_ERR_INVALID, "operator has no gauge field (call lym_op_set_gauge)");
-    if (c->g.L[3] % 2) return fail(LYM_ERR_INVALID, "Lt must be even");
     const int N = c->rep == LYM_REP_SU3_FUND ? 3 : 2;
     nn = N * N;
     a.g = c->g;

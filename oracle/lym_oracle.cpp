@@ -828,11 +828,15 @@ done:
     return ok;
 }
 
-// PreconditionedBiCGStab::solve, even-odd branch (PreconditionedBiCGStab.cpp:33-100).
-// d must describe the clover operator (kind 2) with F and inv (built with the
-// same kappa, csw).  Returns x = D^-1 b.
+// PreconditionedBiCGStab::solve (PreconditionedBiCGStab.cpp:33-100).  d must
+// describe the clover operator (kind 2) with F and inv (built with the same
+// kappa, csw).  With all four extents even, the even-odd branch: returns
+// x = D^-1 b.  Otherwise plain BiCGStab on the operator as d describes it
+// (the reference's fallback, :43).
 int orc_precond_bicgstab(const orc_op_desc* d, const double* bsrc, double* xout, const double* x0,
                          double precision, unsigned max_steps, orc_solve_info* info) {
+    if ((d->dims[0] | d->dims[1] | d->dims[2] | d->dims[3]) % 2)
+        return orc_bicgstab(d, bsrc, xout, x0, precision, max_steps, info);
     Op eo = make_op(d->dims, K_EO_CLOVER, 1, d->kappa, d->csw, 1, d->links, d->F, d->inv);
     const Geo& g = *eo.g;
     const long n = g.V * 12;
@@ -865,8 +869,9 @@ int orc_precond_bicgstab(const orc_op_desc* d, const double* bsrc, double* xout,
 // MesonCorrelator::execute, connected part (MesonCorrelator.cpp:22-199).  The
 // 12 unit sources at `site` (StochasticEstimator.cpp:149-161, there the
 // origin) are solved with PreconditionedBiCGStab on D (gamma5 off, :71): the
-// even-odd driver for the clover operator (kind 2), plain BiCGStab otherwise
-// (PreconditionedBiCGStab.cpp:43 falls back to it); the propagator column of
+// even-odd driver for the clover operator (kind 2) on a lattice with all
+// extents even, plain BiCGStab otherwise (PreconditionedBiCGStab.cpp:43 falls
+// back to it); the propagator column of
 // source (alpha, c) is c * 4 + alpha (:94-98).  The pion (:104-118), scalar
 // (:121-139) and pseudoscalar-axial (:142-160) contractions are summed per
 // timeslice in long double, normalised by 4 kappa^2 (:162), with the PCAC

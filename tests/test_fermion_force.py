@@ -135,3 +135,26 @@ def test_device_two_flavor_force_end_to_end():
     X, _, _ = orc.solve("bicgstab", Y, 1e-26, kind=K_WILSON)
     want = oracle_fermion_force(dims, 0, KAPPA, W, U, X, Y)
     assert np.max(np.abs(got - want)) < 1e-9 * np.max(np.abs(want))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dims", [(3, 4, 4, 6), (4, 4, 4, 5)], ids=lambda d: "x".join(map(str, d)))
+def test_device_two_flavor_force_odd_extent(dims):
+    """A lattice with an odd extent (one site class on the device): the two
+    FP64 solves and the force against the oracle's."""
+    print("two-flavour force: solves + force on", dims, flush=True)
+    v = int(np.prod(dims))
+    U = lym.gen_su3_links(dims, 0.5, 95)
+    W = fold_bc(dims, U, -1)
+    phi = lym.gaussian_spinor(v, 96)
+    ctx = lym.Context(dims, lym.REP_SU3_FUND, -1)
+    gauge = lym.Gauge(ctx, W, folded=True)
+    op = lym.Operator(ctx, lym.OP_WILSON, 1, KAPPA, 0.0, True, gauge=gauge)
+    solver = lym.Solver(ctx, lym.SOLVER_BICGSTAB, precision=1e-26, max_steps=2000)
+    got, infos = solver.two_flavor_force(op, lym.Spinor(ctx, phi))
+    assert infos[0].converged and infos[1].converged
+    orc = OracleField(dims, W, kappa=KAPPA)
+    Y, _, _ = orc.solve("bicgstab", phi, 1e-26, kind=K_WILSON)
+    X, _, _ = orc.solve("bicgstab", Y, 1e-26, kind=K_WILSON)
+    want = oracle_fermion_force(dims, 0, KAPPA, W, U, X, Y)
+    assert np.max(np.abs(got - want)) < 1e-9 * np.max(np.abs(want))

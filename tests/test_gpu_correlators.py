@@ -68,3 +68,21 @@ def test_wilson_correlators_match_oracle():
     # a solver that does not return D^-1 b is refused
     with pytest.raises(lym.LymError):
         lym.meson_correlators(lym.Solver(ctx, lym.SOLVER_CG), op, site)
+
+
+def test_correlators_on_an_odd_extent():
+    """A lattice with odd spatial extents and odd Lt (one site class on the
+    device): the point source, the 12 FP64 BiCGStab solves and the timeslice
+    contractions against the oracle."""
+    dims, kappa, csw = (5, 4, 3, 7), 0.12, 1.0
+    site = (4, 1, 2, 6)
+    U = fold_bc(dims, lym.gen_su3_links(dims, 0.4, 1002))
+    ctx = lym.Context(dims)
+    g = lym.Gauge(ctx, U, folded=True)
+    op = lym.Operator(ctx, lym.OP_CLOVER, 1, kappa, csw, True, gauge=g)
+    s = lym.Solver(ctx, lym.SOLVER_BICGSTAB, precision=1e-24, max_steps=5000)
+    got = lym.meson_correlators(s, op, site)
+    assert all(i.converged for i in got["infos"])
+    want = OracleField(dims, U, kappa=kappa, csw=csw, need_clover=True).meson_correlators(site, 1e-24)
+    assert want["ok"]
+    compare(got, want, 1e-9)
