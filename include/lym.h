@@ -18,8 +18,10 @@
  * operators, the even-odd driver, low-precision applies, mixed-precision,
  * batched and served solves need all four extents even
  * (source/inverters/PreconditionedBiCGStab.cpp:43); on a lattice with an odd
- * extent the full-lattice FP64 operators, multiplyAdd and the FP64 CG and
- * BiCGStab solvers run, and the rest returns LYM_ERR_UNSUPPORTED.
+ * extent the full-lattice FP64 operators, multiplyAdd, the FP64 CG and
+ * BiCGStab solvers (the preconditioned driver falls back to plain BiCGStab),
+ * the meson correlators and the fermion force run, and the rest returns
+ * LYM_ERR_UNSUPPORTED.
  */
 #ifndef LYM_H
 #define LYM_H
