@@ -1055,6 +1055,19 @@ static void launch_one(const DslashParams& a, int nblocks, hipStream_t st) {
         }
     }
     if (a.lane_ops) {  // per-lane operators: the serving engine's batches (never split, no output norms)
+        if constexpr (PR >= 1 && DIAG >= 1 && !INV) {
+            // with the batched solver's fused steps
+            if (a.bmode == 1) {
+                hipLaunchKernelGGL((k_dslash<PR, LK, DIAG, INV, false, true, false, 1>), dim3(nblocks),
+                                   dim3(a.spb * a.nr), 0, st, a);
+                return;
+            }
+            if (a.bmode == 2) {
+                hipLaunchKernelGGL((k_dslash<PR, LK, DIAG, INV, false, true, false, 2>), dim3(nblocks),
+                                   dim3(a.spb * a.nr), 0, st, a);
+                return;
+            }
+        }
         hipLaunchKernelGGL((k_dslash<PR, LK, DIAG, INV, false, true>), dim3(nblocks), dim3(a.spb * a.nr), 0, st, a);
         return;
     }

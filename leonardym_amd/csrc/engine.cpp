@@ -81,6 +81,8 @@ struct LaneEngine {
     float* svsc[4] = {};
     lym::cplx<float> *xs = nullptr, *rs = nullptr;
     double* partials = nullptr;
+    double* bsite = nullptr;    // the fused steps' per-(site, lane) terms
+    bool fused = true;          // the batched solver's fused steps (LYM_FUSED_BATCH=0: separate launches)
     lym::BState* Sd = nullptr;  // device state
     lym::BState* Sh = nullptr;  // pinned host copy
     int* mask_d = nullptr;      // per-lane mask of the admission / harvest launches
@@ -161,9 +163,12 @@ lym::DslashParams base(LaneEngine* e, int prec) {
 }
 
 // Schur complement with gamma5 (the even-odd driver's Qhat) on odd-half
-// nr-fields; lanes whose flag equals skip_value do nothing
+// nr-fields; lanes whose flag equals skip_value do nothing.  bmode: the
+// batched solver's fused steps on the second launch (1: also |Qhat in|^2 per
+// (site, lane) into e->bsite; 2: x_s += alpha p, r_s -= alpha Qhat in with
+// alpha per lane from the state, |r_s|^2 per (site, lane), no output).
 void schur(LaneEngine* e, int prec, void* out, float* out_sc, const void* in, const float* in_sc, void* tmp_e,
-           float* tmp_sc, const int* skip, const int* flags, int skip_value) {
+           float* tmp_sc, const int* skip, const int* flags, int skip_value, int bmode = 0) {
     const int lk = prec == 0 ? e->lk64 : e->lks;
     lym::DslashParams a = base(e, prec);
     a.skip = skip;
@@ -185,6 +190,17 @@ void schur(LaneEngine* e, int prec, void* out, float* out_sc, const void* in, co
     b.out_sc[1] = out_sc;
     b.hop_sign = -1.0;
     b.gamma5 = 1;
+    if (bmode) {
+        b.bmode = bmode;
+        b.bsite[1] = e->bsite;
+        if (bmode == 2) {
+            b.balpha = &e->Sd->alpha[0];
+            b.upd_p[1] = e->sv[0];
+            b.upd_p_sc[1] = e->svsc[0];
+            b.upd_xs[1] = e->xs;
+            b.upd_rs[1] = e->rs;
+        }
+    }
     lym::launch_dslash(b, lk, e->csw != 0.0 ? 2 : 1, false, e->ctx->stream);
 }
 
@@ -217,6 +233,8 @@ lym::BcgArgs bcg_args(LaneEngine* e, const int* mask) {
     a.max_steps = (int)e->max_steps;
     a.partials = e->partials;
     a.mask = mask;
+    a.bsite = e->bsite;
+    a.vh = e->ctx->g.Vh;
     return a;
 }
 
@@ -273,6 +291,7 @@ int set_width(LaneEngine* e, int w) {
     }
     lym::launch_zero_padding(8, e->xs, 12, (int)S, vh, w, st);
     lym::launch_zero_padding(8, e->rs, 12, (int)S, vh, w, st);
+    lym::launch_zero_padding(8, e->bsite, 1, (int)S, vh, w, st);
     ECHECK(hipGetLastError());
     return write_state(e);
 }
@@ -283,9 +302,18 @@ void iteration(LaneEngine* e, int it) {
     lym::BcgArgs a = bcg_args(e, nullptr);
     const int nr = e->width, prec = e->prec;
     // lanes that are done skip the sloppy apply; only lanes with a due update take the FP64 one
-    schur2(e, prec, a.Ap, a.Ap_sc, a.p, a.p_sc, &Sd->all_done, Sd->done, 1);
-    lym::launch_bcg_step(1, prec, nr, Sd, a, st);
-    lym::launch_bcg_step(2, prec, nr, Sd, a, st);
+    if (e->fused && prec >= 1) {
+        // <p, A p> = |Qhat p|^2 and the update fused into the stencil launches
+        // (as the batched solver's even-odd batches do)
+        schur(e, prec, e->sv[2], e->svsc[2], a.p, a.p_sc, e->sv[3], e->svsc[3], &Sd->all_done, Sd->done, 1, 1);
+        lym::launch_bcg_step(7, prec, nr, Sd, a, st);
+        schur(e, prec, nullptr, nullptr, e->sv[2], e->svsc[2], e->sv[3], e->svsc[3], &Sd->all_done, Sd->done, 1, 2);
+        lym::launch_bcg_step(8, prec, nr, Sd, a, st);
+    } else {
+        schur2(e, prec, a.Ap, a.Ap_sc, a.p, a.p_sc, &Sd->all_done, Sd->done, 1);
+        lym::launch_bcg_step(1, prec, nr, Sd, a, st);
+        lym::launch_bcg_step(2, prec, nr, Sd, a, st);
+    }
     lym::launch_bcg_step(5, prec, nr, Sd, a, st);
     if (rel_slot(it)) {
         lym::launch_bcg_step(3, prec, nr, Sd, a, st);
@@ -310,7 +338,7 @@ size_t engine_bytes(const int dims[4], int rep, int storage, int prec, int wmax)
     const double W = next_width(wmax);
     double b = 0;
     b += 2 * 24 * S * W * 16 + 8 * 12 * S * W * 16;  // FP64 nr-fields
-    b += 2 * 12 * S * W * 8;                         // x_s, r_s
+    b += 2 * 12 * S * W * 8 + S * W * 8;             // x_s, r_s, the fused steps' terms
     b += 4 * (12 * S * W * es + S * W * 4);          // sloppy nr-fields
     b += W * (72 * S * 16 + 36 * S * 16 + 72 * S * es + 36 * S * es + 3 * S * 4);  // operator data per lane
     b += 72 * S * le;                                // low-precision links
@@ -377,6 +405,11 @@ int engine_create(lym_ctx* ctx, const lym_gauge* g, double csw, int prec, unsign
     }
     const size_t nch = ((size_t)S + lym::BATCH_CHUNK - 1) / lym::BATCH_CHUNK;
     A((void**)&e->partials, sizeof(double) * lym::BMAX * nch);
+    A((void**)&e->bsite, sizeof(double) * S * W);
+    {
+        const char* fb = std::getenv("LYM_FUSED_BATCH");
+        e->fused = !(fb && fb[0] == '0');
+    }
     A((void**)&e->Sd, sizeof(lym::BState));
     A((void**)&e->mask_d, sizeof(int) * lym::BMAX);
     A((void**)&e->flag_d, sizeof(int) * 2);
