@@ -211,11 +211,13 @@ int lym_solver_solve_batch(lym_solver* s, lym_op* op, unsigned nrhs, const lym_s
  * mixed-precision CG driver of a clover operator `op` (power 1; its gauge
  * field and csw, kappa per RHS) to |r|^2 < precisions[q] on the even-odd
  * system (NULL: the solver's precision), as lym_solver_solve_batch does.  The
- * RHS are lanes of one batch (width: the RHS block, 0 = 8), each with its own
- * kappa, clover blocks and stopping point, in one stencil launch per apply;
- * an RHS that stops frees its lane for the next one.  A result does not
- * depend on the other RHS or the batch width.  Solver kind
- * LYM_SOLVER_PRECOND_MIXED_CG. */
+ * RHS are lanes of one batch (width: the RHS block; 0 = 12 for 12 or more
+ * RHS, else 8), each with its own kappa, clover blocks and stopping point, in
+ * one stencil launch per apply; an RHS that stops frees its lane for the next
+ * one, and RHS enter in order of kappa so that lanes of one kappa share their
+ * clover blocks.  A result does not depend on the other RHS, the order of
+ * admission or the batch width; a kappa that is not finite is an
+ * LYM_ERR_INVALID.  Solver kind LYM_SOLVER_PRECOND_MIXED_CG. */
 int lym_solver_solve_batch_kappas(lym_solver* s, lym_op* op, unsigned nrhs, const double* kappas,
                                   const lym_spinor* const* sources, lym_spinor* const* solutions,
                                   const double* precisions, lym_solve_info* infos);
