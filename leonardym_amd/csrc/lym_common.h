@@ -80,17 +80,26 @@ __host__ __device__ __forceinline__ cplx<T> cmulc(cplx<T> a, cplx<T> b) {
 }
 template <class T>
 __host__ __device__ __forceinline__ cplx<T> conj(cplx<T> a) { return {a.re, -a.im}; }
-// a += b * c
+// one fused multiply-add (a single rounding)
+__host__ __device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__host__ __device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// a += b * c, as two fused multiply-adds per real component: four
+// instructions per complex product instead of six (a multiply, a fused
+// multiply-add and an add per component), and one rounding fewer
 template <class T>
 __host__ __device__ __forceinline__ void cfma(cplx<T>& a, cplx<T> b, cplx<T> c) {
-    a.re += b.re * c.re - b.im * c.im;
-    a.im += b.re * c.im + b.im * c.re;
+    a.re = fmadd(b.re, c.re, a.re);
+    a.re = fmadd(-b.im, c.im, a.re);
+    a.im = fmadd(b.re, c.im, a.im);
+    a.im = fmadd(b.im, c.re, a.im);
 }
-// a += conj(b) * c
+// a += conj(b) * c, likewise
 template <class T>
 __host__ __device__ __forceinline__ void cfmac(cplx<T>& a, cplx<T> b, cplx<T> c) {
-    a.re += b.re * c.re + b.im * c.im;
-    a.im += b.re * c.im - b.im * c.re;
+    a.re = fmadd(b.re, c.re, a.re);
+    a.re = fmadd(b.im, c.im, a.re);
+    a.im = fmadd(b.re, c.im, a.im);
+    a.im = fmadd(-b.im, c.re, a.im);
 }
 
 // multiply by the phase i^PH (PH = 0: 1, 1: i, 2: -1, 3: -i)
