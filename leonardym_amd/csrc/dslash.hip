@@ -136,11 +136,25 @@ __device__ __forceinline__ void hop_dir(cplx<T> acc[12], const cplx<T> psi[12], 
     constexpr int Q0 = FWD ? 4 - D::C0 : 6 - D::C0;
     constexpr int Q1 = FWD ? 4 - D::C1 : 6 - D::C1;
     cplx<T> h0[3], h1[3], r0[3], r1[3];
+    if constexpr (SCALED) {
+        // FP16 operands: each component of h = psi_a + c psi_P as a fused
+        // multiply-add by a unit the compiler cannot see through, so that the
+        // conversions of both FP16 operands fold into one mixed-precision
+        // instruction instead of two conversions and an add.  The value is
+        // the same: the product by one is exact and the sum rounds once.
+        T one = (T)1;
+        asm volatile("" : "+v"(one));
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        h0[c] = psi[c] + mulph<PH0>(psi[3 * D::P0 + c]);
-        h1[c] = psi[3 + c] + mulph<PH1>(psi[3 * D::P1 + c]);
-        if constexpr (!SCALED) {
+        for (int c = 0; c < 3; ++c) {
+            const cplx<T> m0 = mulph<PH0>(psi[3 * D::P0 + c]), m1 = mulph<PH1>(psi[3 * D::P1 + c]);
+            h0[c] = mk(fmadd(m0.re, one, psi[c].re), fmadd(m0.im, one, psi[c].im));
+            h1[c] = mk(fmadd(m1.re, one, psi[3 + c].re), fmadd(m1.im, one, psi[3 + c].im));
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            h0[c] = psi[c] + mulph<PH0>(psi[3 * D::P0 + c]);
+            h1[c] = psi[3 + c] + mulph<PH1>(psi[3 * D::P1 + c]);
             if (flip) {
                 h0[c] = -h0[c];
                 h1[c] = -h1[c];
